@@ -1,0 +1,203 @@
+"""Benchmark: full caustic designs (OT + height) per second on one MI355X.
+
+One step = one complete design from the device-resident target image:
+initialize_solvers (min-max, mesh, target-area clipping), up to 50 transport
+(OT) steps with the reference's stop rule on the step size (main.cpp:243-256),
+then 3 height steps (main.cpp:260-262).  Every inner Poisson solve stops at
+FP64 relative residual ||P b - A phi||_2 / ||P b||_2 <= 1e-6.
+
+Workload (default): 512x512 mesh, 2048x2048 Poisson grid, seeded 3-octave
+value-noise target (1024x1024 f32, nearest-resized to 2048x2048), W = 0.5,
+focal length 1.5.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+
+Prints one JSON line.  The CPU baseline (reference code on the host cores,
+bounded sample, extrapolated with the GPU run's own step and sweep counts) runs
+in a subprocess after the GPU measurement.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "OT+height solves/sec at 512^2 @1e-6 residual; learned-init vs plain, same GPU"
+HBM_PEAK_GBS = 8000.0
+SWEEP_BYTES_PER_CELL = 12.0  # one red+black FP32 sweep: e read + e write + rhs read
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(config: str, seed: int, per_design: dict, timeout: int) -> dict | None:
+    threads = min(os.cpu_count() or 1, 16)
+    cmd = [sys.executable, "-m", "oracle.cpu_baseline", "--config", config, "--seed", str(seed),
+           "--threads", str(threads)]
+    try:
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, cwd=ROOT)
+    except subprocess.TimeoutExpired:
+        log("cpu baseline: timed out")
+        return None
+    if r.returncode != 0:
+        log("cpu baseline failed:", r.stderr[-2000:])
+        return None
+    m = json.loads(r.stdout.strip().splitlines()[-1])
+    t_design = (m["t_init"] + per_design["ot_steps"] * m["t_ot_ns"]
+                + per_design["h_steps"] * m["t_h_ns"] + per_design["sweeps"] * m["t_sweep"])
+    sample = (f"{m['kind']} code, {m['tiles']} SOR tile threads on {m['nproc']} host cpus "
+              f"({m['cpu_model']}): measured initialize_solvers ({m['t_init']:.2f}s), one transport "
+              f"step without its solve ({m['t_ot_ns']:.2f}s), one height step without its solve "
+              f"({m['t_h_ns']:.2f}s) and {m['sweeps_timed']} SOR sweeps ({m['t_sweep']*1e3:.2f} ms/"
+              f"sweep) at {m['grid']}^2; extrapolated to one design with the GPU run's per-design "
+              f"counts ({per_design['ot_steps']:.1f} transport steps, {per_design['h_steps']:.0f} "
+              f"height steps, {per_design['sweeps']:.0f} sweeps; the reference's own omega needs "
+              f"more sweeps, so this favours the CPU)")
+    return {"value": 1.0 / t_design, "unit": "solves/s", "cores": m["tiles"],
+            "kind": m["kind"], "sample": sample, "seconds_per_design": t_design,
+            "components": m}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="noise3")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--mesh-res", type=int, default=None)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-timeout", type=int, default=240)
+    a = ap.parse_args(argv)
+
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
+              "independent designs shard one per GPU with no communication, so N GPUs give "
+              "N x the one-GPU value.", file=sys.stderr)
+        return 2
+
+    import numpy as np
+
+    from poisson_caustic_design_amd import capi, synth
+
+    img, prm = synth.target_for(a.config, a.seed, a.mesh_res)
+    M, N, W, H, f = prm["mesh_res"], prm["res"], prm["width"], prm["height"], prm["focal_l"]
+    log(f"bench: config {a.config} seed {a.seed} mesh {M} grid {N}")
+
+    ctx = capi.Context(0)
+    cd = capi.CausticDesign(ctx)
+    cd.set_mesh_resolution(M, M)
+    cd.set_domain_resolution(N, N)
+    cd.set_mesh_size(W, H)
+    cd.set_lens_focal_length(f)
+    cd.set_lens_thickness(0.1)
+    prec = capi.PREC_MIXED if a.precision == "mixed" else capi.PREC_FP64
+    o = capi.default_opts(tol=a.tol, precision=prec)
+    cd.set_solve_opts(o, o)
+    cd.initialize_solvers(img)          # uploads the image; it stays resident
+    d_img = cd.device_ptr("image")
+    conv = float(np.float32(0.01))      # --conv_tres default 0.01 as float (main.cpp:151,183)
+
+    def one_design():
+        cd.reinitialize_device(d_img)
+        return cd.run(50, conv, 3)
+
+    for i in range(a.warmup):
+        rs = one_design()
+        log(f"warmup {i}: {rs.elapsed_ms:.1f} ms, {rs.transport_steps} OT steps, "
+            f"{rs.phi_sweeps}+{rs.h_sweeps} sweeps")
+
+    ctx.synchronize()
+    ctx.reset_kernel_stats()
+    ctx.set_profiling(True)
+    runs = []
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        rs = one_design()
+        runs.append(rs.as_dict())
+        log(f"step {i}: {rs.elapsed_ms:.1f} ms, {rs.transport_steps} OT steps, "
+            f"{rs.phi_sweeps}+{rs.h_sweeps} sweeps, max residual "
+            f"{max(rs.max_phi_residual, rs.max_h_residual):.2e}")
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    ctx.set_profiling(False)
+    sor_ms, sor_n, sor_bytes = ctx.kernel_stats(capi.KERNEL_SOR_SWEEP)
+    res_ms, res_n, _ = ctx.kernel_stats(capi.KERNEL_RESIDUAL)
+    ras_ms, ras_n, _ = ctx.kernel_stats(capi.KERNEL_RASTER)
+    vtx_ms, vtx_n, _ = ctx.kernel_stats(capi.KERNEL_VERTEX)
+
+    for r in runs:
+        if r["status"] != capi.OK:
+            raise SystemExit(f"design failed: status {r['status']}")
+    value = a.steps / elapsed
+    per_design = {
+        "ot_steps": sum(r["transport_steps"] for r in runs) / len(runs),
+        "h_steps": sum(r["height_steps"] for r in runs) / len(runs),
+        "sweeps": sum(r["phi_sweeps"] + r["h_sweeps"] for r in runs) / len(runs),
+    }
+    avg_launch_ms = sor_ms / max(sor_n, 1)
+    bytes_per_launch = sor_bytes / max(sor_n, 1)
+    achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9 if sor_n else 0.0
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * elapsed / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if prec == capi.PREC_MIXED else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"full caustic design (init + <=50 OT steps + 3 height steps), "
+                         f"{M}x{M} mesh, {N}x{N} Poisson grid, target '{a.config}' seed {a.seed}"),
+            "mesh": M, "grid": N, "width": W, "focal_l": f,
+            "stop": f"FP64 relative residual <= {a.tol:g} for every Poisson solve",
+            "precision": ("FP32 red-black SOR sweeps on a correction, FP64 base/residual/"
+                          "zero-mean projection/defect correction, FP64 geometry")
+            if prec == capi.PREC_MIXED else "FP64 red-black SOR sweeps, FP64 geometry",
+            "learned_init": False,
+            "per_design": per_design,
+        },
+        "roofline": {
+            "kernel": "k_rb_sweep (red-black SOR half-sweep)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "avg_launch_us": avg_launch_ms * 1e3,
+            "bytes_per_launch": bytes_per_launch,
+            "launches": sor_n,
+        },
+        "time_split_ms": {"sor": sor_ms, "residual_checks": res_ms, "raster": ras_ms,
+                          "vertex": vtx_ms, "total": elapsed * 1e3},
+    }
+    cb = None
+    if not a.no_cpu_baseline:
+        cb = cpu_baseline(a.config, a.seed, per_design, a.cpu_timeout)
+    out["cpu_baseline"] = ({k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+                           if cb else None)
+    if cb:
+        out["speedup_vs_cpu"] = value / cb["value"]
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,199 @@
+/*
+ * cd_api.h -- C ABI of the MI355X (gfx950) caustic-design hot path.
+ *
+ * Plain pointers and sizes only.  Host-pointer entry points copy in/out;
+ * *_device entry points take device pointers (HIP global memory) and leave
+ * everything resident.  All work is ordered on the context's HIP stream.
+ *
+ * Each entry point names the reference interface of
+ * dylanmsu/poisson_caustic_design it replaces (file:line).
+ *
+ * Layouts (same as the reference, flattened):
+ *   grids     row-major [y][x]: height rows of width doubles
+ *             (std::vector<std::vector<double>> grid[y][x], solver.h:8)
+ *   vertices  index = y*mesh_res_x + x (src/mesh.cpp:48-54)
+ *   points    3 doubles {x, y, z} per vertex (point_t, src/mesh.h:17)
+ *
+ * Status codes replace the reference's exit(0) on a raster miss
+ * (src/mesh.cpp:277-279, 334-336) and its silent iteration cap
+ * (src/solver.cpp:92).
+ */
+#ifndef CD_API_H
+#define CD_API_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CD_API_VERSION 1
+
+typedef enum {
+    CD_OK = 0,
+    CD_ERR_RASTER_MISS = 1,     /* a raster sample / source vertex lies in no triangle */
+    CD_ERR_FOLDED = 2,          /* a deformed mesh cell has non-positive area           */
+    CD_ERR_NOT_CONVERGED = 3,   /* sweep cap reached before the stop criterion          */
+    CD_ERR_NAN = 4,             /* NaN/Inf in a result                                  */
+    CD_ERR_BAD_ARG = 5,
+    CD_ERR_HIP = 6,             /* HIP runtime error (message via cd_context_last_error) */
+    CD_ERR_NO_DEVICE = 7,
+    CD_ERR_OOM = 8,
+    CD_ERR_STATE = 9            /* call out of order (e.g. step before init)            */
+} cd_status;
+
+typedef enum {
+    CD_PREC_MIXED = 0,  /* FP32 red-black sweeps on a correction, FP64 base, FP64
+                           residual / zero-mean projection / defect correction */
+    CD_PREC_FP64 = 1    /* FP64 red-black sweeps                                  */
+} cd_precision;
+
+typedef enum {
+    CD_STOP_RESIDUAL = 0,   /* ||P b - A phi||_2 / ||P b||_2 <= tol (FP64)             */
+    CD_STOP_MAX_UPDATE = 1  /* max |delta| over one sweep < tol  (solver.cpp:142)       */
+} cd_stop_rule;
+
+typedef struct {
+    int max_sweeps;       /* sweep cap (reference max_iterations, solver.cpp:92)        */
+    int stop;             /* cd_stop_rule                                               */
+    double tol;           /* tolerance of the stop rule                                 */
+    int precision;        /* cd_precision                                               */
+    int check_every;      /* sweeps between FP64 residual checks / flushes; 0 = auto    */
+    int learned_init;     /* 1: learned initial correction (needs cd_context_set_net)   */
+    double omega;         /* > 0: as given; < 0: the reference's 2/(1+3.14159265/width)
+                             (solver.cpp:71); 0: the optimum for this operator,
+                             2/(1+sqrt(1-rho^2)), rho = 1-sin^2(pi/(2 max(w,h)))     */
+} cd_solve_opts;
+
+typedef struct {
+    int status;
+    int sweeps;                 /* device-counted red+black sweeps                   */
+    int checks;                 /* FP64 residual evaluations                         */
+    int learned_applied;        /* learned corrections accepted                      */
+    int learned_rejected;       /* learned corrections rejected (residual rose)      */
+    double rel_residual;        /* final ||P b - A phi||_2/||P b||_2 (FP64)          */
+    double initial_rel_residual;
+    double last_max_update;     /* max |delta| of the last sweep                     */
+    double elapsed_ms;
+} cd_solve_stats;
+
+typedef struct cd_context cd_context;
+typedef struct cd_design cd_design;
+
+/* ---- context -------------------------------------------------------- */
+void cd_solve_opts_default(cd_solve_opts* o);  /* residual 1e-6, mixed, cap 100000, omega 0 */
+int cd_context_create(int device, cd_context** out);
+void cd_context_destroy(cd_context* ctx);
+int cd_context_synchronize(cd_context* ctx);
+const char* cd_context_last_error(const cd_context* ctx);
+const char* cd_status_string(int status);
+int cd_get_api_version(void);
+/* Kernel timing over HIP events on the context stream (off by default).
+ * kernel ids: cd_kernel_id */
+typedef enum {
+    CD_KERNEL_SOR_SWEEP = 0,    /* the red-black SOR sweep kernel(s)                 */
+    CD_KERNEL_RESIDUAL = 1,     /* FP64 residual + correction flush                  */
+    CD_KERNEL_RASTER = 2,       /* mesh rasterization                                */
+    CD_KERNEL_VERTEX = 3,       /* per-vertex kernels                                */
+    CD_KERNEL_COUNT = 4
+} cd_kernel_id;
+int cd_context_set_profiling(cd_context* ctx, int enable);
+/* total elapsed ms, number of launches, algorithmic bytes moved by those launches */
+int cd_context_kernel_stats(cd_context* ctx, int kernel, double* total_ms, long long* launches,
+                            double* bytes);
+int cd_context_reset_kernel_stats(cd_context* ctx);
+
+/* ---- Poisson solve ---------------------------------------------------
+ * Replaces  void poisson_solver(std::vector<std::vector<double>>& D,
+ *           std::vector<std::vector<double>>& phi, int width, int height,
+ *           int max_iterations, double convergence_threshold, int max_threads)
+ * (src/solver.h:8, src/solver.cpp:70-147).  rhs is D ([height][width]); NaN
+ * cells are holes excluded from their neighbours' stencils (solver.cpp:29-44)
+ * and come back NaN in phi.  phi is the warm start and the result.  The
+ * right-hand side is projected to zero mean over non-hole cells (the Neumann
+ * compatibility condition) before sweeping. */
+int cd_poisson_solve(cd_context* ctx, const double* rhs, double* phi, int width, int height,
+                     const cd_solve_opts* opts, cd_solve_stats* stats);
+int cd_poisson_solve_device(cd_context* ctx, const double* d_rhs, double* d_phi, int width,
+                            int height, const cd_solve_opts* opts, cd_solve_stats* stats);
+
+/* ---- design (class Caustic_design, src/caustic_design.h:7-66) ----------- */
+typedef struct {
+    int mesh_res_x, mesh_res_y;   /* set_mesh_resolution   (caustic_design.cpp:28-31) */
+    int res_x, res_y;             /* set_domain_resolution (caustic_design.cpp:33-36) */
+    double width, height;         /* set_mesh_size         (caustic_design.cpp:38-41) */
+    double focal_l;               /* set_lens_focal_length (caustic_design.cpp:43-45) */
+    double thickness;             /* set_lens_thickness    (caustic_design.cpp:47-49) */
+    double ior;                   /* refractive index, 1.49 (caustic_design.cpp:271)  */
+    double step_size;             /* OT step, (double)0.05f (caustic_design.cpp:250)  */
+    cd_solve_opts phi_solve;      /* transport potential solve (caustic_design.cpp:222) */
+    cd_solve_opts h_solve;        /* height solve               (caustic_design.cpp:311) */
+} cd_design_params;
+
+/* Defaults reproduce the reference: ior 1.49, step (double)0.05f, phi solve
+ * max-update 1e-7, h solve max-update 1e-8, 100000-sweep caps, the
+ * reference's omega (solve opts omega = -1). */
+void cd_design_params_default(cd_design_params* p);
+
+int cd_design_create(cd_context* ctx, const cd_design_params* p, cd_design** out);
+void cd_design_destroy(cd_design* d);
+int cd_design_set_solve_opts(cd_design* d, const cd_solve_opts* phi, const cd_solve_opts* h);
+/* initialize_solvers(image) (caustic_design.cpp:334-364): image is res_y x res_x */
+int cd_design_init(cd_design* d, const double* image);
+int cd_design_init_device(cd_design* d, const double* d_image);
+/* perform_transport_iteration() (caustic_design.cpp:190-266); *step_out is
+ * max vertex displacement / width */
+int cd_design_transport_step(cd_design* d, double* step_out);
+/* perform_height_map_iteration(itr) (caustic_design.cpp:269-332); *update_out
+ * is the sum of squared vertex-height updates (mesh.cpp:724-742) */
+int cd_design_height_step(cd_design* d, double* update_out);
+
+typedef struct {
+    int status;
+    int transport_steps;          /* OT steps performed                              */
+    int height_steps;
+    double last_step;             /* last transport step size                        */
+    long long phi_sweeps, h_sweeps;
+    long long checks;
+    int learned_applied, learned_rejected;
+    double max_phi_residual, max_h_residual;
+    double elapsed_ms;
+} cd_run_stats;
+
+/* The reference driver loop (main.cpp:243-262): up to max_transport_steps
+ * OT steps, stopping after the first step whose size is < conv_tres, then
+ * height_steps height steps.  Everything stays on the device. */
+int cd_design_run(cd_design* d, int max_transport_steps, double conv_tres, int height_steps,
+                  cd_run_stats* stats);
+
+typedef enum {
+    CD_FIELD_TARGET_POINTS = 0,   /* V x 3  mesh->target_points            */
+    CD_FIELD_SOURCE_POINTS = 1,   /* V x 3  mesh->source_points            */
+    CD_FIELD_PHI = 2,             /* N     phi                             */
+    CD_FIELD_H = 3,               /* N     h                               */
+    CD_FIELD_RASTER = 4,          /* N     raster (after zero-mean)        */
+    CD_FIELD_ERRORS = 5,          /* V     errors                          */
+    CD_FIELD_TARGET_AREAS = 6,    /* V     target_areas                    */
+    CD_FIELD_DIVERGENCE = 7,      /* N     divergence (after zero-mean)    */
+    CD_FIELD_NORM_X = 8,          /* N     norm_x                          */
+    CD_FIELD_NORM_Y = 9,          /* N     norm_y                          */
+    CD_FIELD_NORMALS = 10,        /* V x 2 normals[0], normals[1]          */
+    CD_FIELD_VERTEX_GRADIENT = 11,/* V x 2 vertex_gradient                 */
+    CD_FIELD_PIXELS = 12,         /* N     pixels (min-max scaled image)   */
+    CD_FIELD_INVERSE_MAP = 13,    /* V x 2 inverted transport map          */
+    CD_FIELD_IMAGE = 14,          /* N     input image as last initialised */
+    CD_FIELD_COUNT = 15
+} cd_field;
+
+/* number of doubles of a field (N = res_x*res_y grid, V = vertices) */
+size_t cd_design_field_size(const cd_design* d, int field);
+int cd_design_get(cd_design* d, int field, double* host_out, size_t count);
+int cd_design_set(cd_design* d, int field, const double* host_in, size_t count);
+/* device pointer of a field's storage (NULL for composite fields) */
+const void* cd_design_device_ptr(cd_design* d, int field);
+int cd_design_last_solve_stats(const cd_design* d, cd_solve_stats* phi, cd_solve_stats* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CD_API_H */

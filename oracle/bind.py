@@ -290,6 +290,7 @@ class RefLib:
         L.ref_design_create.argtypes = [_I, _I, _I, _I, _D, _D, _D, _D, _I]
         L.ref_design_create.restype = _P
         L.ref_design_destroy.argtypes = [_P]
+        L.ref_design_set_threads.argtypes = [_P, _I]
         L.ref_design_init.argtypes = [_P, _P]
         L.ref_transport_step.argtypes = [_P]
         L.ref_transport_step.restype = _D
@@ -365,6 +366,9 @@ class RefDesign:
         image = _f64(image)
         assert image.shape == (self.ny, self.nx)
         self.L.ref_design_init(self.h, _ptr(image))
+
+    def set_threads(self, n):
+        self.L.ref_design_set_threads(self.h, n)
 
     def transport_step(self):
         return self.L.ref_transport_step(self.h)

@@ -132,6 +132,13 @@ void* ref_design_create(int mesh_x, int mesh_y, int res_x, int res_y, double wid
     return cd;
 }
 
+// nthreads == 0 makes poisson_solver run zero tiles and return after one
+// empty iteration (solver.cpp:73-83, 142): the step then costs everything but
+// the Poisson solve.
+void ref_design_set_threads(void* h, int threads) {
+    static_cast<Caustic_design*>(h)->set_solver_max_threads(threads);
+}
+
 void ref_design_destroy(void* h) {
     auto* cd = static_cast<Caustic_design*>(h);
     delete cd->mesh;

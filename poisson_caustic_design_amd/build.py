@@ -1,0 +1,78 @@
+"""Build libcaustic_amd.so (gfx950) in-tree with hipcc.
+
+    python -m poisson_caustic_design_amd.build
+
+Every HIP translation unit is compiled with
+  --offload-arch=gfx950 -O3 -ffp-contract=off
+  -Xclang -target-feature -Xclang -packed-fp32-ops
+FP contraction is off so the FP64 geometry kernels round exactly like the
+FP64 CPU restatement used by the tests.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+BUILD = os.path.join(ROOT, "_build")
+LIB = os.path.join(PKG, "libcaustic_amd.so")
+
+HIP_SOURCES = ["cd_context.hip", "cd_solver.hip", "cd_design.hip", "cd_learned.hip"]
+CXX_SOURCES = ["dropin_solver.cpp", "dropin_caustic_design.cpp"]
+
+ARCH = os.environ.get("CD_OFFLOAD_ARCH", "gfx950")
+
+
+def _hipcc() -> str:
+    for c in (shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+    return r
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    hipcc = _hipcc()
+    common = ["-O3", "-fPIC", "-std=c++17", "-I" + INCLUDE, "-I" + CSRC, "-Wall",
+              "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-result"]
+    hip_flags = ["-x", "hip", "--offload-arch=" + ARCH, "-ffp-contract=off",
+                 "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                 "-munsafe-fp-atomics"]
+    jobs = []
+    for src in HIP_SOURCES:
+        path = os.path.join(CSRC, src)
+        if not os.path.exists(path):
+            continue
+        obj = os.path.join(BUILD, src + ".o")
+        jobs.append([hipcc, *common, *hip_flags, "-c", path, "-o", obj])
+    for src in CXX_SOURCES:
+        obj = os.path.join(BUILD, src + ".o")
+        jobs.append(["g++", *common, "-c", os.path.join(CSRC, src), "-o", obj])
+    with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        results = list(ex.map(_run, jobs))
+    if verbose:
+        for r in results:
+            sys.stdout.write(r.stdout + r.stderr)
+    objs = [j[-1] for j in jobs]
+    tmp = LIB + ".tmp"
+    _run([hipcc, "-shared", "-fPIC", "--offload-arch=" + ARCH, *objs, "-o", tmp,
+          "-Wl,-soname,libcaustic_amd.so"])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,379 @@
+"""ctypes binding of libcaustic_amd.so (the C ABI declared in include/cd_api.h).
+
+Python mirror of the reference's C++ interfaces for this path:
+
+* ``poisson_solver(D, phi, width, height, max_iterations, threshold, max_threads)``
+  mirrors ``poisson_solver`` (src/solver.h:8): D and phi are [height][width]
+  arrays, phi is updated in place, the stop rule is the reference's
+  max-update threshold.
+* ``CausticDesign`` mirrors ``class Caustic_design`` (src/caustic_design.h:7-66):
+  the same setters, ``initialize_solvers``, ``perform_transport_iteration`` and
+  ``perform_height_map_iteration``, and the same public state as properties.
+
+There is no CPU fallback: loading fails loudly when the library is missing,
+and every call fails with ``CDError`` when the HIP device is unavailable.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcaustic_amd.so")
+
+_D, _I, _P, _SZ, _LL = C.c_double, C.c_int, C.c_void_p, C.c_size_t, C.c_longlong
+
+OK, RASTER_MISS, FOLDED, NOT_CONVERGED, NAN_, BAD_ARG, HIP_ERROR, NO_DEVICE, OOM, STATE = range(10)
+PREC_MIXED, PREC_FP64 = 0, 1
+STOP_RESIDUAL, STOP_MAX_UPDATE = 0, 1
+KERNEL_SOR_SWEEP, KERNEL_RESIDUAL, KERNEL_RASTER, KERNEL_VERTEX = 0, 1, 2, 3
+
+FIELD = {
+    "target_points": 0, "source_points": 1, "phi": 2, "h": 3, "raster": 4, "errors": 5,
+    "target_areas": 6, "divergence": 7, "norm_x": 8, "norm_y": 9, "normals": 10,
+    "vertex_gradient": 11, "pixels": 12, "inverse_map": 13, "image": 14,
+}
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("max_sweeps", _I), ("stop", _I), ("tol", _D), ("precision", _I),
+                ("check_every", _I), ("learned_init", _I), ("omega", _D)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("status", _I), ("sweeps", _I), ("checks", _I), ("learned_applied", _I),
+                ("learned_rejected", _I), ("rel_residual", _D), ("initial_rel_residual", _D),
+                ("last_max_update", _D), ("elapsed_ms", _D)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class DesignParams(C.Structure):
+    _fields_ = [("mesh_res_x", _I), ("mesh_res_y", _I), ("res_x", _I), ("res_y", _I),
+                ("width", _D), ("height", _D), ("focal_l", _D), ("thickness", _D),
+                ("ior", _D), ("step_size", _D), ("phi_solve", SolveOpts), ("h_solve", SolveOpts)]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("status", _I), ("transport_steps", _I), ("height_steps", _I),
+                ("last_step", _D), ("phi_sweeps", _LL), ("h_sweeps", _LL), ("checks", _LL),
+                ("learned_applied", _I), ("learned_rejected", _I),
+                ("max_phi_residual", _D), ("max_h_residual", _D), ("elapsed_ms", _D)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class CDError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(msg)
+        self.status = status
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load libcaustic_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} not built: run `python -m poisson_caustic_design_amd.build`")
+    L = C.CDLL(path)
+    sig = {
+        "cd_get_api_version": ([], _I),
+        "cd_status_string": ([_I], C.c_char_p),
+        "cd_solve_opts_default": ([C.POINTER(SolveOpts)], None),
+        "cd_context_create": ([_I, C.POINTER(_P)], _I),
+        "cd_context_destroy": ([_P], None),
+        "cd_context_synchronize": ([_P], _I),
+        "cd_context_last_error": ([_P], C.c_char_p),
+        "cd_context_set_profiling": ([_P, _I], _I),
+        "cd_context_kernel_stats": ([_P, _I, C.POINTER(_D), C.POINTER(_LL), C.POINTER(_D)], _I),
+        "cd_context_reset_kernel_stats": ([_P], _I),
+        "cd_poisson_solve": ([_P, _P, _P, _I, _I, C.POINTER(SolveOpts), C.POINTER(SolveStats)], _I),
+        "cd_poisson_solve_device": ([_P, _P, _P, _I, _I, C.POINTER(SolveOpts),
+                                     C.POINTER(SolveStats)], _I),
+        "cd_design_params_default": ([C.POINTER(DesignParams)], None),
+        "cd_design_create": ([_P, C.POINTER(DesignParams), C.POINTER(_P)], _I),
+        "cd_design_destroy": ([_P], None),
+        "cd_design_set_solve_opts": ([_P, C.POINTER(SolveOpts), C.POINTER(SolveOpts)], _I),
+        "cd_design_init": ([_P, _P], _I),
+        "cd_design_init_device": ([_P, _P], _I),
+        "cd_design_transport_step": ([_P, C.POINTER(_D)], _I),
+        "cd_design_height_step": ([_P, C.POINTER(_D)], _I),
+        "cd_design_run": ([_P, _I, _D, _I, C.POINTER(RunStats)], _I),
+        "cd_design_field_size": ([_P, _I], _SZ),
+        "cd_design_get": ([_P, _I, _P, _SZ], _I),
+        "cd_design_set": ([_P, _I, _P, _SZ], _I),
+        "cd_design_device_ptr": ([_P, _I], _P),
+        "cd_design_last_solve_stats": ([_P, C.POINTER(SolveStats), C.POINTER(SolveStats)], _I),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    return [
+        "cd_get_api_version", "cd_status_string", "cd_solve_opts_default", "cd_context_create",
+        "cd_context_destroy", "cd_context_synchronize", "cd_context_last_error",
+        "cd_context_set_profiling", "cd_context_kernel_stats", "cd_context_reset_kernel_stats",
+        "cd_poisson_solve", "cd_poisson_solve_device", "cd_design_params_default",
+        "cd_design_create", "cd_design_destroy", "cd_design_set_solve_opts", "cd_design_init",
+        "cd_design_init_device", "cd_design_transport_step", "cd_design_height_step",
+        "cd_design_run", "cd_design_field_size", "cd_design_get", "cd_design_set",
+        "cd_design_device_ptr", "cd_design_last_solve_stats",
+    ]
+
+
+def status_string(s: int) -> str:
+    return load().cd_status_string(s).decode()
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def default_opts(**kw) -> SolveOpts:
+    o = SolveOpts()
+    load().cd_solve_opts_default(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Context:
+    """One HIP device + stream (cd_context)."""
+
+    def __init__(self, device: int = 0):
+        L = load()
+        h = _P()
+        rc = L.cd_context_create(device, C.byref(h))
+        if rc != OK:
+            raise CDError(rc, f"cd_context_create: {status_string(rc)}")
+        self.L, self.h = L, h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cd_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc: int, what: str, allow=(OK,)):
+        if rc not in allow:
+            msg = self.L.cd_context_last_error(self.h).decode()
+            raise CDError(rc, f"{what}: {status_string(rc)}: {msg}")
+        return rc
+
+    def synchronize(self):
+        self.check(self.L.cd_context_synchronize(self.h), "synchronize")
+
+    def set_profiling(self, on: bool):
+        self.check(self.L.cd_context_set_profiling(self.h, int(on)), "set_profiling")
+
+    def reset_kernel_stats(self):
+        self.check(self.L.cd_context_reset_kernel_stats(self.h), "reset_kernel_stats")
+
+    def kernel_stats(self, kernel: int):
+        ms, n, b = _D(), _LL(), _D()
+        self.check(self.L.cd_context_kernel_stats(self.h, kernel, C.byref(ms), C.byref(n),
+                                                  C.byref(b)), "kernel_stats")
+        return ms.value, n.value, b.value
+
+    def poisson_solve(self, rhs, phi, opts: Optional[SolveOpts] = None, allow=(OK, NOT_CONVERGED)):
+        """Solve in place on host arrays ([height][width]); returns (phi, stats)."""
+        rhs = _f64(rhs)
+        phi = _f64(phi).copy()
+        h, w = rhs.shape
+        if phi.shape != rhs.shape:
+            raise ValueError("phi and rhs shapes differ")
+        st = SolveStats()
+        rc = self.L.cd_poisson_solve(self.h, rhs.ctypes.data, phi.ctypes.data, w, h,
+                                     C.byref(opts or default_opts()), C.byref(st))
+        self.check(rc, "cd_poisson_solve", allow)
+        return phi, st
+
+    def poisson_solve_device(self, d_rhs: int, d_phi: int, width: int, height: int,
+                             opts: Optional[SolveOpts] = None, allow=(OK, NOT_CONVERGED)):
+        st = SolveStats()
+        rc = self.L.cd_poisson_solve_device(self.h, d_rhs, d_phi, width, height,
+                                            C.byref(opts or default_opts()), C.byref(st))
+        self.check(rc, "cd_poisson_solve_device", allow)
+        return st
+
+
+_shared_ctx = None
+
+
+def poisson_solver(D, phi, width, height, max_iterations, convergence_threshold, max_threads):
+    """Mirror of poisson_solver (src/solver.h:8): phi (numpy [height][width]) updated in place."""
+    global _shared_ctx
+    if max_threads <= 0 or max_iterations <= 0:  # zero tiles: no sweep (solver.cpp:73-83)
+        return None
+    if _shared_ctx is None:
+        _shared_ctx = Context(int(os.environ.get("CD_DEVICE", "0")))
+    D = _f64(D)[:height, :width]
+    o = default_opts(max_sweeps=max_iterations, stop=STOP_MAX_UPDATE, tol=convergence_threshold,
+                     omega=-1.0)
+    out, st = _shared_ctx.poisson_solve(D, np.asarray(phi)[:height, :width], o,
+                                        allow=(OK, NOT_CONVERGED, NAN_))
+    phi[:height, :width] = out
+    return st
+
+
+class CausticDesign:
+    """Mirror of class Caustic_design (src/caustic_design.h:7-66) on the device."""
+
+    def __init__(self, ctx: Optional[Context] = None):
+        self.ctx = ctx or Context(int(os.environ.get("CD_DEVICE", "0")))
+        self.L = self.ctx.L
+        self.mesh_res_x = self.mesh_res_y = 0
+        self.resolution_x = self.resolution_y = 0
+        self.width = self.height = 0.0
+        self.focal_l = 0.0
+        self.thickness = 0.0
+        self.nthreads = 0
+        self.phi_opts: Optional[SolveOpts] = None
+        self.h_opts: Optional[SolveOpts] = None
+        self.d = None
+
+    # setters of caustic_design.cpp:28-53
+    def set_mesh_resolution(self, w, h): self.mesh_res_x, self.mesh_res_y = int(w), int(h)
+    def set_domain_resolution(self, w, h): self.resolution_x, self.resolution_y = int(w), int(h)
+    def set_mesh_size(self, w, h): self.width, self.height = float(w), float(h)
+    def set_lens_focal_length(self, f): self.focal_l = float(f)
+    def set_lens_thickness(self, t): self.thickness = float(t)
+    def set_solver_max_threads(self, n): self.nthreads = int(n)
+
+    def set_solve_opts(self, phi: Optional[SolveOpts] = None, h: Optional[SolveOpts] = None):
+        self.phi_opts = phi or self.phi_opts
+        self.h_opts = h or self.h_opts
+        if self.d is not None:
+            self.ctx.check(self.L.cd_design_set_solve_opts(
+                self.d, C.byref(self.phi_opts) if self.phi_opts else None,
+                C.byref(self.h_opts) if self.h_opts else None), "set_solve_opts")
+
+    def _params(self) -> DesignParams:
+        p = DesignParams()
+        self.L.cd_design_params_default(C.byref(p))
+        p.mesh_res_x, p.mesh_res_y = self.mesh_res_x, self.mesh_res_y
+        p.res_x, p.res_y = self.resolution_x, self.resolution_y
+        p.width, p.height = self.width, self.height
+        p.focal_l, p.thickness = self.focal_l, self.thickness
+        if self.phi_opts is not None:
+            p.phi_solve = self.phi_opts
+        if self.h_opts is not None:
+            p.h_solve = self.h_opts
+        return p
+
+    def create(self):
+        if self.d is not None:
+            self.L.cd_design_destroy(self.d)
+        h = _P()
+        p = self._params()
+        self.ctx.check(self.L.cd_design_create(self.ctx.h, C.byref(p), C.byref(h)),
+                       "cd_design_create")
+        self.d = h
+
+    def close(self):
+        if getattr(self, "d", None):
+            self.L.cd_design_destroy(self.d)
+            self.d = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def initialize_solvers(self, image):
+        """initialize_solvers (caustic_design.cpp:334-364); image [res_y][res_x]."""
+        image = _f64(image)
+        if image.shape != (self.resolution_y, self.resolution_x):
+            raise ValueError(f"image shape {image.shape} != {(self.resolution_y, self.resolution_x)}")
+        self.create()
+        self.ctx.check(self.L.cd_design_init(self.d, image.ctypes.data), "initialize_solvers")
+
+    def initialize_solvers_device(self, d_image: int):
+        self.create()
+        self.ctx.check(self.L.cd_design_init_device(self.d, d_image), "initialize_solvers")
+
+    def reinitialize_device(self, d_image: int):
+        """Re-run initialisation on an existing design (device-resident image)."""
+        self.ctx.check(self.L.cd_design_init_device(self.d, d_image), "initialize_solvers")
+
+    def perform_transport_iteration(self) -> float:
+        s = _D()
+        self.ctx.check(self.L.cd_design_transport_step(self.d, C.byref(s)),
+                       "perform_transport_iteration")
+        return s.value
+
+    def perform_height_map_iteration(self, itr: int = 0) -> float:
+        u = _D()
+        self.ctx.check(self.L.cd_design_height_step(self.d, C.byref(u)),
+                       "perform_height_map_iteration")
+        return u.value
+
+    def run(self, max_transport_steps=50, conv_tres=float(np.float32(0.01)), height_steps=3,
+            allow=(OK,)):
+        rs = RunStats()
+        rc = self.L.cd_design_run(self.d, max_transport_steps, conv_tres, height_steps,
+                                  C.byref(rs))
+        self.ctx.check(rc, "cd_design_run", allow)
+        return rs
+
+    def get(self, name: str) -> np.ndarray:
+        f = FIELD[name]
+        n = self.L.cd_design_field_size(self.d, f)
+        out = np.empty(n)
+        self.ctx.check(self.L.cd_design_get(self.d, f, out.ctypes.data, n), f"get {name}")
+        V = self.mesh_res_x * self.mesh_res_y
+        if name in ("target_points", "source_points"):
+            return out.reshape(V, 3)
+        if name in ("vertex_gradient", "inverse_map"):
+            return out.reshape(V, 2)
+        if name == "normals":
+            return out.reshape(2, V)
+        if n == self.resolution_x * self.resolution_y and name not in ("errors", "target_areas"):
+            return out.reshape(self.resolution_y, self.resolution_x)
+        return out
+
+    def set(self, name: str, values):
+        f = FIELD[name]
+        v = _f64(values).ravel()
+        self.ctx.check(self.L.cd_design_set(self.d, f, v.ctypes.data, v.size), f"set {name}")
+
+    def device_ptr(self, name: str) -> int:
+        return self.L.cd_design_device_ptr(self.d, FIELD[name])
+
+    def last_solve_stats(self):
+        a, b = SolveStats(), SolveStats()
+        self.ctx.check(self.L.cd_design_last_solve_stats(self.d, C.byref(a), C.byref(b)), "stats")
+        return a, b
+
+    # public members of the reference class
+    phi = property(lambda self: self.get("phi"))
+    h = property(lambda self: self.get("h"))
+    raster = property(lambda self: self.get("raster"))
+    errors = property(lambda self: self.get("errors"))
+    target_areas = property(lambda self: self.get("target_areas"))
+    divergence = property(lambda self: self.get("divergence"))
+    norm_x = property(lambda self: self.get("norm_x"))
+    norm_y = property(lambda self: self.get("norm_y"))
+    pixels = property(lambda self: self.get("pixels"))
+    normals = property(lambda self: self.get("normals"))
+    vertex_gradient = property(lambda self: self.get("vertex_gradient").T.copy())

@@ -1,0 +1,1003 @@
+// Caustic_design on the device: the transport (OT) step, the height step and
+// the initialisation of src/caustic_design.cpp:190-364, as gfx950 kernels over
+// device-resident state.  All geometry is FP64 (single pass, bandwidth-
+// trivial); the Poisson solves use cd_solve_device_internal.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstring>
+
+#include "cd_geometry.h"
+#include "cd_internal.h"
+
+using namespace cdgeo;
+
+namespace {
+
+enum ReduceOp { R_SUM = 0, R_MIN = 1, R_MAX = 2 };
+
+inline int div_up(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+// ---------------------------------------------------------------------------
+// generic deterministic finish of per-block partials [block][ncomp]
+// ---------------------------------------------------------------------------
+__global__ void k_finish(const double* __restrict__ partials, int nblocks, int ncomp, int op0,
+                         int op1, int op2, double* __restrict__ out) {
+    __shared__ double smem[16];
+    const int ops[3] = {op0, op1, op2};
+    for (int c = 0; c < ncomp; ++c) {
+        const int op = ops[c];
+        double acc = op == R_SUM ? 0.0 : (op == R_MIN ? INFINITY : -INFINITY);
+        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+            const double v = partials[(size_t)b * ncomp + c];
+            acc = op == R_SUM ? acc + v : (op == R_MIN ? fmin(acc, v) : fmax(acc, v));
+        }
+        if (op == R_SUM) {
+            acc = cd_block_sum(acc, smem);
+        } else {
+            acc = op == R_MIN ? cd_wave_min(acc) : cd_wave_max_d(acc);
+            if ((threadIdx.x & 63) == 0) smem[threadIdx.x / 64] = acc;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                for (int k = 1; k < (int)blockDim.x / 64; ++k)
+                    acc = op == R_MIN ? fmin(acc, smem[k]) : fmax(acc, smem[k]);
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[c] = acc;
+    }
+}
+
+__device__ __forceinline__ double block_min(double v, double* smem) {
+    v = cd_wave_min(v);
+    const int tid = threadIdx.x;
+    if ((tid & 63) == 0) smem[tid / 64] = v;
+    __syncthreads();
+    double r = smem[0];
+    for (int k = 1; k < (int)blockDim.x / 64; ++k) r = fmin(r, smem[k]);
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ double block_max(double v, double* smem) {
+    v = cd_wave_max_d(v);
+    const int tid = threadIdx.x;
+    if ((tid & 63) == 0) smem[tid / 64] = v;
+    __syncthreads();
+    double r = smem[0];
+    for (int k = 1; k < (int)blockDim.x / 64; ++k) r = fmax(r, smem[k]);
+    __syncthreads();
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// initialisation (caustic_design.cpp:334-364)
+// ---------------------------------------------------------------------------
+// min/max over non-NaN pixels (utils.cpp:104-115)
+__global__ __launch_bounds__(256) void k_minmax(const double* __restrict__ img, size_t n,
+                                                double* __restrict__ partials) {
+    __shared__ double smem[16];
+    double mn = INFINITY, mx = -INFINITY;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const double v = img[i];
+        if (!isnan(v)) { mn = fmin(mn, v); mx = fmax(mx, v); }
+    }
+    mn = block_min(mn, smem);
+    mx = block_max(mx, smem);
+    if (threadIdx.x == 0) { partials[2 * blockIdx.x] = mn; partials[2 * blockIdx.x + 1] = mx; }
+}
+
+// scale to [0,1] (utils.cpp:117-126); NaN pixels become 0 as the reference's
+// default-initialised output; a NaN first pixel seeds min/max with NaN there.
+__global__ __launch_bounds__(256) void k_scale(const double* __restrict__ img,
+                                               double* __restrict__ out, size_t n,
+                                               const double* __restrict__ mm) {
+    double mn = mm[0], mx = mm[1];
+    if (isnan(img[0])) { mn = NAN; mx = NAN; }
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const double v = img[i];
+        out[i] = isnan(v) ? 0.0 : 0.0 + (1.0 - 0.0) * (v - mn) / (mx - mn);
+    }
+}
+
+// regular mesh (mesh.cpp:45-54), source heights zero
+__global__ __launch_bounds__(256) void k_mesh_init(double* __restrict__ txy,
+                                                   double* __restrict__ sxy,
+                                                   double* __restrict__ sz, int mx, int my,
+                                                   double W, double H) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= mx * my) return;
+    const int y = i / mx, x = i - y * mx;
+    const double px = (double)x * W / (mx - 1), py = (double)y * H / (my - 1);
+    txy[2 * i] = px;
+    txy[2 * i + 1] = py;
+    sxy[2 * i] = px;
+    sxy[2 * i + 1] = py;
+    sz[i] = 0.0;
+}
+
+// image integral over quad slot k of vertex v (polygon_utils.cpp:311-366):
+// one thread per (vertex, adjacent-triangle slot)
+__global__ __launch_bounds__(256) void k_quad_integrals(const double* __restrict__ pixels,
+                                                        int nx, int ny, double W,
+                                                        const double* __restrict__ xy, int mx,
+                                                        int my, double* __restrict__ quadacc) {
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long V = (long long)mx * my;
+    if (gid >= 6 * V) return;
+    const int v = (int)(gid / 6), slot = (int)(gid - 6LL * v);
+    const int y = v / mx, x = v - y * mx;
+    int adj[6];
+    const int n = adjacent_triangles(mx, my, x, y, adj);
+    double acc = 0.0;
+    if (slot < n) {
+        double q[8];
+        triangle_quad(mx, xy, v, adj[slot], q);
+        double xmin = q[0], xmax = q[0], ymin = q[1], ymax = q[1];
+        for (int i = 1; i < 4; ++i) {
+            xmin = fmin(xmin, q[2 * i]); xmax = fmax(xmax, q[2 * i]);
+            ymin = fmin(ymin, q[2 * i + 1]); ymax = fmax(ymax, q[2 * i + 1]);
+        }
+        const double px = W / (double)nx;
+        const double ylo = fmax(floor(ymin / px), 0.0), yhi = fmin(ceil(ymax / px), (double)ny);
+        const double xlo = fmax(floor(xmin / px), 0.0), xhi = fmin(ceil(xmax / px), (double)nx);
+        for (int py = (int)ylo; py < yhi; ++py)
+            for (int pxi = (int)xlo; pxi < xhi; ++pxi) {
+                const double cx = ((double)pxi + 0.5) * px, cy = ((double)py + 0.5) * px;
+                acc += quad_pixel_area(q, cx, cy, px) * pixels[(size_t)py * nx + pxi];
+            }
+    }
+    quadacc[gid] = acc;
+}
+
+// per-vertex sum of the quad integrals in slot order + block partial sums
+__global__ __launch_bounds__(256) void k_cell_sums(const double* __restrict__ quadacc, int V,
+                                                   double* __restrict__ out,
+                                                   double* __restrict__ partials) {
+    __shared__ double smem[16];
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    double a = 0.0;
+    if (v < V) {
+        for (int k = 0; k < 6; ++k) a += quadacc[6 * (size_t)v + k];
+        out[v] = a;
+    }
+    const double s = cd_block_sum(v < V ? a : 0.0, smem);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// normalise so the target areas sum to W*H (polygon_utils.cpp:382-386)
+__global__ __launch_bounds__(256) void k_scale_areas(double* __restrict__ a, int V, double WH,
+                                                     const double* __restrict__ total) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < V) a[v] *= WH / total[0];
+}
+
+// ---------------------------------------------------------------------------
+// transport step (caustic_design.cpp:190-266)
+// ---------------------------------------------------------------------------
+// per-vertex partitioned dual-cell area and relative mismatch
+// (mesh.cpp:121-231, polygon_utils.cpp:216-236, caustic_design.cpp:199-209)
+__global__ __launch_bounds__(256) void k_cell_errors(const double* __restrict__ xy,
+                                                     const double* __restrict__ tareas, int mx,
+                                                     int my, double* __restrict__ area,
+                                                     double* __restrict__ errors,
+                                                     double* __restrict__ partials) {
+    __shared__ double smem[16];
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    double folded = 0.0;
+    if (v < mx * my) {
+        const int y = v / mx, x = v - y * mx;
+        int adj[6];
+        const int n = adjacent_triangles(mx, my, x, y, adj);
+        double a = 0.0;
+        for (int k = 0; k < n; ++k) {
+            double q[8];
+            triangle_quad(mx, xy, v, adj[k], q);
+            a += shoelace4(q);
+        }
+        area[v] = a;
+        errors[v] = (tareas[v] - a) / a;
+        if (!(a > 0.0)) folded = 1.0;
+    }
+    const double s = cd_block_sum(folded, smem);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// Rasterization, pass 1: every triangle claims the samples it contains; a
+// sample keeps the lowest claiming triangle index.  Samples sit at the
+// node-centred points x_j = j*(W-eps)/(nx-1) + eps/2 (mesh.cpp:241-254).
+__global__ __launch_bounds__(256) void k_raster_scatter(const double* __restrict__ xy, int mx,
+                                                        int my, int nx, int ny, double W,
+                                                        double H, int* __restrict__ owner) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * (mx - 1) * (my - 1)) return;
+    int a, b, c;
+    tri_vertices(mx, t, a, b, c);
+    const double ax = xy[2 * a], ay = xy[2 * a + 1];
+    const double bx = xy[2 * b], by = xy[2 * b + 1];
+    const double cx = xy[2 * c], cy = xy[2 * c + 1];
+    const double xmin = fmin(ax, fmin(bx, cx)), xmax = fmax(ax, fmax(bx, cx));
+    const double ymin = fmin(ay, fmin(by, cy)), ymax = fmax(ay, fmax(by, cy));
+    if (!(isfinite(xmin) && isfinite(xmax) && isfinite(ymin) && isfinite(ymax))) return;
+    const double eps = 1e-8;
+    const double sxs = (W - eps) / (nx - 1), sys = (H - eps) / (ny - 1);
+    const int j0 = max(0, (int)floor((xmin - 0.5 * eps) / sxs) - 1);
+    const int j1 = min(nx - 1, (int)ceil((xmax - 0.5 * eps) / sxs) + 1);
+    const int i0 = max(0, (int)floor((ymin - 0.5 * eps) / sys) - 1);
+    const int i1 = min(ny - 1, (int)ceil((ymax - 0.5 * eps) / sys) + 1);
+    for (int i = i0; i <= i1; ++i) {
+        const double py = (double)i * (H - eps) / (ny - 1) + 0.5 * eps;
+        if (py < ymin - 1e-9 || py > ymax + 1e-9) continue;
+        for (int j = j0; j <= j1; ++j) {
+            const double px = (double)j * (W - eps) / (nx - 1) + 0.5 * eps;
+            double u, v, w;
+            barycentric(ax, ay, bx, by, cx, cy, px, py, u, v, w);
+            if (inside(u, v)) atomicMin(&owner[(size_t)i * nx + j], t);
+        }
+    }
+}
+
+// Rasterization, pass 2: piecewise-linear interpolation of per-vertex values
+// in the owning triangle (mesh.cpp:264-274).  partials: (sum, misses).
+__global__ __launch_bounds__(256) void k_raster_gather(const double* __restrict__ xy, int mx,
+                                                       const double* __restrict__ vals0,
+                                                       const double* __restrict__ vals1,
+                                                       int nx, int ny, double W, double H,
+                                                       const int* __restrict__ owner,
+                                                       double* __restrict__ out0,
+                                                       double* __restrict__ out1,
+                                                       double* __restrict__ partials) {
+    __shared__ double smem[16];
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double sum = 0.0, miss = 0.0;
+    if (s < (long long)nx * ny) {
+        const int i = (int)(s / nx), j = (int)(s - (long long)i * nx);
+        const int t = owner[s];
+        if (t == INT_MAX) {
+            out0[s] = NAN;
+            if (out1) out1[s] = NAN;
+            miss = 1.0;
+        } else {
+            const double eps = 1e-8;
+            const double py = (double)i * (H - eps) / (ny - 1) + 0.5 * eps;
+            const double px = (double)j * (W - eps) / (nx - 1) + 0.5 * eps;
+            int a, b, c;
+            tri_vertices(mx, t, a, b, c);
+            double u, v, w;
+            barycentric(xy[2 * a], xy[2 * a + 1], xy[2 * b], xy[2 * b + 1], xy[2 * c],
+                        xy[2 * c + 1], px, py, u, v, w);
+            const double r0 = vals0[a] * u + vals0[b] * v + vals0[c] * w;
+            out0[s] = r0;
+            sum = r0;
+            if (out1) out1[s] = vals1[a] * u + vals1[b] * v + vals1[c] * w;
+        }
+    }
+    const double a0 = cd_block_sum(sum, smem);
+    const double a1 = cd_block_sum(miss, smem);
+    if (threadIdx.x == 0) { partials[2 * blockIdx.x] = a0; partials[2 * blockIdx.x + 1] = a1; }
+}
+
+// subtractAverage (utils.cpp:60-86) with the sum already reduced
+__global__ __launch_bounds__(256) void k_subtract_mean(double* __restrict__ a, size_t n,
+                                                       const double* __restrict__ sum,
+                                                       double count) {
+    const double mean = sum[0] / count;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const double v = a[i];
+        if (!isnan(v)) a[i] = v - mean;
+    }
+}
+
+// Gradient of phi at a cell by clamped central differences (utils.cpp:3-20).
+struct GradX {
+    const double* p; int w, h;
+    __device__ double operator()(int x, int y) const {
+        const int xp = min(x + 1, w - 1), xm = max(x - 1, 0);
+        return (p[(size_t)y * w + xp] - p[(size_t)y * w + xm]) / 2.0;
+    }
+};
+struct GradY {
+    const double* p; int w, h;
+    __device__ double operator()(int x, int y) const {
+        const int yp = min(y + 1, h - 1), ym = max(y - 1, 0);
+        return (p[(size_t)yp * w + x] - p[(size_t)ym * w + x]) / 2.0;
+    }
+};
+struct Grid {
+    const double* p; int w;
+    __device__ double operator()(int x, int y) const { return p[(size_t)y * w + x]; }
+};
+
+// grad(phi) at the cell-centred vertex lookup + boundary-constrained step
+// (caustic_design.cpp:224-265, mesh.cpp:485-532).  partials: max displacement.
+__global__ __launch_bounds__(256) void k_vertex_step(const double* __restrict__ phi, int nx,
+                                                     int ny, double W, double H, int mx, int my,
+                                                     double step, double* __restrict__ xy,
+                                                     double* __restrict__ vgrad,
+                                                     double* __restrict__ partials) {
+    __shared__ double smem[16];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double dist = 0.0;
+    if (i < mx * my) {
+        const int y = i / mx, x = i - y * mx;
+        const double px = xy[2 * i], py = xy[2 * i + 1];
+        const double fx = (px / W) * nx - 0.5, fy = (py / H) * ny - 0.5;
+        double vx = bilinear(GradX{phi, nx, ny}, nx, ny, fx, fy);
+        double vy = bilinear(GradY{phi, nx, ny}, nx, ny, fx, fy);
+        vgrad[2 * i] = vx;
+        vgrad[2 * i + 1] = vy;
+        if (x == 0 || x == mx - 1) vx = 0.0;
+        if (y == 0 || y == my - 1) vy = 0.0;
+        const double dt = W / mx;
+        const double nxp = px + vx * dt * step, nyp = py + vy * dt * step;
+        const double dx = px - nxp, dy = py - nyp;
+        dist = sqrt(dx * dx + dy * dy + 0.0 * 0.0);
+        xy[2 * i] = nxp;
+        xy[2 * i + 1] = nyp;
+    }
+    const double m = block_max(dist, smem);
+    if (threadIdx.x == 0) partials[blockIdx.x] = m;
+}
+
+// ---------------------------------------------------------------------------
+// height step (caustic_design.cpp:269-332)
+// ---------------------------------------------------------------------------
+// Inverse map, pass 1: each deformed target triangle claims the inset
+// regular source vertices it contains (mesh.cpp:354-364).
+__global__ __launch_bounds__(256) void k_inverse_scatter(const double* __restrict__ xy, int mx,
+                                                         int my, double W, double H,
+                                                         int* __restrict__ owner) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * (mx - 1) * (my - 1)) return;
+    int a, b, c;
+    tri_vertices(mx, t, a, b, c);
+    const double ax = xy[2 * a], ay = xy[2 * a + 1];
+    const double bx = xy[2 * b], by = xy[2 * b + 1];
+    const double cx = xy[2 * c], cy = xy[2 * c + 1];
+    const double xmin = fmin(ax, fmin(bx, cx)), xmax = fmax(ax, fmax(bx, cx));
+    const double ymin = fmin(ay, fmin(by, cy)), ymax = fmax(ay, fmax(by, cy));
+    if (!(isfinite(xmin) && isfinite(xmax) && isfinite(ymin) && isfinite(ymax))) return;
+    const double eps = 1e-8;
+    const double kx = (W - 2 * eps) / W, ky = (H - 2 * eps) / H;
+    const double gx = W / (mx - 1), gy = H / (my - 1);
+    const int j0 = max(0, (int)floor((xmin - eps) / kx / gx) - 1);
+    const int j1 = min(mx - 1, (int)ceil((xmax - eps) / kx / gx) + 1);
+    const int i0 = max(0, (int)floor((ymin - eps) / ky / gy) - 1);
+    const int i1 = min(my - 1, (int)ceil((ymax - eps) / ky / gy) + 1);
+    for (int i = i0; i <= i1; ++i) {
+        const double sy = (double)i * H / (my - 1);
+        const double py = eps + sy * ((H - 2 * eps) / H);
+        for (int j = j0; j <= j1; ++j) {
+            const double sx = (double)j * W / (mx - 1);
+            const double px = eps + sx * ((W - 2 * eps) / W);
+            double u, v, w;
+            barycentric(ax, ay, bx, by, cx, cy, px, py, u, v, w);
+            if (inside(u, v)) atomicMin(&owner[(size_t)i * mx + j], t);
+        }
+    }
+}
+
+// Inverse map, pass 2 + refractive normals (mesh.cpp:365-402, 677-722).
+// partials: misses.
+__global__ __launch_bounds__(256) void k_inverse_normals(const double* __restrict__ xy, int mx,
+                                                         int my, double W, double H,
+                                                         const int* __restrict__ owner,
+                                                         const double* __restrict__ sz,
+                                                         double focal, double ior,
+                                                         double* __restrict__ inv,
+                                                         double* __restrict__ nrm_x,
+                                                         double* __restrict__ nrm_y,
+                                                         double* __restrict__ partials) {
+    __shared__ double smem[16];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double miss = 0.0;
+    if (i < mx * my) {
+        const int y = i / mx, x = i - y * mx;
+        const int t = owner[i];
+        const double sx = (double)x * W / (mx - 1), sy = (double)y * H / (my - 1);
+        if (t == INT_MAX) {
+            miss = 1.0;
+            inv[2 * i] = NAN; inv[2 * i + 1] = NAN;
+            nrm_x[i] = NAN; nrm_y[i] = NAN;
+        } else {
+            const double eps = 1e-8;
+            const double px = eps + sx * ((W - 2 * eps) / W);
+            const double py = eps + sy * ((H - 2 * eps) / H);
+            int a, b, c;
+            tri_vertices(mx, t, a, b, c);
+            double u, v, w;
+            barycentric(xy[2 * a], xy[2 * a + 1], xy[2 * b], xy[2 * b + 1], xy[2 * c],
+                        xy[2 * c + 1], px, py, u, v, w);
+            const int ay_ = a / mx, ax_ = a - ay_ * mx;
+            const int by_ = b / mx, bx_ = b - by_ * mx;
+            const int cy_ = c / mx, cx_ = c - cy_ * mx;
+            const double sax = (double)ax_ * W / (mx - 1), say = (double)ay_ * H / (my - 1);
+            const double sbx = (double)bx_ * W / (mx - 1), sby = (double)by_ * H / (my - 1);
+            const double scx = (double)cx_ * W / (mx - 1), scy = (double)cy_ * H / (my - 1);
+            double ix = sax * u + sbx * v + scx * w;
+            double iy = say * u + sby * v + scy * w;
+            if (x == 0) ix = 0.0;
+            if (x == mx - 1) ix = W;
+            if (y == 0) iy = 0.0;
+            if (y == my - 1) iy = H;
+            inv[2 * i] = ix;
+            inv[2 * i + 1] = iy;
+            // transmitted direction, normalised (utils.cpp:370-384)
+            double t0 = ix - sx, t1 = iy - sy, t2 = 0 - sz[i] + focal;
+            const double len = sqrt(0.0 + t0 * t0 + t1 * t1 + t2 * t2);
+            t0 /= len; t1 /= len; t2 /= len;
+            const double n0 = t0 - 0.0 * ior, n1 = t1 - 0.0 * ior, n2 = t2 - 1.0 * ior;
+            nrm_x[i] = n0 / n2;
+            nrm_y[i] = n1 / n2;
+        }
+    }
+    const double m = cd_block_sum(miss, smem);
+    if (threadIdx.x == 0) partials[blockIdx.x] = m;
+}
+
+// divergence with a zero outer ring (utils.cpp:22-39); partials: sum
+__global__ __launch_bounds__(256) void k_divergence(const double* __restrict__ nx_,
+                                                    const double* __restrict__ ny_, int w, int h,
+                                                    double* __restrict__ out,
+                                                    double* __restrict__ partials) {
+    __shared__ double smem[16];
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (s < (long long)w * h) {
+        const int y = (int)(s / w), x = (int)(s - (long long)y * w);
+        if (x == 0 || x == w - 1 || y == 0 || y == h - 1) {
+            v = 0.0;
+        } else {
+            const double dx = (nx_[s + 1] - nx_[s - 1]) / 2.0;
+            const double dy = (ny_[s + w] - ny_[s - w]) / 2.0;
+            v = dx + dy;
+        }
+        out[s] = v;
+    }
+    const double a = cd_block_sum(v, smem);
+    if (threadIdx.x == 0) partials[blockIdx.x] = a;
+}
+
+// vertex heights from h (caustic_design.cpp:322-329); partials: min
+__global__ __launch_bounds__(256) void k_height_sample(const double* __restrict__ hg, int nx,
+                                                       int ny, double W, double H, int mx, int my,
+                                                       double* __restrict__ hv,
+                                                       double* __restrict__ partials) {
+    __shared__ double smem[16];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double mn = INFINITY;
+    if (i < mx * my) {
+        const int y = i / mx, x = i - y * mx;
+        const double sx = (double)x * W / (mx - 1), sy = (double)y * H / (my - 1);
+        const double v = bilinear(Grid{hg, nx}, nx, ny, (sx / W) * nx - 0.5, (sy / H) * ny - 0.5);
+        hv[i] = v;
+        mn = v;
+    }
+    const double m = block_min(mn, smem);
+    if (threadIdx.x == 0) partials[blockIdx.x] = m;
+}
+
+// shift by min(0, min h) and store (mesh.cpp:724-742); partials: update sum
+__global__ __launch_bounds__(256) void k_height_apply(const double* __restrict__ hv, int V,
+                                                      const double* __restrict__ mn,
+                                                      double* __restrict__ sz,
+                                                      double* __restrict__ partials) {
+    __shared__ double smem[16];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double u = 0.0;
+    if (i < V) {
+        const double shift = fmin(0.0, mn[0]);
+        const double hh = hv[i] - shift;
+        u = (hh - sz[i]) * (hh - sz[i]);
+        sz[i] = hh;
+    }
+    const double a = cd_block_sum(u, smem);
+    if (threadIdx.x == 0) partials[blockIdx.x] = a;
+}
+
+__global__ void k_points3(const double* __restrict__ xy, const double* __restrict__ z, int V,
+                          double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    out[3 * i] = xy[2 * i];
+    out[3 * i + 1] = xy[2 * i + 1];
+    out[3 * i + 2] = z ? z[i] : 0.0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// host side
+// ===========================================================================
+struct cd_design {
+    cd_context* ctx = nullptr;
+    cd_design_params p;
+    int mx = 0, my = 0, nx = 0, ny = 0;
+    size_t V = 0, G = 0;
+    int T = 0;
+    bool initialized = false;
+    DevBuf txy, sxy, sz, tareas, errors, area, vgrad, inv, nrm_x, nrm_y, hv, quadacc;
+    DevBuf pixels, image, raster, phi, h, div, normx, normy, owner, owner_src, owner_v;
+    DevBuf partials, scalars, tmp3;
+    cd_solve_stats last_phi{}, last_h{};
+    long long folded_cells = 0;
+};
+
+namespace {
+
+int alloc_design(cd_design* d) {
+    cd_context* c = d->ctx;
+    const size_t V = d->V, G = d->G;
+    CD_TRY(c->ensure(d->txy, 16 * V));
+    CD_TRY(c->ensure(d->sxy, 16 * V));
+    CD_TRY(c->ensure(d->sz, 8 * V));
+    CD_TRY(c->ensure(d->tareas, 8 * V));
+    CD_TRY(c->ensure(d->errors, 8 * V));
+    CD_TRY(c->ensure(d->area, 8 * V));
+    CD_TRY(c->ensure(d->vgrad, 16 * V));
+    CD_TRY(c->ensure(d->inv, 16 * V));
+    CD_TRY(c->ensure(d->nrm_x, 8 * V));
+    CD_TRY(c->ensure(d->nrm_y, 8 * V));
+    CD_TRY(c->ensure(d->hv, 8 * V));
+    CD_TRY(c->ensure(d->quadacc, 6 * 8 * V));
+    CD_TRY(c->ensure(d->owner_v, 4 * V));
+    CD_TRY(c->ensure(d->pixels, 8 * G));
+    CD_TRY(c->ensure(d->image, 8 * G));
+    CD_TRY(c->ensure(d->raster, 8 * G));
+    CD_TRY(c->ensure(d->phi, 8 * G));
+    CD_TRY(c->ensure(d->h, 8 * G));
+    CD_TRY(c->ensure(d->div, 8 * G));
+    CD_TRY(c->ensure(d->normx, 8 * G));
+    CD_TRY(c->ensure(d->normy, 8 * G));
+    CD_TRY(c->ensure(d->owner, 4 * G));
+    CD_TRY(c->ensure(d->owner_src, 4 * G));
+    const size_t nb = (std::max(G, 6 * V) + CD_BLOCK - 1) / CD_BLOCK;
+    CD_TRY(c->ensure(d->partials, 8 * 3 * nb));
+    CD_TRY(c->ensure(d->scalars, 8 * 16));
+    CD_TRY(c->ensure(d->tmp3, 24 * V));
+    return CD_OK;
+}
+
+int finish(cd_design* d, int nblocks, int ncomp, int op0, int op1, int op2, int slot) {
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, d->ctx->stream, d->partials.as<double>(),
+                       nblocks, ncomp, op0, op1, op2, d->scalars.as<double>() + slot);
+    CD_CHECK_LAUNCH(d->ctx);
+    return CD_OK;
+}
+
+int read_scalars(cd_design* d, double* out, int n) {
+    cd_context* c = d->ctx;
+    CD_HIP(c, hipMemcpyAsync(c->host_status, d->scalars.ptr, sizeof(double) * n,
+                             hipMemcpyDeviceToHost, c->stream));
+    CD_HIP(c, hipStreamSynchronize(c->stream));
+    CD_TRY(c->resolve_timers());
+    std::memcpy(out, c->host_status, sizeof(double) * n);
+    return CD_OK;
+}
+
+// rasterize per-vertex values (one or two fields) over triangulation xy
+int raster(cd_design* d, const double* xy, int* owner, bool rebuild_owner, const double* v0,
+           const double* v1, double* out0, double* out1, double* misses_out) {
+    cd_context* c = d->ctx;
+    hipStream_t s = c->stream;
+    const int tb = div_up(d->T, CD_BLOCK);
+    const int gb = div_up((long long)d->G, CD_BLOCK);
+    CD_TRY(c->timer_begin());
+    if (rebuild_owner) {
+        CD_TRY(cd_fill_i32(c, owner, d->G, INT_MAX));
+        hipLaunchKernelGGL(k_raster_scatter, dim3(tb), dim3(CD_BLOCK), 0, s, xy, d->mx, d->my,
+                           d->nx, d->ny, d->p.width, d->p.height, owner);
+    }
+    hipLaunchKernelGGL(k_raster_gather, dim3(gb), dim3(CD_BLOCK), 0, s, xy, d->mx, v0, v1, d->nx,
+                       d->ny, d->p.width, d->p.height, owner, out0, out1,
+                       d->partials.as<double>());
+    CD_CHECK_LAUNCH(c);
+    CD_TRY(finish(d, gb, 2, R_SUM, R_SUM, R_SUM, 0));  // scalars[0]=sum, [1]=misses
+    CD_TRY(c->timer_end(CD_KERNEL_RASTER, rebuild_owner ? 3 : 1,
+                        (rebuild_owner ? 8.0 : 0.0) * d->G + (out1 ? 20.0 : 12.0) * d->G));
+    if (misses_out) {
+        double sc[2];
+        CD_TRY(read_scalars(d, sc, 2));
+        *misses_out = sc[1];
+    }
+    return CD_OK;
+}
+
+int subtract_mean(cd_design* d, double* a, int slot_sum) {
+    hipLaunchKernelGGL(k_subtract_mean, dim3(std::min(div_up((long long)d->G, CD_BLOCK), 65536)),
+                       dim3(CD_BLOCK), 0, d->ctx->stream, a, d->G,
+                       d->scalars.as<double>() + slot_sum, (double)d->G);
+    CD_CHECK_LAUNCH(d->ctx);
+    return CD_OK;
+}
+
+int design_init_from_device_image(cd_design* d, const double* img) {
+    cd_context* c = d->ctx;
+    hipStream_t s = c->stream;
+    const int V = (int)d->V;
+    // min-max normalisation (caustic_design.cpp:335, utils.cpp:88-129)
+    const int nbm = std::min(div_up((long long)d->G, CD_BLOCK), 4096);
+    hipLaunchKernelGGL(k_minmax, dim3(nbm), dim3(CD_BLOCK), 0, s, img, d->G,
+                       d->partials.as<double>());
+    CD_CHECK_LAUNCH(c);
+    CD_TRY(finish(d, nbm, 2, R_MIN, R_MAX, R_SUM, 0));
+    hipLaunchKernelGGL(k_scale, dim3(std::min(div_up((long long)d->G, CD_BLOCK), 65536)),
+                       dim3(CD_BLOCK), 0, s, img, d->pixels.as<double>(), d->G,
+                       d->scalars.as<double>());
+    // mesh (caustic_design.cpp:339; mesh.cpp:3-27)
+    hipLaunchKernelGGL(k_mesh_init, dim3(div_up(V, CD_BLOCK)), dim3(CD_BLOCK), 0, s,
+                       d->txy.as<double>(), d->sxy.as<double>(), d->sz.as<double>(), d->mx,
+                       d->my, d->p.width, d->p.height);
+    // target areas over the initial dual cells (caustic_design.cpp:341-348)
+    hipLaunchKernelGGL(k_quad_integrals, dim3(div_up(6LL * V, CD_BLOCK)), dim3(CD_BLOCK), 0, s,
+                       d->pixels.as<double>(), d->nx, d->ny, d->p.width, d->txy.as<double>(),
+                       d->mx, d->my, d->quadacc.as<double>());
+    hipLaunchKernelGGL(k_cell_sums, dim3(div_up(V, CD_BLOCK)), dim3(CD_BLOCK), 0, s,
+                       d->quadacc.as<double>(), V, d->tareas.as<double>(),
+                       d->partials.as<double>());
+    CD_CHECK_LAUNCH(c);
+    CD_TRY(finish(d, div_up(V, CD_BLOCK), 1, R_SUM, R_SUM, R_SUM, 2));
+    hipLaunchKernelGGL(k_scale_areas, dim3(div_up(V, CD_BLOCK)), dim3(CD_BLOCK), 0, s,
+                       d->tareas.as<double>(), V, d->p.width * d->p.height,
+                       d->scalars.as<double>() + 2);
+    // phi = h = 0 (caustic_design.cpp:354-363)
+    CD_TRY(cd_fill_f64(c, d->phi.as<double>(), d->G, 0.0));
+    CD_TRY(cd_fill_f64(c, d->h.as<double>(), d->G, 0.0));
+    // owners of the static source triangulation (mesh.cpp:291-345 locates
+    // samples in the undeformed source mesh, whose xy never change)
+    CD_TRY(cd_fill_i32(c, d->owner_src.as<int>(), d->G, INT_MAX));
+    hipLaunchKernelGGL(k_raster_scatter, dim3(div_up(d->T, CD_BLOCK)), dim3(CD_BLOCK), 0, s,
+                       d->sxy.as<double>(), d->mx, d->my, d->nx, d->ny, d->p.width, d->p.height,
+                       d->owner_src.as<int>());
+    CD_CHECK_LAUNCH(c);
+    d->initialized = true;
+    d->folded_cells = 0;
+    return CD_OK;
+}
+
+int check_params(const cd_design_params* p) {
+    if (!p) return CD_ERR_BAD_ARG;
+    if (p->mesh_res_x < 2 || p->mesh_res_y < 2 || p->res_x < 2 || p->res_y < 2) return CD_ERR_BAD_ARG;
+    if (!(p->width > 0) || !(p->height > 0)) return CD_ERR_BAD_ARG;
+    return CD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void cd_design_params_default(cd_design_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->ior = 1.49;
+    p->step_size = (double)0.05f;
+    cd_solve_opts_default(&p->phi_solve);
+    p->phi_solve.stop = CD_STOP_MAX_UPDATE;
+    p->phi_solve.tol = 0.0000001;
+    p->phi_solve.omega = -1.0;
+    cd_solve_opts_default(&p->h_solve);
+    p->h_solve.stop = CD_STOP_MAX_UPDATE;
+    p->h_solve.tol = 0.00000001;
+    p->h_solve.omega = -1.0;
+}
+
+int cd_design_create(cd_context* ctx, const cd_design_params* p, cd_design** out) {
+    if (!ctx || !out) return CD_ERR_BAD_ARG;
+    *out = nullptr;
+    CD_TRY(check_params(p));
+    CD_HIP(ctx, hipSetDevice(ctx->device));
+    auto* d = new cd_design();
+    d->ctx = ctx;
+    d->p = *p;
+    d->mx = p->mesh_res_x; d->my = p->mesh_res_y;
+    d->nx = p->res_x; d->ny = p->res_y;
+    d->V = (size_t)d->mx * d->my;
+    d->G = (size_t)d->nx * d->ny;
+    d->T = 2 * (d->mx - 1) * (d->my - 1);
+    int st = alloc_design(d);
+    if (st != CD_OK) { delete d; return st; }
+    *out = d;
+    return CD_OK;
+}
+
+void cd_design_destroy(cd_design* d) {
+    if (!d) return;
+    (void)hipStreamSynchronize(d->ctx->stream);
+    delete d;
+}
+
+int cd_design_set_solve_opts(cd_design* d, const cd_solve_opts* phi, const cd_solve_opts* h) {
+    if (!d) return CD_ERR_BAD_ARG;
+    if (phi) d->p.phi_solve = *phi;
+    if (h) d->p.h_solve = *h;
+    return CD_OK;
+}
+
+int cd_design_init(cd_design* d, const double* image) {
+    if (!d || !image) return CD_ERR_BAD_ARG;
+    cd_context* c = d->ctx;
+    CD_HIP(c, hipSetDevice(c->device));
+    CD_HIP(c, hipMemcpyAsync(d->image.ptr, image, 8 * d->G, hipMemcpyHostToDevice, c->stream));
+    CD_TRY(design_init_from_device_image(d, d->image.as<double>()));
+    CD_HIP(c, hipStreamSynchronize(c->stream));
+    return CD_OK;
+}
+
+int cd_design_init_device(cd_design* d, const double* d_image) {
+    if (!d || !d_image) return CD_ERR_BAD_ARG;
+    CD_HIP(d->ctx, hipSetDevice(d->ctx->device));
+    return design_init_from_device_image(d, d_image);
+}
+
+int cd_design_transport_step(cd_design* d, double* step_out) {
+    if (!d) return CD_ERR_BAD_ARG;
+    if (!d->initialized) return CD_ERR_STATE;
+    cd_context* c = d->ctx;
+    hipStream_t s = c->stream;
+    CD_HIP(c, hipSetDevice(c->device));
+    const int V = (int)d->V;
+    const int vb = div_up(V, CD_BLOCK);
+    // dual-cell areas and mismatch (caustic_design.cpp:194-209)
+    CD_TRY(c->timer_begin());
+    hipLaunchKernelGGL(k_cell_errors, dim3(vb), dim3(CD_BLOCK), 0, s, d->txy.as<double>(),
+                       d->tareas.as<double>(), d->mx, d->my, d->area.as<double>(),
+                       d->errors.as<double>(), d->partials.as<double>());
+    CD_CHECK_LAUNCH(c);
+    CD_TRY(finish(d, vb, 1, R_SUM, R_SUM, R_SUM, 4));  // scalars[4] = folded cells
+    CD_TRY(c->timer_end(CD_KERNEL_VERTEX, 1, 64.0 * V));
+    // rasterize onto the grid (caustic_design.cpp:213, mesh.cpp:234-288)
+    double misses = 0.0;
+    CD_TRY(raster(d, d->txy.as<double>(), d->owner.as<int>(), true, d->errors.as<double>(),
+                  nullptr, d->raster.as<double>(), nullptr, &misses));
+    if (misses > 0) {
+        c->set_error("raster miss: " + std::to_string((long long)misses) + " samples");
+        if (step_out) *step_out = NAN;
+        return CD_ERR_RASTER_MISS;
+    }
+    // zero-mean projection (caustic_design.cpp:221) and the phi solve (:222)
+    CD_TRY(subtract_mean(d, d->raster.as<double>(), 0));
+    int st = cd_solve_device_internal(c, d->raster.as<double>(), d->phi.as<double>(), nullptr,
+                                      (long long)d->G, d->nx, d->ny, &d->p.phi_solve,
+                                      &d->last_phi);
+    if (st != CD_OK && st != CD_ERR_NOT_CONVERGED) return st;
+    // gradient, bilinear vertex lookup, boundary-constrained step (:224-265)
+    CD_TRY(c->timer_begin());
+    hipLaunchKernelGGL(k_vertex_step, dim3(vb), dim3(CD_BLOCK), 0, s, d->phi.as<double>(), d->nx,
+                       d->ny, d->p.width, d->p.height, d->mx, d->my, d->p.step_size,
+                       d->txy.as<double>(), d->vgrad.as<double>(), d->partials.as<double>());
+    CD_CHECK_LAUNCH(c);
+    CD_TRY(finish(d, vb, 1, R_MAX, R_SUM, R_SUM, 3));
+    CD_TRY(c->timer_end(CD_KERNEL_VERTEX, 1, 64.0 * V));
+    double sc[5];
+    CD_TRY(read_scalars(d, sc, 5));
+    d->folded_cells = (long long)sc[4];
+    if (step_out) *step_out = sc[3] / d->p.width;
+    return CD_OK;
+}
+
+int cd_design_height_step(cd_design* d, double* update_out) {
+    if (!d) return CD_ERR_BAD_ARG;
+    if (!d->initialized) return CD_ERR_STATE;
+    cd_context* c = d->ctx;
+    hipStream_t s = c->stream;
+    CD_HIP(c, hipSetDevice(c->device));
+    const int V = (int)d->V;
+    const int vb = div_up(V, CD_BLOCK);
+    // inverse transport map + refractive normals (caustic_design.cpp:271, mesh.cpp:348-409, 677-722)
+    const double focal = d->nx / d->p.width * d->p.focal_l;
+    CD_TRY(c->timer_begin());
+    CD_TRY(cd_fill_i32(c, d->owner_v.as<int>(), d->V, INT_MAX));
+    hipLaunchKernelGGL(k_inverse_scatter, dim3(div_up(d->T, CD_BLOCK)), dim3(CD_BLOCK), 0, s,
+                       d->txy.as<double>(), d->mx, d->my, d->p.width, d->p.height,
+                       d->owner_v.as<int>());
+    hipLaunchKernelGGL(k_inverse_normals, dim3(vb), dim3(CD_BLOCK), 0, s, d->txy.as<double>(),
+                       d->mx, d->my, d->p.width, d->p.height, d->owner_v.as<int>(),
+                       d->sz.as<double>(), focal, d->p.ior, d->inv.as<double>(),
+                       d->nrm_x.as<double>(), d->nrm_y.as<double>(), d->partials.as<double>());
+    CD_CHECK_LAUNCH(c);
+    CD_TRY(finish(d, vb, 1, R_SUM, R_SUM, R_SUM, 5));
+    CD_TRY(c->timer_end(CD_KERNEL_VERTEX, 3, 72.0 * V));
+    double sc6[6];
+    CD_TRY(read_scalars(d, sc6, 6));
+    if (sc6[5] > 0) {
+        c->set_error("inverse map: " + std::to_string((long long)sc6[5]) +
+                     " source vertices outside the target mesh");
+        return CD_ERR_RASTER_MISS;
+    }
+    // normals rasterized over the static source mesh (caustic_design.cpp:274-277)
+    double misses = 0.0;
+    CD_TRY(raster(d, d->sxy.as<double>(), d->owner_src.as<int>(), false, d->nrm_x.as<double>(),
+                  d->nrm_y.as<double>(), d->normx.as<double>(), d->normy.as<double>(), &misses));
+    if (misses > 0) {
+        c->set_error("source raster miss");
+        return CD_ERR_RASTER_MISS;
+    }
+    // divergence, zero-mean projection, height solve (caustic_design.cpp:307-311)
+    const int gb = div_up((long long)d->G, CD_BLOCK);
+    hipLaunchKernelGGL(k_divergence, dim3(gb), dim3(CD_BLOCK), 0, s, d->normx.as<double>(),
+                       d->normy.as<double>(), d->nx, d->ny, d->div.as<double>(),
+                       d->partials.as<double>());
+    CD_CHECK_LAUNCH(c);
+    CD_TRY(finish(d, gb, 1, R_SUM, R_SUM, R_SUM, 0));
+    CD_TRY(subtract_mean(d, d->div.as<double>(), 0));
+    int st = cd_solve_device_internal(c, d->div.as<double>(), d->h.as<double>(), nullptr,
+                                      (long long)d->G, d->nx, d->ny, &d->p.h_solve, &d->last_h);
+    if (st != CD_OK && st != CD_ERR_NOT_CONVERGED) return st;
+    // vertex heights (caustic_design.cpp:322-329, mesh.cpp:724-742)
+    CD_TRY(c->timer_begin());
+    hipLaunchKernelGGL(k_height_sample, dim3(vb), dim3(CD_BLOCK), 0, s, d->h.as<double>(), d->nx,
+                       d->ny, d->p.width, d->p.height, d->mx, d->my, d->hv.as<double>(),
+                       d->partials.as<double>());
+    CD_CHECK_LAUNCH(c);
+    CD_TRY(finish(d, vb, 1, R_MIN, R_SUM, R_SUM, 6));
+    hipLaunchKernelGGL(k_height_apply, dim3(vb), dim3(CD_BLOCK), 0, s, d->hv.as<double>(), V,
+                       d->scalars.as<double>() + 6, d->sz.as<double>(), d->partials.as<double>());
+    CD_CHECK_LAUNCH(c);
+    CD_TRY(finish(d, vb, 1, R_SUM, R_SUM, R_SUM, 7));
+    CD_TRY(c->timer_end(CD_KERNEL_VERTEX, 2, 48.0 * V));
+    double sc[8];
+    CD_TRY(read_scalars(d, sc, 8));
+    if (update_out) *update_out = sc[7];
+    return CD_OK;
+}
+
+int cd_design_run(cd_design* d, int max_transport_steps, double conv_tres, int height_steps,
+                  cd_run_stats* rs) {
+    if (!d) return CD_ERR_BAD_ARG;
+    if (!d->initialized) return CD_ERR_STATE;
+    auto t0 = std::chrono::steady_clock::now();
+    cd_run_stats r;
+    std::memset(&r, 0, sizeof(r));
+    int status = CD_OK;
+    // main.cpp:243-256: OT steps, stop after the first step below conv_tres
+    for (int it = 0; it < max_transport_steps; ++it) {
+        double step = 0.0;
+        int st = cd_design_transport_step(d, &step);
+        r.transport_steps++;
+        r.last_step = step;
+        r.phi_sweeps += d->last_phi.sweeps;
+        r.checks += d->last_phi.checks;
+        r.learned_applied += d->last_phi.learned_applied;
+        r.learned_rejected += d->last_phi.learned_rejected;
+        if (d->last_phi.rel_residual > r.max_phi_residual) r.max_phi_residual = d->last_phi.rel_residual;
+        if (st != CD_OK) { status = st; break; }
+        if (step < conv_tres) break;
+    }
+    // main.cpp:260-262: height steps
+    for (int it = 0; status == CD_OK && it < height_steps; ++it) {
+        double upd = 0.0;
+        int st = cd_design_height_step(d, &upd);
+        r.height_steps++;
+        r.h_sweeps += d->last_h.sweeps;
+        r.checks += d->last_h.checks;
+        r.learned_applied += d->last_h.learned_applied;
+        r.learned_rejected += d->last_h.learned_rejected;
+        if (d->last_h.rel_residual > r.max_h_residual) r.max_h_residual = d->last_h.rel_residual;
+        if (st != CD_OK) { status = st; break; }
+    }
+    r.status = status;
+    r.elapsed_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (rs) *rs = r;
+    return status;
+}
+
+size_t cd_design_field_size(const cd_design* d, int field) {
+    if (!d) return 0;
+    switch (field) {
+        case CD_FIELD_TARGET_POINTS:
+        case CD_FIELD_SOURCE_POINTS: return 3 * d->V;
+        case CD_FIELD_NORMALS:
+        case CD_FIELD_VERTEX_GRADIENT:
+        case CD_FIELD_INVERSE_MAP: return 2 * d->V;
+        case CD_FIELD_ERRORS:
+        case CD_FIELD_TARGET_AREAS: return d->V;
+        case CD_FIELD_PHI:
+        case CD_FIELD_H:
+        case CD_FIELD_RASTER:
+        case CD_FIELD_DIVERGENCE:
+        case CD_FIELD_NORM_X:
+        case CD_FIELD_NORM_Y:
+        case CD_FIELD_PIXELS:
+        case CD_FIELD_IMAGE: return d->G;
+        default: return 0;
+    }
+}
+
+const void* cd_design_device_ptr(cd_design* d, int field) {
+    if (!d) return nullptr;
+    switch (field) {
+        case CD_FIELD_PHI: return d->phi.ptr;
+        case CD_FIELD_H: return d->h.ptr;
+        case CD_FIELD_RASTER: return d->raster.ptr;
+        case CD_FIELD_ERRORS: return d->errors.ptr;
+        case CD_FIELD_TARGET_AREAS: return d->tareas.ptr;
+        case CD_FIELD_DIVERGENCE: return d->div.ptr;
+        case CD_FIELD_NORM_X: return d->normx.ptr;
+        case CD_FIELD_NORM_Y: return d->normy.ptr;
+        case CD_FIELD_PIXELS: return d->pixels.ptr;
+        case CD_FIELD_IMAGE: return d->image.ptr;
+        case CD_FIELD_INVERSE_MAP: return d->inv.ptr;
+        case CD_FIELD_VERTEX_GRADIENT: return d->vgrad.ptr;
+        default: return nullptr;
+    }
+}
+
+int cd_design_get(cd_design* d, int field, double* out, size_t count) {
+    if (!d || !out) return CD_ERR_BAD_ARG;
+    const size_t n = cd_design_field_size(d, field);
+    if (n == 0 || count < n) return CD_ERR_BAD_ARG;
+    cd_context* c = d->ctx;
+    hipStream_t s = c->stream;
+    CD_HIP(c, hipSetDevice(c->device));
+    const void* src = nullptr;
+    const int V = (int)d->V;
+    switch (field) {
+        case CD_FIELD_TARGET_POINTS:
+            hipLaunchKernelGGL(k_points3, dim3(div_up(V, CD_BLOCK)), dim3(CD_BLOCK), 0, s,
+                               d->txy.as<double>(), (const double*)nullptr, V, d->tmp3.as<double>());
+            CD_CHECK_LAUNCH(c);
+            src = d->tmp3.ptr;
+            break;
+        case CD_FIELD_SOURCE_POINTS:
+            hipLaunchKernelGGL(k_points3, dim3(div_up(V, CD_BLOCK)), dim3(CD_BLOCK), 0, s,
+                               d->sxy.as<double>(), d->sz.as<double>(), V, d->tmp3.as<double>());
+            CD_CHECK_LAUNCH(c);
+            src = d->tmp3.ptr;
+            break;
+        case CD_FIELD_NORMALS:
+            CD_HIP(c, hipMemcpyAsync(out, d->nrm_x.ptr, 8 * d->V, hipMemcpyDeviceToHost, s));
+            CD_HIP(c, hipMemcpyAsync(out + d->V, d->nrm_y.ptr, 8 * d->V, hipMemcpyDeviceToHost, s));
+            CD_HIP(c, hipStreamSynchronize(s));
+            return CD_OK;
+        default:
+            src = cd_design_device_ptr(d, field);
+    }
+    if (!src) return CD_ERR_BAD_ARG;
+    CD_HIP(c, hipMemcpyAsync(out, src, 8 * n, hipMemcpyDeviceToHost, s));
+    CD_HIP(c, hipStreamSynchronize(s));
+    return CD_OK;
+}
+
+int cd_design_set(cd_design* d, int field, const double* in, size_t count) {
+    if (!d || !in) return CD_ERR_BAD_ARG;
+    const size_t n = cd_design_field_size(d, field);
+    if (n == 0 || count < n) return CD_ERR_BAD_ARG;
+    cd_context* c = d->ctx;
+    hipStream_t s = c->stream;
+    CD_HIP(c, hipSetDevice(c->device));
+    if (field == CD_FIELD_TARGET_POINTS || field == CD_FIELD_SOURCE_POINTS) {
+        // accept {x,y,z} triples; target z and source xy are fixed by construction
+        std::vector<double> xy(2 * d->V), z(d->V);
+        for (size_t i = 0; i < d->V; ++i) {
+            xy[2 * i] = in[3 * i];
+            xy[2 * i + 1] = in[3 * i + 1];
+            z[i] = in[3 * i + 2];
+        }
+        if (field == CD_FIELD_TARGET_POINTS) {
+            CD_HIP(c, hipMemcpyAsync(d->txy.ptr, xy.data(), 16 * d->V, hipMemcpyHostToDevice, s));
+        } else {
+            CD_HIP(c, hipMemcpyAsync(d->sz.ptr, z.data(), 8 * d->V, hipMemcpyHostToDevice, s));
+        }
+        CD_HIP(c, hipStreamSynchronize(s));
+        return CD_OK;
+    }
+    void* dst = const_cast<void*>(cd_design_device_ptr(d, field));
+    if (!dst || field == CD_FIELD_NORMALS) return CD_ERR_BAD_ARG;
+    CD_HIP(c, hipMemcpyAsync(dst, in, 8 * n, hipMemcpyHostToDevice, s));
+    CD_HIP(c, hipStreamSynchronize(s));
+    return CD_OK;
+}
+
+int cd_design_last_solve_stats(const cd_design* d, cd_solve_stats* phi, cd_solve_stats* h) {
+    if (!d) return CD_ERR_BAD_ARG;
+    if (phi) *phi = d->last_phi;
+    if (h) *h = d->last_h;
+    return CD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,177 @@
+// Internal declarations shared by the translation units of libcaustic_amd.so.
+// gfx950 only: 64-wide wavefronts, 256-thread workgroups by default.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "cd_api.h"
+
+#define CD_WAVE 64
+#define CD_BLOCK 256
+
+// ---------------------------------------------------------------------------
+// error plumbing: every HIP call in host code goes through CD_HIP, which
+// records the message on the context and returns CD_ERR_HIP from the caller.
+// ---------------------------------------------------------------------------
+#define CD_HIP(ctx, expr)                                                          \
+    do {                                                                           \
+        hipError_t _e = (expr);                                                    \
+        if (_e != hipSuccess) {                                                    \
+            (ctx)->set_error(std::string(#expr) + ": " + hipGetErrorString(_e));   \
+            return CD_ERR_HIP;                                                     \
+        }                                                                          \
+    } while (0)
+
+#define CD_CHECK_LAUNCH(ctx) CD_HIP(ctx, hipGetLastError())
+
+#define CD_TRY(expr)                   \
+    do {                               \
+        int _s = (expr);               \
+        if (_s != CD_OK) return _s;    \
+    } while (0)
+
+// Device buffer owned by the library (grow-only).
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { release(); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    template <typename T>
+    T* as() const { return static_cast<T*>(ptr); }
+};
+
+// Per-kernel-family timing (HIP events, recorded only when profiling is on).
+struct KernelTimer {
+    double total_ms = 0.0;
+    long long launches = 0;
+    double bytes = 0.0;
+};
+
+// Device-side status of one Poisson solve (written by kernels, read by the
+// host through a pinned copy at every convergence check).
+struct SolveDev {
+    int done;            // set by a check kernel once the stop rule holds
+    int done_check;      // check index that set `done` (-1: none)
+    int sweeps_at_done;  // max-update rule: sweep count when `done` was set
+    int checks;
+    double bsum;         // sum of the (non-hole) right-hand side
+    double bmean;        // its mean (the zero-mean projection)
+    double bnorm;        // ||P b||_2
+    double rsum, rnorm;  // last residual sum and ||P b - A phi||_2
+    double rel;          // last relative residual
+    double rel0;         // first relative residual
+    double last_max_update;
+    float rmean32;       // mean of the last FP32 residual (projected out in sweeps)
+    unsigned int maxupd_bits[2];
+    int pad;
+};
+
+// An event pair recorded around a group of launches; resolved (elapsed time
+// read) only after the stream has been synchronised for other reasons, so
+// profiling adds no host round trips.
+struct PendingTimer {
+    hipEvent_t a, b;
+    int kernel;
+    long long launches;
+    double bytes;
+};
+
+struct cd_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    bool profiling = false;
+    KernelTimer timers[CD_KERNEL_COUNT];
+    std::vector<hipEvent_t> event_pool;
+    std::vector<PendingTimer> pending;
+    hipEvent_t open_event = nullptr;
+
+    // solver workspace (grow-only)
+    DevBuf e32, r32, e64w, phi64w, b64w, mask, partials, solve_dev, fill;
+    SolveDev* host_status = nullptr;  // pinned
+    int ws_w = 0, ws_h = 0;
+
+    // learned-init network handle (opaque, owned elsewhere)
+    void* net = nullptr;
+
+    void set_error(const std::string& s) { last_error = s; }
+    int ensure(DevBuf& b, size_t bytes);
+    int timer_begin();
+    int timer_end(int kernel, long long launches, double bytes);
+    int resolve_timers();  // call after the stream has been synchronised
+    hipEvent_t take_event();
+};
+
+// ---- shared device helpers -------------------------------------------------
+__device__ __forceinline__ double cd_wave_sum(double v) {
+#pragma unroll
+    for (int o = CD_WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, CD_WAVE);
+    return v;
+}
+
+__device__ __forceinline__ float cd_wave_max(float v) {
+#pragma unroll
+    for (int o = CD_WAVE / 2; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, CD_WAVE));
+    return v;
+}
+
+__device__ __forceinline__ double cd_wave_min(double v) {
+#pragma unroll
+    for (int o = CD_WAVE / 2; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, CD_WAVE));
+    return v;
+}
+
+__device__ __forceinline__ double cd_wave_max_d(double v) {
+#pragma unroll
+    for (int o = CD_WAVE / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, CD_WAVE));
+    return v;
+}
+
+// Block-wide sum of one double per thread (any block size that is a multiple
+// of 64, at most 1024).  Result valid in thread 0.
+__device__ __forceinline__ double cd_block_sum(double v, double* smem /* >= 16 */) {
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    const int nthr = blockDim.x * blockDim.y;
+    v = cd_wave_sum(v);
+    if ((tid & (CD_WAVE - 1)) == 0) smem[tid / CD_WAVE] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (tid < CD_WAVE) {
+        t = (tid < nthr / CD_WAVE) ? smem[tid] : 0.0;
+        t = cd_wave_sum(t);
+    }
+    __syncthreads();
+    return t;
+}
+
+// ---- host-side helpers shared across units ---------------------------------
+// Deterministic sum of `n` doubles at stride 1 (count values) into out[k] for
+// each of `ncomp` interleaved components: partials laid out [block][ncomp].
+int cd_reduce_partials(cd_context* ctx, const double* partials, int nblocks, int ncomp,
+                       double* out_dev);
+
+// Fill helpers (graph-safe: kernels, not hipMemsetAsync).
+int cd_fill_f32(cd_context* ctx, float* p, size_t n, float v);
+int cd_fill_f64(cd_context* ctx, double* p, size_t n, double v);
+int cd_fill_i32(cd_context* ctx, int* p, size_t n, int v);
+
+// Poisson solve on device-resident data.  b is the right-hand side (not
+// modified), phi the warm start/result (FP64).  mask: optional per-cell
+// neighbour-validity bits (NULL: no holes).
+int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, const uint8_t* mask,
+                             long long valid_cells, int w, int h, const cd_solve_opts* o,
+                             cd_solve_stats* st);
+
+// Learned-init hook (implemented in cd_learned.hip when a net is attached):
+// predicts a correction e (FP32, w*h) from the FP32 residual r.
+int cd_learned_predict(cd_context* ctx, const float* r32, const float* rmean, float* e32, int w,
+                       int h);

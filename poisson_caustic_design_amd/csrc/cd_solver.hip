@@ -1,0 +1,500 @@
+// Poisson solver for the reference operator (src/solver.cpp:12-61):
+//     sum_{valid nbr n} phi_n - cnt * phi_c = D_c
+// on the cell-centred grid with Neumann edges (a neighbour outside the grid or
+// on a NaN hole of D is not counted).  Replaces poisson_solver
+// (src/solver.h:8, src/solver.cpp:70-147).
+//
+// Method (MI355X-first):
+//   * red-black SOR.  Red-black ordering removes the reference's tile data
+//     race by construction.  omega defaults to the optimum for this operator
+//     (cd_sor_omega); the reference's 2/(1+3.14159265/width) (solver.cpp:71)
+//     is kept for the drop-in entry points.
+//   * CD_PREC_MIXED: sweeps run in FP32 on a correction e with the right-hand
+//     side r = P b - A phi64 (FP64 residual, zero-mean projected); every
+//     `check_every` sweeps e is folded into the FP64 base (defect correction)
+//     and the FP64 relative residual is evaluated on the device.
+//   * CD_PREC_FP64: the same sweeps in FP64 directly on phi.
+//   * stop rules are evaluated by device kernels that set a `done` flag; later
+//     sweep launches of the same batch see the flag and exit immediately.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "cd_internal.h"
+
+namespace {
+
+constexpr int SWEEP_BX = 64;   // x-pairs per block (128 columns)
+constexpr int SWEEP_BY = 4;    // rows per block
+constexpr int RED_BX = 64;
+constexpr int RED_BY = 4;
+
+// mask bits: neighbour validity (left, up, right, down) and "cell is a hole"
+constexpr uint8_t M_L = 1, M_U = 2, M_R = 4, M_D = 8, M_HOLE = 0x80;
+
+// ---------------------------------------------------------------------------
+// right-hand-side statistics, hole mask
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_build_mask(const double* __restrict__ b,
+                                                    uint8_t* __restrict__ mask, int w, int h,
+                                                    double* __restrict__ partials) {
+    __shared__ double smem[16];
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y * blockDim.y + threadIdx.y;
+    double holes = 0.0;
+    if (x < w && y < h) {
+        size_t i = (size_t)y * w + x;
+        uint8_t m = 0;
+        if (isnan(b[i])) { m = M_HOLE; holes = 1.0; }
+        if (x > 0 && !isnan(b[i - 1])) m |= M_L;
+        if (y > 0 && !isnan(b[i - w])) m |= M_U;
+        if (x < w - 1 && !isnan(b[i + 1])) m |= M_R;
+        if (y < h - 1 && !isnan(b[i + w])) m |= M_D;
+        mask[i] = m;
+    }
+    double s = cd_block_sum(holes, smem);
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+        partials[blockIdx.y * gridDim.x + blockIdx.x] = s;
+}
+
+// pass 1: sum of b over non-hole cells; pass 2 (mean given): sum of (b-mean)^2
+__global__ __launch_bounds__(256) void k_rhs_moments(const double* __restrict__ b, int w, int h,
+                                                     const SolveDev* __restrict__ st, int pass,
+                                                     double* __restrict__ partials) {
+    __shared__ double smem[16];
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y * blockDim.y + threadIdx.y;
+    double v = 0.0;
+    if (x < w && y < h) {
+        double bv = b[(size_t)y * w + x];
+        if (!isnan(bv)) {
+            if (pass == 0) v = bv;
+            else { double d = bv - st->bmean; v = d * d; }
+        }
+    }
+    double s = cd_block_sum(v, smem);
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+        partials[blockIdx.y * gridDim.x + blockIdx.x] = s;
+}
+
+// single block: reduce partials and finish the right-hand-side statistics
+__global__ void k_rhs_finish(const double* __restrict__ partials, int nblocks, SolveDev* st,
+                             int pass, double count) {
+    __shared__ double smem[16];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += partials[b];
+    s = cd_block_sum(s, smem);
+    if (threadIdx.x == 0) {
+        if (pass == 0) {
+            st->done = 0;
+            st->done_check = -1;
+            st->sweeps_at_done = -1;
+            st->checks = 0;
+            st->bsum = s;
+            st->bmean = count > 0 ? s / count : 0.0;
+            st->rel = INFINITY;
+            st->rel0 = INFINITY;
+            st->last_max_update = INFINITY;
+            st->rmean32 = 0.f;
+            st->maxupd_bits[0] = 0u;
+            st->maxupd_bits[1] = 0u;
+        } else {
+            st->bnorm = sqrt(s);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// defect correction: phi64 += e32, e32 = 0 (elementwise)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_flush(double* __restrict__ phi, float* __restrict__ e,
+                                               size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        float ev = e[i];
+        phi[i] += (double)ev;
+        e[i] = 0.f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 residual r = (b - bmean) - A phi  (+ FP32 copy for the inner sweeps)
+// partials[block] = (sum r, sum r^2)
+// ---------------------------------------------------------------------------
+template <bool WRITE_R32>
+__global__ __launch_bounds__(256) void k_residual(const double* __restrict__ phi,
+                                                  const double* __restrict__ b,
+                                                  const uint8_t* __restrict__ mask,
+                                                  const SolveDev* __restrict__ st,
+                                                  float* __restrict__ r32, int w, int h,
+                                                  double* __restrict__ partials) {
+    __shared__ double smem[16];
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y * blockDim.y + threadIdx.y;
+    double rs = 0.0, rq = 0.0;
+    if (x < w && y < h) {
+        size_t i = (size_t)y * w + x;
+        double c = phi[i];
+        double s = 0.0, cnt = 0.0;
+        bool hole = false;
+        if (mask) {
+            uint8_t m = mask[i];
+            hole = (m & M_HOLE) != 0;
+            if (m & M_L) { cnt += 1.0; s += phi[i - 1]; }
+            if (m & M_U) { cnt += 1.0; s += phi[i - w]; }
+            if (m & M_R) { cnt += 1.0; s += phi[i + 1]; }
+            if (m & M_D) { cnt += 1.0; s += phi[i + w]; }
+        } else {
+            if (x > 0) { cnt += 1.0; s += phi[i - 1]; }
+            if (y > 0) { cnt += 1.0; s += phi[i - w]; }
+            if (x < w - 1) { cnt += 1.0; s += phi[i + 1]; }
+            if (y < h - 1) { cnt += 1.0; s += phi[i + w]; }
+        }
+        double r = 0.0;
+        if (!hole) {
+            r = (b[i] - st->bmean) - (s - cnt * c);
+            rs = r;
+            rq = r * r;
+        }
+        if (WRITE_R32) r32[i] = (float)r;
+    }
+    double a = cd_block_sum(rs, smem);
+    double q = cd_block_sum(rq, smem);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        size_t bid = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+        partials[2 * bid] = a;
+        partials[2 * bid + 1] = q;
+    }
+}
+
+// single block: finish a convergence check
+__global__ void k_check(const double* __restrict__ partials, int nblocks, SolveDev* st,
+                        double count, int check_id, int stop_rule, double tol) {
+    __shared__ double smem[16];
+    double a = 0.0, q = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+        a += partials[2 * b];
+        q += partials[2 * b + 1];
+    }
+    a = cd_block_sum(a, smem);
+    q = cd_block_sum(q, smem);
+    if (threadIdx.x == 0) {
+        double mean = count > 0 ? a / count : 0.0;
+        double rn2 = q - count * mean * mean;
+        double rn = sqrt(rn2 > 0 ? rn2 : 0.0);
+        double rel = st->bnorm > 0 ? rn / st->bnorm : (rn > 0 ? INFINITY : 0.0);
+        st->rsum = a;
+        st->rnorm = rn;
+        st->rel = rel;
+        if (st->checks == 0) st->rel0 = rel;
+        st->checks += 1;
+        st->rmean32 = (float)mean;
+        if (stop_rule == CD_STOP_RESIDUAL && !st->done && rel <= tol) {
+            st->done = 1;
+            st->done_check = check_id;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// red-black SOR half-sweep.  `color` 0 updates cells with (x+y) even.
+// e: iterate (FP32 correction or FP64 phi); r: right-hand side with its mean
+// *rmean projected out; the update is the reference's
+//   delta = omega / cnt * (sum - cnt * val - rhs)   (solver.cpp:47, 56)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_rb_sweep(T* __restrict__ e, const T* __restrict__ r,
+                                                  const uint8_t* __restrict__ mask, int w, int h,
+                                                  int color, T omega,
+                                                  const T* __restrict__ rmean, SolveDev* st,
+                                                  int track_max, int sweep) {
+    if (st->done) return;
+    __shared__ float smax[SWEEP_BX * SWEEP_BY / CD_WAVE];
+    const int y = blockIdx.y * SWEEP_BY + threadIdx.y;
+    const int x = 2 * (blockIdx.x * SWEEP_BX + threadIdx.x) + ((y + color) & 1);
+    float m = 0.f;
+    if (y < h && x < w) {
+        const size_t i = (size_t)y * w + x;
+        const T c = e[i];
+        T s = 0, cnt = 0;
+        bool hole = false;
+        if (mask) {
+            const uint8_t mk = mask[i];
+            hole = (mk & M_HOLE) != 0;
+            if (mk & M_L) { cnt += 1; s += e[i - 1]; }
+            if (mk & M_U) { cnt += 1; s += e[i - w]; }
+            if (mk & M_R) { cnt += 1; s += e[i + 1]; }
+            if (mk & M_D) { cnt += 1; s += e[i + w]; }
+        } else {
+            if (x > 0) { cnt += 1; s += e[i - 1]; }
+            if (y > 0) { cnt += 1; s += e[i - w]; }
+            if (x < w - 1) { cnt += 1; s += e[i + 1]; }
+            if (y < h - 1) { cnt += 1; s += e[i + w]; }
+        }
+        if (!hole) {
+            const T rv = r[i] - *rmean;
+            const T d = omega / cnt * (s - cnt * c - rv);
+            e[i] = c + d;
+            m = (float)fabs(d);
+        }
+    }
+    if (track_max) {
+        const int tid = threadIdx.y * SWEEP_BX + threadIdx.x;
+        m = cd_wave_max(m);
+        if ((tid & (CD_WAVE - 1)) == 0) smax[tid / CD_WAVE] = m;
+        __syncthreads();
+        if (tid == 0) {
+            float t = smax[0];
+            for (int k = 1; k < SWEEP_BX * SWEEP_BY / CD_WAVE; ++k) t = fmaxf(t, smax[k]);
+            // non-negative floats order like their bit patterns
+            atomicMax(&st->maxupd_bits[sweep & 1], __float_as_uint(t));
+        }
+    }
+}
+
+// max-update stop rule after the black half of sweep `sweep` (solver.cpp:142)
+__global__ void k_maxcheck(SolveDev* st, int sweep, double tol) {
+    if (threadIdx.x != 0) return;
+    if (!st->done) {
+        float mu = __uint_as_float(st->maxupd_bits[sweep & 1]);
+        st->last_max_update = (double)mu;
+        if ((double)mu < tol) {
+            st->done = 1;
+            st->sweeps_at_done = sweep + 1;
+        }
+    }
+    st->maxupd_bits[(sweep + 1) & 1] = 0u;
+}
+
+__global__ void k_set_holes_nan(double* __restrict__ phi, const uint8_t* __restrict__ mask,
+                                size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && (mask[i] & M_HOLE)) phi[i] = NAN;
+}
+
+inline int div_up(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+// Over-relaxation factor.  > 0: as given.  < 0: the reference's
+// 2/(1+3.14159265/width) (solver.cpp:71).  == 0: the optimum for this
+// operator: the Jacobi iteration I + A/4 has spectral radius
+// rho_J = 1 - sin^2(pi/(2n)) on the zero-mean subspace (n = larger side), and
+// red-black ordering is consistent, so omega = 2/(1 + sqrt(1 - rho_J^2)).
+double cd_sor_omega(double requested, int w, int h) {
+    if (requested > 0) return requested;
+    if (requested < 0) return 2.0 / (1.0 + 3.14159265 / w);
+    const int n = std::max(w, h);
+    const double s = std::sin(M_PI / (2.0 * n));
+    const double rj = 1.0 - s * s;
+    return 2.0 / (1.0 + std::sqrt(1.0 - rj * rj));
+}
+
+int auto_check_every(int w, int h) {
+    // SOR with omega near 2 contracts every mode by ~(1 - 2*pi/n) per sweep.
+    // The FP32 correction loses accuracy once it has absorbed a ~10x residual
+    // reduction, which takes ~ n*ln(10)/(2*pi) sweeps.
+    int n = std::max(w, h);
+    int k = (int)std::lround(0.36 * n);
+    return std::max(8, std::min(k, 2048));
+}
+
+}  // namespace
+
+int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, const uint8_t* mask,
+                             long long valid_cells, int w, int h, const cd_solve_opts* o,
+                             cd_solve_stats* st_out) {
+    auto t_start = std::chrono::steady_clock::now();
+    if (w < 2 || h < 2 || !b || !phi || !o) return CD_ERR_BAD_ARG;
+    if (o->stop != CD_STOP_RESIDUAL && o->stop != CD_STOP_MAX_UPDATE) return CD_ERR_BAD_ARG;
+    if (o->precision != CD_PREC_MIXED && o->precision != CD_PREC_FP64) return CD_ERR_BAD_ARG;
+    const bool mixed = o->precision == CD_PREC_MIXED;
+    const size_t n = (size_t)w * h;
+    const double count = (double)valid_cells;
+
+    // workspace
+    const dim3 rblk(RED_BX, RED_BY);
+    const dim3 rgrid(div_up(w, RED_BX), div_up(h, RED_BY));
+    const int nred = rgrid.x * rgrid.y;
+    CD_TRY(ctx->ensure(ctx->partials, sizeof(double) * 2 * (size_t)nred));
+    CD_TRY(ctx->ensure(ctx->solve_dev, sizeof(SolveDev)));
+    if (mixed) {
+        CD_TRY(ctx->ensure(ctx->e32, sizeof(float) * n));
+        CD_TRY(ctx->ensure(ctx->r32, sizeof(float) * n));
+    }
+    double* partials = ctx->partials.as<double>();
+    SolveDev* sd = ctx->solve_dev.as<SolveDev>();
+    float* e32 = mixed ? ctx->e32.as<float>() : nullptr;
+    float* r32 = mixed ? ctx->r32.as<float>() : nullptr;
+
+    const double omega = cd_sor_omega(o->omega, w, h);
+    const int kmax = o->check_every > 0 ? o->check_every : auto_check_every(w, h);
+    const bool adaptive = o->check_every <= 0 && o->stop == CD_STOP_RESIDUAL;
+    const int track = o->stop == CD_STOP_MAX_UPDATE;
+    hipStream_t s = ctx->stream;
+
+    // right-hand-side statistics: mean over non-hole cells, ||P b||
+    hipLaunchKernelGGL(k_rhs_moments, rgrid, rblk, 0, s, b, w, h, sd, 0, partials);
+    hipLaunchKernelGGL(k_rhs_finish, dim3(1), dim3(1024), 0, s, partials, nred, sd, 0, count);
+    hipLaunchKernelGGL(k_rhs_moments, rgrid, rblk, 0, s, b, w, h, sd, 1, partials);
+    hipLaunchKernelGGL(k_rhs_finish, dim3(1), dim3(1024), 0, s, partials, nred, sd, 1, count);
+    CD_CHECK_LAUNCH(ctx);
+    if (mixed) CD_TRY(cd_fill_f32(ctx, e32, n, 0.f));
+
+    const int fblocks = std::min(div_up((long long)n, CD_BLOCK), 65536);
+    auto enqueue_check = [&](int check_id) -> int {
+        CD_TRY(ctx->timer_begin());
+        if (mixed) {
+            hipLaunchKernelGGL(k_flush, dim3(fblocks), dim3(CD_BLOCK), 0, s, phi, e32, n);
+            hipLaunchKernelGGL(k_residual<true>, rgrid, rblk, 0, s, phi, b, mask, sd, r32, w, h,
+                               partials);
+        } else {
+            hipLaunchKernelGGL(k_residual<false>, rgrid, rblk, 0, s, phi, b, mask, sd, r32, w, h,
+                               partials);
+        }
+        hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, partials, nred, sd, count,
+                           check_id, o->stop, o->tol);
+        CD_CHECK_LAUNCH(ctx);
+        CD_TRY(ctx->timer_end(CD_KERNEL_RESIDUAL, mixed ? 3 : 2,
+                              (mixed ? 44.0 : 16.0) * (double)n));
+        CD_HIP(ctx, hipMemcpyAsync(ctx->host_status, sd, sizeof(SolveDev), hipMemcpyDeviceToHost, s));
+        CD_HIP(ctx, hipStreamSynchronize(s));
+        CD_TRY(ctx->resolve_timers());
+        return CD_OK;
+    };
+
+    const dim3 sblk(SWEEP_BX, SWEEP_BY);
+    const dim3 sgrid(div_up(div_up(w, 2), SWEEP_BX), div_up(h, SWEEP_BY));
+    auto enqueue_sweeps = [&](int first, int count_sweeps) -> int {
+        CD_TRY(ctx->timer_begin());
+        for (int k = 0; k < count_sweeps; ++k) {
+            const int sw = first + k;
+            for (int color = 0; color < 2; ++color) {
+                if (mixed) {
+                    hipLaunchKernelGGL(k_rb_sweep<float>, sgrid, sblk, 0, s, e32, r32, mask, w,
+                                       h, color, (float)omega, &sd->rmean32, sd, track, sw);
+                } else {
+                    hipLaunchKernelGGL(k_rb_sweep<double>, sgrid, sblk, 0, s, phi, b, mask, w,
+                                       h, color, omega, &sd->bmean, sd, track, sw);
+                }
+            }
+            if (track) hipLaunchKernelGGL(k_maxcheck, dim3(1), dim3(64), 0, s, sd, sw, o->tol);
+        }
+        CD_CHECK_LAUNCH(ctx);
+        CD_TRY(ctx->timer_end(CD_KERNEL_SOR_SWEEP, 2LL * count_sweeps,
+                              (mixed ? 12.0 : 24.0) * (double)n * count_sweeps));
+        return CD_OK;
+    };
+
+    std::vector<int> cum_at_check;
+    int check_id = 0;
+    int cum = 0;
+    CD_TRY(enqueue_check(check_id));
+    cum_at_check.push_back(0);
+    double prev_rel = ctx->host_status->rel;
+    int k = kmax;
+    if (adaptive) k = std::min(kmax, std::max(8, kmax / 4));
+    bool done = ctx->host_status->done != 0;
+    while (!done && cum < o->max_sweeps) {
+        const int nsw = std::min(k, o->max_sweeps - cum);
+        CD_TRY(enqueue_sweeps(cum, nsw));
+        cum += nsw;
+        ++check_id;
+        CD_TRY(enqueue_check(check_id));
+        cum_at_check.push_back(cum);
+        const SolveDev& hs = *ctx->host_status;
+        done = hs.done != 0;
+        if (!std::isfinite(hs.rel)) break;
+        if (adaptive && !done) {
+            // geometric rate estimate over the last batch -> sweeps still needed
+            double rel = hs.rel;
+            double rho = (prev_rel > 0 && rel > 0 && rel < prev_rel) ? std::pow(rel / prev_rel, 1.0 / nsw) : 1.0;
+            int want = kmax;
+            if (rho < 0.999999) {
+                double need = std::log(o->tol / rel) / std::log(rho);
+                want = (int)std::ceil(need * 1.05) + 2;
+            }
+            k = std::max(8, std::min(kmax, want));
+            prev_rel = rel;
+        }
+    }
+    const SolveDev hs = *ctx->host_status;
+    if (mask && o->max_sweeps > 0) {
+        hipLaunchKernelGGL(k_set_holes_nan, dim3(div_up((long long)n, CD_BLOCK)), dim3(CD_BLOCK), 0,
+                           s, phi, mask, n);
+        CD_CHECK_LAUNCH(ctx);
+    }
+    CD_HIP(ctx, hipStreamSynchronize(s));
+
+    int status = CD_OK;
+    if (!std::isfinite(hs.rel)) status = CD_ERR_NAN;
+    else if (!hs.done) status = CD_ERR_NOT_CONVERGED;
+    if (st_out) {
+        std::memset(st_out, 0, sizeof(*st_out));
+        st_out->status = status;
+        if (hs.done && o->stop == CD_STOP_MAX_UPDATE) st_out->sweeps = hs.sweeps_at_done;
+        else if (hs.done && hs.done_check >= 0 && hs.done_check < (int)cum_at_check.size())
+            st_out->sweeps = cum_at_check[hs.done_check];
+        else st_out->sweeps = cum;
+        st_out->checks = hs.checks;
+        st_out->rel_residual = hs.rel;
+        st_out->initial_rel_residual = hs.rel0;
+        st_out->last_max_update = hs.last_max_update;
+        st_out->elapsed_ms = std::chrono::duration<double, std::milli>(
+                                 std::chrono::steady_clock::now() - t_start).count();
+    }
+    return status;
+}
+
+extern "C" {
+
+int cd_poisson_solve_device(cd_context* ctx, const double* d_rhs, double* d_phi, int width,
+                            int height, const cd_solve_opts* opts, cd_solve_stats* stats) {
+    if (!ctx || !d_rhs || !d_phi || width < 2 || height < 2) return CD_ERR_BAD_ARG;
+    cd_solve_opts def;
+    if (!opts) { cd_solve_opts_default(&def); opts = &def; }
+    CD_HIP(ctx, hipSetDevice(ctx->device));
+    const size_t n = (size_t)width * height;
+    // hole mask: only materialised when the right-hand side contains NaN
+    CD_TRY(ctx->ensure(ctx->mask, n));
+    const dim3 rblk(RED_BX, RED_BY);
+    const dim3 rgrid(div_up(width, RED_BX), div_up(height, RED_BY));
+    const int nred = rgrid.x * rgrid.y;
+    CD_TRY(ctx->ensure(ctx->partials, sizeof(double) * 2 * (size_t)nred));
+    CD_TRY(ctx->ensure(ctx->fill, sizeof(double) * 4));
+    hipLaunchKernelGGL(k_build_mask, rgrid, rblk, 0, ctx->stream, d_rhs, ctx->mask.as<uint8_t>(),
+                       width, height, ctx->partials.as<double>());
+    CD_CHECK_LAUNCH(ctx);
+    CD_TRY(cd_reduce_partials(ctx, ctx->partials.as<double>(), nred, 1, ctx->fill.as<double>()));
+    double holes = 0.0;
+    CD_HIP(ctx, hipMemcpyAsync(&holes, ctx->fill.ptr, sizeof(double), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    CD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const uint8_t* mask = holes > 0 ? ctx->mask.as<uint8_t>() : nullptr;
+    return cd_solve_device_internal(ctx, d_rhs, d_phi, mask, (long long)(n - (size_t)holes), width,
+                                    height, opts, stats);
+}
+
+int cd_poisson_solve(cd_context* ctx, const double* rhs, double* phi, int width, int height,
+                     const cd_solve_opts* opts, cd_solve_stats* stats) {
+    if (!ctx || !rhs || !phi || width < 2 || height < 2) return CD_ERR_BAD_ARG;
+    CD_HIP(ctx, hipSetDevice(ctx->device));
+    const size_t n = (size_t)width * height;
+    CD_TRY(ctx->ensure(ctx->b64w, sizeof(double) * n));
+    CD_TRY(ctx->ensure(ctx->phi64w, sizeof(double) * n));
+    CD_HIP(ctx, hipMemcpyAsync(ctx->b64w.ptr, rhs, sizeof(double) * n, hipMemcpyHostToDevice,
+                               ctx->stream));
+    CD_HIP(ctx, hipMemcpyAsync(ctx->phi64w.ptr, phi, sizeof(double) * n, hipMemcpyHostToDevice,
+                               ctx->stream));
+    int st = cd_poisson_solve_device(ctx, ctx->b64w.as<double>(), ctx->phi64w.as<double>(), width,
+                                     height, opts, stats);
+    if (st != CD_OK && st != CD_ERR_NOT_CONVERGED && st != CD_ERR_NAN) return st;
+    CD_HIP(ctx, hipMemcpyAsync(phi, ctx->phi64w.ptr, sizeof(double) * n, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    CD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,194 @@
+// Caustic_design drop-in (src/caustic_design.h:7-66, src/caustic_design.cpp)
+// over the C ABI.  All state lives on the device; public members are host
+// mirrors refreshed after each call.
+#include <cstdio>
+#include <cstdlib>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "caustic_design.h"
+#include "cd_api.h"
+
+namespace {
+
+void check(cd_context* ctx, int rc, const char* what) {
+    if (rc == CD_OK) return;
+    throw std::runtime_error(std::string(what) + ": " + cd_status_string(rc) + ": " +
+                             (ctx ? cd_context_last_error(ctx) : ""));
+}
+
+std::vector<std::vector<double>> to_grid(const std::vector<double>& a, int w, int h) {
+    std::vector<std::vector<double>> g(h, std::vector<double>(w));
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) g[y][x] = a[(size_t)y * w + x];
+    return g;
+}
+
+}  // namespace
+
+// constructor defaults as caustic_design.cpp:3-14
+Caustic_design::Caustic_design()
+    : mesh(nullptr), mesh_res_x(0), mesh_res_y(0), resolution_x(0), resolution_y(0), width(0.0),
+      height(0.0), focal_l(0.0), thickness(0.0), nthreads(0) {}
+
+Caustic_design::~Caustic_design() {
+    cd_design_destroy(design_);
+    cd_context_destroy(ctx_);
+    delete mesh;
+}
+
+void Caustic_design::set_mesh_resolution(int w, int h) { mesh_res_x = w; mesh_res_y = h; }
+void Caustic_design::set_domain_resolution(int w, int h) { resolution_x = w; resolution_y = h; }
+void Caustic_design::set_mesh_size(double w, double h) { width = w; height = h; }
+void Caustic_design::set_lens_focal_length(double f) { focal_l = f; }
+void Caustic_design::set_lens_thickness(double t) { thickness = t; }
+void Caustic_design::set_solver_max_threads(int n) { nthreads = n; }
+void Caustic_design::set_host_mirroring(bool on) { mirror_ = on; }
+void Caustic_design::set_exit_on_miss(bool on) { exit_on_miss_ = on; }
+void Caustic_design::set_residual_stop(bool on, double rel_tol) {
+    residual_stop_ = on;
+    rel_tol_ = rel_tol;
+}
+
+void Caustic_design::initialize_solvers(std::vector<std::vector<double>> image) {
+    if (!ctx_) {
+        const char* env = std::getenv("CD_DEVICE");
+        check(nullptr, cd_context_create(env ? std::atoi(env) : 0, &ctx_), "cd_context_create");
+    }
+    cd_design_destroy(design_);
+    design_ = nullptr;
+    cd_design_params p;
+    cd_design_params_default(&p);
+    p.mesh_res_x = mesh_res_x;
+    p.mesh_res_y = mesh_res_y;
+    p.res_x = resolution_x;
+    p.res_y = resolution_y;
+    p.width = width;
+    p.height = height;
+    p.focal_l = focal_l;
+    p.thickness = thickness;
+    if (residual_stop_) {
+        p.phi_solve.stop = CD_STOP_RESIDUAL;
+        p.phi_solve.tol = rel_tol_;
+        p.phi_solve.omega = 0.0;
+        p.h_solve.stop = CD_STOP_RESIDUAL;
+        p.h_solve.tol = rel_tol_;
+        p.h_solve.omega = 0.0;
+    }
+    check(ctx_, cd_design_create(ctx_, &p, &design_), "cd_design_create");
+    if ((int)image.size() < resolution_y) throw std::invalid_argument("image rows < resolution_y");
+    std::vector<double> img((size_t)resolution_x * resolution_y);
+    for (int y = 0; y < resolution_y; ++y) {
+        if ((int)image[y].size() < resolution_x) throw std::invalid_argument("image row too short");
+        for (int x = 0; x < resolution_x; ++x) img[(size_t)y * resolution_x + x] = image[y][x];
+    }
+    check(ctx_, cd_design_init(design_, img.data()), "initialize_solvers");
+    delete mesh;
+    mesh = new Mesh();
+    mesh->width = width;
+    mesh->height = height;
+    mesh->res_x = mesh_res_x;
+    mesh->res_y = mesh_res_y;
+    for (int i = 0; i < mesh_res_y - 1; ++i)
+        for (int j = 0; j < mesh_res_x - 1; ++j) {
+            int idx = i * mesh_res_x + j;
+            mesh->triangles.push_back({idx, idx + 1, idx + mesh_res_x});
+            mesh->triangles.push_back({idx + mesh_res_x, idx + 1, idx + mesh_res_x + 1});
+        }
+    refresh_vertices();
+    std::vector<double> ta(cd_design_field_size(design_, CD_FIELD_TARGET_AREAS));
+    check(ctx_, cd_design_get(design_, CD_FIELD_TARGET_AREAS, ta.data(), ta.size()), "get");
+    target_areas = ta;
+    if (mirror_) {
+        std::vector<double> g(cd_design_field_size(design_, CD_FIELD_PIXELS));
+        check(ctx_, cd_design_get(design_, CD_FIELD_PIXELS, g.data(), g.size()), "get");
+        pixels = to_grid(g, resolution_x, resolution_y);
+        refresh_grids();
+    }
+}
+
+void Caustic_design::refresh_vertices() {
+    const size_t V = (size_t)mesh_res_x * mesh_res_y;
+    std::vector<double> t(3 * V), s(3 * V);
+    check(ctx_, cd_design_get(design_, CD_FIELD_TARGET_POINTS, t.data(), t.size()), "get");
+    check(ctx_, cd_design_get(design_, CD_FIELD_SOURCE_POINTS, s.data(), s.size()), "get");
+    mesh->target_points.assign(V, point_t(3));
+    mesh->source_points.assign(V, point_t(3));
+    for (size_t i = 0; i < V; ++i)
+        for (int k = 0; k < 3; ++k) {
+            mesh->target_points[i][k] = t[3 * i + k];
+            mesh->source_points[i][k] = s[3 * i + k];
+        }
+}
+
+void Caustic_design::refresh_grids() {
+    auto grid = [&](int field) {
+        std::vector<double> g(cd_design_field_size(design_, field));
+        check(ctx_, cd_design_get(design_, field, g.data(), g.size()), "get");
+        return to_grid(g, resolution_x, resolution_y);
+    };
+    phi = grid(CD_FIELD_PHI);
+    h = grid(CD_FIELD_H);
+    raster = grid(CD_FIELD_RASTER);
+    divergence = grid(CD_FIELD_DIVERGENCE);
+    norm_x = grid(CD_FIELD_NORM_X);
+    norm_y = grid(CD_FIELD_NORM_Y);
+    const size_t V = (size_t)mesh_res_x * mesh_res_y;
+    std::vector<double> v(V), v2(2 * V);
+    check(ctx_, cd_design_get(design_, CD_FIELD_ERRORS, v.data(), V), "get");
+    errors = v;
+    check(ctx_, cd_design_get(design_, CD_FIELD_VERTEX_GRADIENT, v2.data(), 2 * V), "get");
+    vertex_gradient.assign(2, std::vector<double>(V));
+    for (size_t i = 0; i < V; ++i) {
+        vertex_gradient[0][i] = v2[2 * i];
+        vertex_gradient[1][i] = v2[2 * i + 1];
+    }
+    check(ctx_, cd_design_get(design_, CD_FIELD_NORMALS, v2.data(), 2 * V), "get");
+    normals.assign(2, std::vector<double>(V));
+    for (size_t i = 0; i < V; ++i) {
+        normals[0][i] = v2[i];
+        normals[1][i] = v2[V + i];
+    }
+}
+
+double Caustic_design::perform_transport_iteration() {
+    if (!design_) throw std::logic_error("perform_transport_iteration before initialize_solvers");
+    double step = 0.0;
+    int rc = cd_design_transport_step(design_, &step);
+    if (rc == CD_ERR_RASTER_MISS && exit_on_miss_) {
+        std::printf("interpolation miss!\r\n");
+        std::exit(0);
+    }
+    check(ctx_, rc, "perform_transport_iteration");
+    refresh_vertices();
+    if (mirror_) refresh_grids();
+    return step;
+}
+
+void Caustic_design::perform_height_map_iteration(int /*itr*/) {
+    if (!design_) throw std::logic_error("perform_height_map_iteration before initialize_solvers");
+    double upd = 0.0;
+    int rc = cd_design_height_step(design_, &upd);
+    if (rc == CD_ERR_RASTER_MISS && exit_on_miss_) {
+        std::printf("interpolation miss!\r\n");
+        std::exit(0);
+    }
+    check(ctx_, rc, "perform_height_map_iteration");
+    std::printf("height max update %.5e\r\n", upd);
+    refresh_vertices();
+    if (mirror_) refresh_grids();
+}
+
+void Caustic_design::save_solid_obj_target(const std::string&) {
+    throw std::logic_error("save_solid_obj_target: file output is not part of this build");
+}
+void Caustic_design::save_solid_obj_source(const std::string&) {
+    throw std::logic_error("save_solid_obj_source: file output is not part of this build");
+}
+void Caustic_design::export_paramererization_to_svg(const std::string&, double) {
+    throw std::logic_error("export_paramererization_to_svg: file output is not part of this build");
+}
+void Caustic_design::export_inverted_transport_map(std::string, double) {
+    throw std::logic_error("export_inverted_transport_map: file output is not part of this build");
+}

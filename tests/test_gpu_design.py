@@ -1,0 +1,99 @@
+"""GPU parity of the design pipeline (Caustic_design through the C ABI)
+against the FP64 oracle restatement of caustic_design.cpp."""
+import numpy as np
+import pytest
+
+from oracle.bind import RED_BLACK, STOP_RESIDUAL, OracleDesign
+from poisson_caustic_design_amd import capi
+from tests.helpers import small_image
+
+pytestmark = pytest.mark.gpu
+
+
+def make_pair(ctx, oracle, M, tol=1e-12, precision=capi.PREC_MIXED, seed=3, W=0.5):
+    N = 4 * M
+    img = small_image(seed, N)
+    cd = capi.CausticDesign(ctx)
+    cd.set_mesh_resolution(M, M)
+    cd.set_domain_resolution(N, N)
+    cd.set_mesh_size(W, W)
+    cd.set_lens_focal_length(1.5)
+    cd.set_lens_thickness(0.1)
+    o = capi.default_opts(tol=tol, precision=precision)
+    cd.set_solve_opts(o, o)
+    cd.initialize_solvers(img)
+    od = OracleDesign(oracle, M, M, N, N, W, W, 1.5)
+    od.set_solver(STOP_RESIDUAL, tol, tol, 1000000, RED_BLACK, 10)
+    od.init(img)
+    return cd, od
+
+
+def test_init_parity(ctx, oracle):
+    cd, od = make_pair(ctx, oracle, 16)
+    assert np.array_equal(cd.pixels, od.grid(6))
+    ta = cd.target_areas
+    assert np.abs(ta - od.vec(2)).max() <= 1e-14 * np.abs(od.vec(2)).max()
+    tp = cd.get("target_points")
+    assert np.array_equal(tp[:, :2], od.vec(0)) and (tp[:, 2] == 0).all()
+
+
+@pytest.mark.parametrize("precision", [capi.PREC_MIXED, capi.PREC_FP64])
+def test_transport_steps_parity(ctx, oracle, precision):
+    cd, od = make_pair(ctx, oracle, 16, precision=precision)
+    W = 0.5
+    for it in range(4):
+        s_gpu = cd.perform_transport_iteration()
+        rc, s_orc = od.transport_step()
+        assert rc == 0
+        assert np.abs(cd.errors - od.vec(3)).max() < 1e-9, it
+        r = cd.raster
+        assert np.abs(r - od.grid(2)).max() < 1e-9 * np.abs(od.grid(2)).max(), it
+        tp = cd.get("target_points")[:, :2]
+        assert np.abs(tp - od.vec(0)).max() / W < 1e-10, it
+        assert s_gpu == pytest.approx(s_orc, rel=1e-6)
+        phi_s, _ = cd.last_solve_stats()
+        assert phi_s.rel_residual <= 1e-12
+
+
+def test_height_steps_parity(ctx, oracle):
+    cd, od = make_pair(ctx, oracle, 12)
+    for _ in range(3):
+        cd.perform_transport_iteration()
+        od.transport_step()
+    for it in range(3):
+        u_gpu = cd.perform_height_map_iteration(it)
+        rc, u_orc = od.height_step()
+        assert rc == 0
+        z = cd.get("source_points")[:, 2]
+        zo = od.vec(1)
+        assert np.abs(z - zo).max() <= 1e-9 * max(np.abs(zo).max(), 1e-30), it
+        # h is defined up to the Neumann constant: compare mean-free
+        h, ho = cd.h, od.grid(1)
+        assert np.abs((h - h.mean()) - (ho - ho.mean())).max() <= 1e-9 * np.abs(ho).max(), it
+        nx = cd.get("normals")
+        assert np.abs(nx[0] - od.vec(4)).max() <= 1e-12 * np.abs(od.vec(4)).max()
+
+
+def test_run_loop_matches_manual_steps(ctx, oracle):
+    a, _ = make_pair(ctx, oracle, 12, tol=1e-10)
+    b, _ = make_pair(ctx, oracle, 12, tol=1e-10)
+    rs = a.run(5, 0.0, 2)
+    assert rs.transport_steps == 5 and rs.height_steps == 2
+    for _ in range(5):
+        b.perform_transport_iteration()
+    for _ in range(2):
+        b.perform_height_map_iteration()
+    assert np.array_equal(a.get("target_points"), b.get("target_points"))
+    assert np.array_equal(a.get("source_points"), b.get("source_points"))
+
+
+def test_raster_miss_is_an_error_not_exit(ctx, oracle):
+    cd, _ = make_pair(ctx, oracle, 10)
+    tp = cd.get("target_points")
+    M = 10
+    # pull the bottom-right corner vertex inside: samples near that corner are uncovered
+    tp[M * M - 1, :2] = [0.5 * 0.9, 0.5 * 0.9]
+    cd.set("target_points", tp)
+    with pytest.raises(capi.CDError) as ei:
+        cd.perform_transport_iteration()
+    assert ei.value.status == capi.RASTER_MISS
