@@ -173,6 +173,7 @@ void cd_context_destroy(cd_context* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->e32.release();
+    ctx->e32b.release();
     ctx->r32.release();
     ctx->e64w.release();
     ctx->phi64w.release();

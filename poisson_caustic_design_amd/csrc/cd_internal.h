@@ -71,8 +71,8 @@ struct SolveDev {
     double rel0;         // first relative residual
     double last_max_update;
     float rmean32;       // mean of the last FP32 residual (projected out in sweeps)
-    unsigned int maxupd_bits[2];
-    int pad;
+    int cur;             // blocked path: ping-pong buffer holding the current correction
+    unsigned int maxupd_bits[16];  // per-sweep max |delta| (float bits) for the max-update rule
 };
 
 // An event pair recorded around a group of launches; resolved (elapsed time
@@ -96,7 +96,7 @@ struct cd_context {
     hipEvent_t open_event = nullptr;
 
     // solver workspace (grow-only)
-    DevBuf e32, r32, e64w, phi64w, b64w, mask, partials, solve_dev, fill;
+    DevBuf e32, e32b, r32, e64w, phi64w, b64w, mask, partials, solve_dev, fill;
     SolveDev* host_status = nullptr;  // pinned
     int ws_w = 0, ws_h = 0;
 

@@ -99,8 +99,8 @@ __global__ void k_rhs_finish(const double* __restrict__ partials, int nblocks, S
             st->rel0 = INFINITY;
             st->last_max_update = INFINITY;
             st->rmean32 = 0.f;
-            st->maxupd_bits[0] = 0u;
-            st->maxupd_bits[1] = 0u;
+            st->cur = 0;
+            for (int k = 0; k < 16; ++k) st->maxupd_bits[k] = 0u;
         } else {
             st->bnorm = sqrt(s);
         }
@@ -121,11 +121,34 @@ __global__ __launch_bounds__(256) void k_flush(double* __restrict__ phi, float* 
     }
 }
 
+// blocked path: fold the current ping-pong correction into phi64, zero both
+__global__ __launch_bounds__(256) void k_flush_packed(double* __restrict__ phi,
+                                                      float* __restrict__ ea,
+                                                      float* __restrict__ eb,
+                                                      const SolveDev* __restrict__ st, int w,
+                                                      int h) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= w || y >= h) return;
+    const size_t pi = ((size_t)((x + y) & 1) * h + y) * ((w + 1) / 2) + (x >> 1);
+    const float ev = st->cur ? eb[pi] : ea[pi];
+    phi[(size_t)y * w + x] += (double)ev;
+    ea[pi] = 0.f;
+    eb[pi] = 0.f;
+}
+
 // ---------------------------------------------------------------------------
 // FP64 residual r = (b - bmean) - A phi  (+ FP32 copy for the inner sweeps)
 // partials[block] = (sum r, sum r^2)
 // ---------------------------------------------------------------------------
-template <bool WRITE_R32>
+// Checkerboard-packed FP32 layout of the blocked sweeps: cell (X,Y) of colour
+// c = (X+Y)&1 lives at c*h*wp + Y*wp + (X>>1), wp = (w+1)/2, so every colour
+// is contiguous along a row both in HBM and in LDS.
+__device__ __forceinline__ size_t packed_index(int X, int Y, int h, int wp) {
+    return (size_t)((X + Y) & 1) * h * wp + (size_t)Y * wp + (X >> 1);
+}
+
+template <bool WRITE_R32, bool PACKED>
 __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ phi,
                                                   const double* __restrict__ b,
                                                   const uint8_t* __restrict__ mask,
@@ -160,7 +183,7 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ phi
             rs = r;
             rq = r * r;
         }
-        if (WRITE_R32) r32[i] = (float)r;
+        if (WRITE_R32) r32[PACKED ? packed_index(x, y, h, (w + 1) / 2) : i] = (float)r;
     }
     double a = cd_block_sum(rs, smem);
     double q = cd_block_sum(rq, smem);
@@ -256,6 +279,221 @@ __global__ __launch_bounds__(256) void k_rb_sweep(T* __restrict__ e, const T* __
     }
 }
 
+// ---------------------------------------------------------------------------
+// Temporally blocked red-black SOR (FP32, packed layout, no holes).
+//
+// A workgroup of 4 waves owns a region of 4*R rows x 128 columns; the halo is
+// H cells on every side.  Lane l of wave v keeps packed column l (cells
+// 2l and 2l+1 of every row, one per colour) of rows v*R .. v*R+R-1 and their
+// right-hand side in registers.  Up/down neighbours of a cell are the other
+// colour in the rows above/below: registers of the same lane, except at a
+// wave's first/last row, which comes through a small LDS exchange.  Left/right
+// neighbours are the other colour in the same row at packed column l-1+p or
+// l+p (p = column parity of the cell): the same lane or the adjacent lane,
+// fetched with a DPP wave shift.  Half-sweep s (1-based) updates the cells at
+// distance >= s from the region border, whose neighbours are exact at that
+// point; after nhalf <= H half-sweeps the interior (distance >= H) equals
+// nhalf global half-sweeps and is written to the other ping-pong buffer.
+// ---------------------------------------------------------------------------
+constexpr int BK_RX = 128;   // region columns (64 packed columns, one per lane)
+constexpr int BK_R = 16;     // rows per wave
+constexpr int BK_RY = 4 * BK_R;
+
+// value of lane l-1 (0 in lane 0) / lane l+1 (0 in lane 63): DPP wave shifts
+__device__ __forceinline__ float lane_prev(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float lane_next(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, true));
+}
+
+template <int C, int H, bool EDGE, bool TRACK>
+__device__ __forceinline__ void blocked_half(float (&ev)[2][BK_R], const float (&rv)[2][BK_R],
+                                             float up_row, float down_row, int s, int dx0,
+                                             int dx1, int dyw, bool l0, bool l1, bool r0, bool r1,
+                                             bool in0, bool in1, int Yw, int h, float om2,
+                                             float om3, float om4, float& mx) {
+#pragma unroll
+    for (int k = 0; k < BK_R; ++k) {
+        constexpr int dummy = 0;
+        (void)dummy;
+        const int p = (k + C) & 1;  // column parity of colour-C cells in row k (R even)
+        const float o = ev[1 - C][k];
+        const float left = p ? o : lane_prev(o);
+        const float right = p ? lane_next(o) : o;
+        const float up = k > 0 ? ev[1 - C][k - 1] : up_row;
+        const float down = k < BK_R - 1 ? ev[1 - C][k + 1] : down_row;
+        const int dy = min(dyw + k, BK_RY - 1 - (dyw + k));
+        const int dx = p ? dx1 : dx0;
+        bool upd = dx >= s && dy >= s;
+        float sum;
+        float oc = om4;
+        float cnt = 4.f;
+        if (EDGE) {
+            const int Y = Yw + k;
+            const bool lu = Y > 0, ld = Y < h - 1;
+            const bool ll = p ? l1 : l0, lr = p ? r1 : r0;
+            upd = upd && (p ? in1 : in0) && Y >= 0 && Y < h;
+            sum = (ll ? left : 0.f) + (lu ? up : 0.f) + (lr ? right : 0.f) + (ld ? down : 0.f);
+            const int n = (int)ll + (int)lu + (int)lr + (int)ld;
+            cnt = (float)n;
+            oc = n == 4 ? om4 : (n == 3 ? om3 : om2);
+        } else {
+            sum = left + up + right + down;
+        }
+        const float v = ev[C][k];
+        const float dl = oc * (sum - cnt * v - rv[C][k]);
+        ev[C][k] = upd ? v + dl : v;
+        if (TRACK && upd && dx >= H && dy >= H) mx = fmaxf(mx, fabsf(dl));
+    }
+}
+
+template <int C, int H, bool EDGE, bool TRACK>
+__device__ __forceinline__ void blocked_step(float (&ev)[2][BK_R], const float (&rv)[2][BK_R],
+                                             float (*xch)[4][2][64], int lane, int wave, int s,
+                                             int dx0, int dx1, int dyw, bool l0, bool l1, bool r0,
+                                             bool r1, bool in0, bool in1, int Yw, int h,
+                                             float om2, float om3, float om4, float& mx) {
+    // publish this wave's first/last row of the colour the update reads
+    xch[1 - C][wave][0][lane] = ev[1 - C][0];
+    xch[1 - C][wave][1][lane] = ev[1 - C][BK_R - 1];
+    __syncthreads();
+    const float up_row = xch[1 - C][wave > 0 ? wave - 1 : 0][1][lane];
+    const float down_row = xch[1 - C][wave < 3 ? wave + 1 : 3][0][lane];
+    blocked_half<C, H, EDGE, TRACK>(ev, rv, up_row, down_row, s, dx0, dx1, dyw, l0, l1, r0, r1,
+                                    in0, in1, Yw, h, om2, om3, om4, mx);
+}
+
+template <int H, bool EDGE, bool TRACK>
+__device__ __forceinline__ void blocked_run(float (&ev)[2][BK_R], const float (&rv)[2][BK_R],
+                                            float (*xch)[4][2][64], int lane, int wave, int nhalf,
+                                            int dx0, int dx1, int dyw, bool l0, bool l1, bool r0,
+                                            bool r1, bool in0, bool in1, int Yw, int h, float om2,
+                                            float om3, float om4, float (&mxs)[H / 2]) {
+#pragma unroll
+    for (int j = 0; j < H / 2; ++j) {
+        const int s = 2 * j + 1;
+        if (s > nhalf) break;
+        float mx = 0.f;
+        blocked_step<0, H, EDGE, TRACK>(ev, rv, xch, lane, wave, s, dx0, dx1, dyw, l0, l1, r0, r1,
+                                        in0, in1, Yw, h, om2, om3, om4, mx);
+        if (s + 1 <= nhalf)
+            blocked_step<1, H, EDGE, TRACK>(ev, rv, xch, lane, wave, s + 1, dx0, dx1, dyw, l0, l1,
+                                            r0, r1, in0, in1, Yw, h, om2, om3, om4, mx);
+        mxs[j] = mx;
+    }
+}
+
+template <int H, bool TRACK>
+__global__ __launch_bounds__(256) void k_rb_blocked(const float* __restrict__ ein,
+                                                    float* __restrict__ eout,
+                                                    const float* __restrict__ r, int w, int h,
+                                                    int tiles_x, int ntiles, int nhalf,
+                                                    float om2, float om3, float om4,
+                                                    const float* __restrict__ rmean,
+                                                    SolveDev* st, int out_index) {
+    static_assert(H % 2 == 0 && H >= 2 && 2 * H < BK_R * 2 && BK_R % 2 == 0, "geometry");
+    if (st->done) return;
+    __shared__ float xch[2][4][2][64];
+    __shared__ float smax[4][H / 2];
+    // XCD-aware tile order: gridDim.x is a multiple of 8 and consecutive
+    // workgroups go round-robin to the 8 XCDs; give each XCD a contiguous band
+    // of tiles so halo re-reads hit its own L2.
+    const int per = gridDim.x >> 3;
+    const int tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (tile >= ntiles) return;
+    constexpr int IX = BK_RX - 2 * H, IY = BK_RY - 2 * H;
+    const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
+    const int X0 = tx * IX - H, Y0 = ty * IY - H;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wp = (w + 1) / 2;
+    const float rm = *rmean;
+    const int Yw = Y0 + BK_R * wave;  // first global row of this wave
+    const int dyw = BK_R * wave;      // first region row of this wave
+    // per-lane column facts for the two cells of packed column `lane`
+    const int X0l = X0 + 2 * lane;
+    const int dx0 = min(2 * lane, BK_RX - 1 - 2 * lane);
+    const int dx1 = min(2 * lane + 1, BK_RX - 2 - 2 * lane);
+    const bool in0 = (unsigned)X0l < (unsigned)w, in1 = (unsigned)(X0l + 1) < (unsigned)w;
+    const bool l0 = X0l > 0, l1 = X0l + 1 > 0;
+    const bool r0 = X0l < w - 1, r1 = X0l + 1 < w - 1;
+    // packed column of both cells, clamped into the row: every load below is
+    // a uniform row pointer plus this one 32-bit lane offset
+    const unsigned col = (unsigned)min(max(X0 / 2 + lane, 0), wp - 1);
+    float ev[2][BK_R], rv[2][BK_R];
+#pragma unroll
+    for (int k = 0; k < BK_R; ++k) {
+        const int Y = Yw + k;
+        const bool iny = (unsigned)Y < (unsigned)h;
+        const int Yc = min(max(Y, 0), h - 1);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int p = (k + c) & 1;
+            const bool in = iny && (p ? in1 : in0);
+            const size_t row = ((size_t)c * h + Yc) * wp;
+            const float a = ein[row + col];
+            const float b = r[row + col];
+            ev[c][k] = in ? a : 0.f;
+            rv[c][k] = in ? b - rm : 0.f;
+        }
+    }
+    float mxs[H / 2];
+#pragma unroll
+    for (int j = 0; j < H / 2; ++j) mxs[j] = 0.f;
+    const bool edge = X0 < 1 || X0 + BK_RX > w - 1 || Y0 < 1 || Y0 + BK_RY > h - 1;
+    if (edge)
+        blocked_run<H, true, TRACK>(ev, rv, xch, lane, wave, nhalf, dx0, dx1, dyw, l0, l1, r0, r1,
+                                    in0, in1, Yw, h, om2, om3, om4, mxs);
+    else
+        blocked_run<H, false, TRACK>(ev, rv, xch, lane, wave, nhalf, dx0, dx1, dyw, l0, l1, r0,
+                                     r1, in0, in1, Yw, h, om2, om3, om4, mxs);
+    // write the interior
+#pragma unroll
+    for (int k = 0; k < BK_R; ++k) {
+        const int Y = Yw + k;
+        const int dy = min(dyw + k, BK_RY - 1 - (dyw + k));
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int p = (k + c) & 1;
+            const int dx = p ? dx1 : dx0;
+            if (dx >= H && dy >= H && (unsigned)Y < (unsigned)h && (p ? in1 : in0))
+                eout[((size_t)c * h + Y) * wp + col] = ev[c][k];
+        }
+    }
+    if (TRACK) {
+#pragma unroll
+        for (int j = 0; j < H / 2; ++j) {
+            const float m = cd_wave_max(mxs[j]);
+            if (lane == 0) smax[wave][j] = m;
+        }
+        __syncthreads();
+        if (threadIdx.x < H / 2 && 2 * (int)threadIdx.x < nhalf) {
+            const int j = threadIdx.x;
+            const float m = fmaxf(fmaxf(smax[0][j], smax[1][j]), fmaxf(smax[2][j], smax[3][j]));
+            atomicMax(&st->maxupd_bits[j], __float_as_uint(m));
+        }
+    }
+    if (threadIdx.x == 0) st->cur = out_index;  // same value from every workgroup
+}
+
+// max-update stop rule over the sweeps of one blocked launch (solver.cpp:142)
+__global__ void k_maxcheck_blocked(SolveDev* st, int first_sweep, int nsweeps, double tol) {
+    if (threadIdx.x != 0) return;
+    if (!st->done) {
+        for (int j = 0; j < nsweeps; ++j) {
+            const float mu = __uint_as_float(st->maxupd_bits[j]);
+            st->last_max_update = (double)mu;
+            if ((double)mu < tol) {
+                st->done = 1;
+                st->sweeps_at_done = first_sweep + j + 1;
+                break;
+            }
+        }
+    }
+    for (int j = 0; j < 16; ++j) st->maxupd_bits[j] = 0u;
+}
+
 // max-update stop rule after the black half of sweep `sweep` (solver.cpp:142)
 __global__ void k_maxcheck(SolveDev* st, int sweep, double tol) {
     if (threadIdx.x != 0) return;
@@ -320,13 +558,19 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     const int nred = rgrid.x * rgrid.y;
     CD_TRY(ctx->ensure(ctx->partials, sizeof(double) * 2 * (size_t)nred));
     CD_TRY(ctx->ensure(ctx->solve_dev, sizeof(SolveDev)));
+    // FP32 sweeps without holes take the temporally blocked kernel on the
+    // packed layout; FP64 sweeps and hole masks take the per-half-sweep kernel.
+    const bool blocked = mixed && mask == nullptr;
+    const size_t npk = 2 * (size_t)h * ((w + 1) / 2);
     if (mixed) {
-        CD_TRY(ctx->ensure(ctx->e32, sizeof(float) * n));
-        CD_TRY(ctx->ensure(ctx->r32, sizeof(float) * n));
+        CD_TRY(ctx->ensure(ctx->e32, sizeof(float) * (blocked ? npk : n)));
+        CD_TRY(ctx->ensure(ctx->r32, sizeof(float) * (blocked ? npk : n)));
+        if (blocked) CD_TRY(ctx->ensure(ctx->e32b, sizeof(float) * npk));
     }
     double* partials = ctx->partials.as<double>();
     SolveDev* sd = ctx->solve_dev.as<SolveDev>();
     float* e32 = mixed ? ctx->e32.as<float>() : nullptr;
+    float* e32b = blocked ? ctx->e32b.as<float>() : nullptr;
     float* r32 = mixed ? ctx->r32.as<float>() : nullptr;
 
     const double omega = cd_sor_omega(o->omega, w, h);
@@ -341,24 +585,29 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     hipLaunchKernelGGL(k_rhs_moments, rgrid, rblk, 0, s, b, w, h, sd, 1, partials);
     hipLaunchKernelGGL(k_rhs_finish, dim3(1), dim3(1024), 0, s, partials, nred, sd, 1, count);
     CD_CHECK_LAUNCH(ctx);
-    if (mixed) CD_TRY(cd_fill_f32(ctx, e32, n, 0.f));
+    if (mixed) CD_TRY(cd_fill_f32(ctx, e32, blocked ? npk : n, 0.f));
+    if (blocked) CD_TRY(cd_fill_f32(ctx, e32b, npk, 0.f));
 
     const int fblocks = std::min(div_up((long long)n, CD_BLOCK), 65536);
     auto enqueue_check = [&](int check_id) -> int {
         CD_TRY(ctx->timer_begin());
-        if (mixed) {
+        if (blocked) {
+            hipLaunchKernelGGL(k_flush_packed, rgrid, rblk, 0, s, phi, e32, e32b, sd, w, h);
+            hipLaunchKernelGGL((k_residual<true, true>), rgrid, rblk, 0, s, phi, b, mask, sd, r32,
+                               w, h, partials);
+        } else if (mixed) {
             hipLaunchKernelGGL(k_flush, dim3(fblocks), dim3(CD_BLOCK), 0, s, phi, e32, n);
-            hipLaunchKernelGGL(k_residual<true>, rgrid, rblk, 0, s, phi, b, mask, sd, r32, w, h,
-                               partials);
+            hipLaunchKernelGGL((k_residual<true, false>), rgrid, rblk, 0, s, phi, b, mask, sd, r32,
+                               w, h, partials);
         } else {
-            hipLaunchKernelGGL(k_residual<false>, rgrid, rblk, 0, s, phi, b, mask, sd, r32, w, h,
-                               partials);
+            hipLaunchKernelGGL((k_residual<false, false>), rgrid, rblk, 0, s, phi, b, mask, sd,
+                               r32, w, h, partials);
         }
         hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, partials, nred, sd, count,
                            check_id, o->stop, o->tol);
         CD_CHECK_LAUNCH(ctx);
         CD_TRY(ctx->timer_end(CD_KERNEL_RESIDUAL, mixed ? 3 : 2,
-                              (mixed ? 44.0 : 16.0) * (double)n));
+                              (blocked ? 48.0 : (mixed ? 44.0 : 16.0)) * (double)n));
         CD_HIP(ctx, hipMemcpyAsync(ctx->host_status, sd, sizeof(SolveDev), hipMemcpyDeviceToHost, s));
         CD_HIP(ctx, hipStreamSynchronize(s));
         CD_TRY(ctx->resolve_timers());
@@ -367,7 +616,44 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
 
     const dim3 sblk(SWEEP_BX, SWEEP_BY);
     const dim3 sgrid(div_up(div_up(w, 2), SWEEP_BX), div_up(h, SWEEP_BY));
+    // blocked geometry
+    constexpr int BH = 8;  // halo = half-sweeps per launch
+    const int bix = BK_RX - 2 * BH, biy = BK_RY - 2 * BH;
+    const int tiles_x = div_up(w, bix), tiles_y = div_up(h, biy);
+    const int ntiles = tiles_x * tiles_y;
+    const int bgrid = div_up(ntiles, 8) * 8;
+    const float om = (float)omega;
+    const float om2 = om / 2.f, om3 = om / 3.f, om4 = om / 4.f;
+    auto enqueue_blocked = [&](int first, int count_sweeps) -> int {
+        CD_TRY(ctx->timer_begin());
+        int launches = 0;
+        for (int done_sw = 0; done_sw < count_sweeps;) {
+            const int nsw = std::min(BH / 2, count_sweeps - done_sw);
+            const int in = launches & 1;
+            float* ein = in ? e32b : e32;
+            float* eout = in ? e32 : e32b;
+            if (track) {
+                hipLaunchKernelGGL((k_rb_blocked<BH, true>), dim3(bgrid), dim3(256), 0, s, ein,
+                                   eout, r32, w, h, tiles_x, ntiles, 2 * nsw, om2, om3, om4,
+                                   &sd->rmean32, sd, 1 - in);
+                hipLaunchKernelGGL(k_maxcheck_blocked, dim3(1), dim3(64), 0, s, sd,
+                                   first + done_sw, nsw, o->tol);
+            } else {
+                hipLaunchKernelGGL((k_rb_blocked<BH, false>), dim3(bgrid), dim3(256), 0, s, ein,
+                                   eout, r32, w, h, tiles_x, ntiles, 2 * nsw, om2, om3, om4,
+                                   &sd->rmean32, sd, 1 - in);
+            }
+            ++launches;
+            done_sw += nsw;
+        }
+        CD_CHECK_LAUNCH(ctx);
+        // algorithmic bytes: 12 B per cell per red+black sweep (e read, e write,
+        // rhs read), whatever the number of sweeps fused into one launch
+        CD_TRY(ctx->timer_end(CD_KERNEL_SOR_SWEEP, launches, 12.0 * (double)n * count_sweeps));
+        return CD_OK;
+    };
     auto enqueue_sweeps = [&](int first, int count_sweeps) -> int {
+        if (blocked) return enqueue_blocked(first, count_sweeps);
         CD_TRY(ctx->timer_begin());
         for (int k = 0; k < count_sweeps; ++k) {
             const int sw = first + k;

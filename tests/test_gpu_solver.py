@@ -31,6 +31,23 @@ def test_fp64_red_black_iterates_match_oracle(ctx, oracle, rng):
         assert np.abs(g - ref).max() <= 1e-12 * np.abs(ref).max(), (w, h)
 
 
+@pytest.mark.parametrize("w,h,k,every", [(64, 64, 13, 5), (300, 130, 9, 4), (129, 77, 22, 7),
+                                         (1000, 50, 6, 6)])
+def test_fp32_blocked_iterates_match_oracle(ctx, oracle, rng, w, h, k, every):
+    """Mixed precision (temporally blocked FP32 sweeps, FP64 flushes every
+    `every` sweeps): after k sweeps the iterate equals k red-black sweeps of the
+    oracle to FP32 accuracy, across launch and tile boundaries."""
+    b = rng.standard_normal((h, w)); b -= b.mean()
+    p0 = 0.1 * rng.standard_normal((h, w))
+    o = capi.default_opts(max_sweeps=k, stop=capi.STOP_MAX_UPDATE, tol=1e-300,
+                          precision=capi.PREC_MIXED, omega=-1.0, check_every=every)
+    g, st = ctx.poisson_solve(b, p0, o)
+    ref, _ = oracle.poisson_sor(b, p0, k, max_update_tol=1e-300, ordering=RED_BLACK)
+    assert st.sweeps == k
+    scale = np.abs(ref - p0).max()
+    assert np.abs(g - ref).max() <= 2e-5 * scale, (w, h, np.abs(g - ref).max(), scale)
+
+
 @pytest.mark.parametrize("precision", [capi.PREC_MIXED, capi.PREC_FP64])
 @pytest.mark.parametrize("w,h", [(256, 256), (129, 77), (64, 200)])
 def test_solve_to_tolerance_matches_dct(ctx, rng, precision, w, h):
