@@ -66,6 +66,67 @@ def cpu_baseline(config: str, seed: int, per_design: dict, timeout: int) -> dict
             "components": m}
 
 
+PROFILE_SUMMARY = os.path.join(ROOT, "profiles", "k_rb_blocked_design_512mesh_2048grid.md")
+
+
+def profiled_traffic(grid: int):
+    """HBM bytes per k_rb_blocked launch at 2048^2 from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (counter runs cannot happen inside the timed
+    benchmark); None for other grids or when the summary is absent."""
+    if grid != 2048 or not os.path.exists(PROFILE_SUMMARY):
+        return None
+    for line in open(PROFILE_SUMMARY):
+        if line.startswith("* traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def fixed_sweeps(a) -> int:
+    """Exactly a.sweeps red-black sweeps (mixed precision) on a seeded smooth
+    right-hand side of the workload grid size, device-resident."""
+    import numpy as np
+    import torch
+
+    from poisson_caustic_design_amd import capi, synth
+
+    _, prm = synth.target_for(a.config, a.seed, a.mesh_res)
+    n = prm["res"]
+    rng = np.random.default_rng(a.seed)
+    yy, xx = np.mgrid[0:n, 0:n] / n
+    b = np.cos(3 * np.pi * xx) * np.cos(2 * np.pi * yy) + 0.1 * rng.standard_normal((n, n))
+    b -= b.mean()
+    ctx = capi.Context(0)
+    d_b = torch.tensor(b, dtype=torch.float64, device="cuda:0")
+    d_phi = torch.zeros_like(d_b)
+    torch.cuda.synchronize()
+    # residual rule with tol 0 never stops early: exactly a.sweeps sweeps of the
+    # production (non-tracking) kernel, one FP64 check at the end
+    o = capi.default_opts(max_sweeps=a.sweeps, stop=capi.STOP_RESIDUAL, tol=0.0,
+                          check_every=a.sweeps)
+    ctx.set_profiling(True)
+    st = ctx.poisson_solve_device(d_b.data_ptr(), d_phi.data_ptr(), n, n, o)
+    ms, launches, nbytes = ctx.kernel_stats(capi.KERNEL_SOR_SWEEP)
+    print(json.dumps({"mode": "sweeps", "grid": n, "sweeps": st.sweeps, "sor_ms": ms,
+                      "launches": launches, "avg_launch_us": 1e3 * ms / max(launches, 1),
+                      "bytes_per_launch": nbytes / max(launches, 1),
+                      "achieved_GBs": nbytes / max(ms, 1e-9) / 1e6}), flush=True)
+    return 0
+
+
+def device_copy(a) -> int:
+    """A plain device-to-device copy kernel of a.mib MiB (read + write)."""
+    import torch
+
+    n = a.mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"mode": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n}), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,7 +139,17 @@ def main(argv=None):
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-timeout", type=int, default=240)
+    ap.add_argument("--mode", choices=["design", "sweeps", "copy"], default="design",
+                    help="design: the benchmark; sweeps: a fixed number of SOR sweeps at the "
+                         "workload grid (profiling); copy: a device copy of known size "
+                         "(calibrates rocprofv3 byte counters)")
+    ap.add_argument("--sweeps", type=int, default=64)
+    ap.add_argument("--mib", type=int, default=256)
     a = ap.parse_args(argv)
+    if a.mode == "sweeps":
+        return fixed_sweeps(a)
+    if a.mode == "copy":
+        return device_copy(a)
 
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
@@ -174,13 +245,14 @@ def main(argv=None):
             "per_design": per_design,
         },
         "roofline": {
-            "kernel": "k_rb_sweep (red-black SOR half-sweep)",
+            "kernel": "k_rb_blocked (temporally blocked red-black SOR, 4 sweeps per launch)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": profiled_traffic(N),
+            "traffic_source": os.path.relpath(PROFILE_SUMMARY, ROOT) if profiled_traffic(N) else None,
             "avg_launch_us": avg_launch_ms * 1e3,
             "bytes_per_launch": bytes_per_launch,
             "launches": sor_n,

@@ -43,23 +43,26 @@ def _run(cmd):
     return r
 
 
-def build(verbose: bool = False) -> str:
-    os.makedirs(BUILD, exist_ok=True)
+def build(verbose: bool = False, defines=(), out: str = LIB) -> str:
+    """Compile and link; `defines` are extra -D flags (kernel-geometry variants),
+    `out` the library path."""
+    build_dir = BUILD if out == LIB else out + ".objs"
+    os.makedirs(build_dir, exist_ok=True)
     hipcc = _hipcc()
     common = ["-O3", "-fPIC", "-std=c++17", "-I" + INCLUDE, "-I" + CSRC, "-Wall",
               "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-result"]
     hip_flags = ["-x", "hip", "--offload-arch=" + ARCH, "-ffp-contract=off",
                  "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-                 "-munsafe-fp-atomics"]
+                 "-munsafe-fp-atomics", *defines]
     jobs = []
     for src in HIP_SOURCES:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue
-        obj = os.path.join(BUILD, src + ".o")
+        obj = os.path.join(build_dir, src + ".o")
         jobs.append([hipcc, *common, *hip_flags, "-c", path, "-o", obj])
     for src in CXX_SOURCES:
-        obj = os.path.join(BUILD, src + ".o")
+        obj = os.path.join(build_dir, src + ".o")
         jobs.append(["g++", *common, "-c", os.path.join(CSRC, src), "-o", obj])
     with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         results = list(ex.map(_run, jobs))
@@ -67,11 +70,11 @@ def build(verbose: bool = False) -> str:
         for r in results:
             sys.stdout.write(r.stdout + r.stderr)
     objs = [j[-1] for j in jobs]
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     _run([hipcc, "-shared", "-fPIC", "--offload-arch=" + ARCH, *objs, "-o", tmp,
           "-Wl,-soname,libcaustic_amd.so"])
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

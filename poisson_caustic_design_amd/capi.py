@@ -21,7 +21,9 @@ from typing import Optional
 
 import numpy as np
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcaustic_amd.so")
+LIB_PATH = os.environ.get(
+    "CD_LIB_PATH",
+    os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcaustic_amd.so"))
 
 _D, _I, _P, _SZ, _LL = C.c_double, C.c_int, C.c_void_p, C.c_size_t, C.c_longlong
 

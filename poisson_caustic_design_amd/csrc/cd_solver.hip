@@ -22,6 +22,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "cd_internal.h"
 
@@ -295,9 +296,39 @@ __global__ __launch_bounds__(256) void k_rb_sweep(T* __restrict__ e, const T* __
 // point; after nhalf <= H half-sweeps the interior (distance >= H) equals
 // nhalf global half-sweeps and is written to the other ping-pong buffer.
 // ---------------------------------------------------------------------------
-constexpr int BK_RX = 128;   // region columns (64 packed columns, one per lane)
-constexpr int BK_R = 16;     // rows per wave
+// Geometry is fixed at compile time; CD_BK_R / CD_BK_H / CD_BK_RV_LDS exist to
+// compare variants on the hardware.
+#ifndef CD_BK_R
+#define CD_BK_R 16
+#endif
+#ifndef CD_BK_H
+#define CD_BK_H 8
+#endif
+#ifndef CD_BK_RV_LDS
+#define CD_BK_RV_LDS 1
+#endif
+constexpr int BK_RX = 128;      // region columns (64 packed columns, one per lane)
+constexpr int BK_R = CD_BK_R;   // rows per wave
 constexpr int BK_RY = 4 * BK_R;
+constexpr bool BK_RV_LDS = CD_BK_RV_LDS != 0;  // right-hand side in LDS (else registers)
+
+// Right-hand side of the region: registers of the owning thread, or LDS
+// (frees VGPRs for occupancy).  Element (colour c, region row y, lane).
+struct RhsRegs {
+    float v[2][BK_R];
+    float (*lds)[BK_RY][64];  // unused
+    __device__ __forceinline__ float get(int c, int k, int, int) const { return v[c][k]; }
+    __device__ __forceinline__ void set(int c, int k, int, int, float x) { v[c][k] = x; }
+};
+struct RhsLds {
+    float (*lds)[BK_RY][64];
+    __device__ __forceinline__ float get(int c, int k, int dyw, int lane) const {
+        return lds[c][dyw + k][lane];
+    }
+    __device__ __forceinline__ void set(int c, int k, int dyw, int lane, float x) {
+        lds[c][dyw + k][lane] = x;
+    }
+};
 
 // value of lane l-1 (0 in lane 0) / lane l+1 (0 in lane 63): DPP wave shifts
 __device__ __forceinline__ float lane_prev(float v) {
@@ -307,8 +338,8 @@ __device__ __forceinline__ float lane_next(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, true));
 }
 
-template <int C, int H, bool EDGE, bool TRACK>
-__device__ __forceinline__ void blocked_half(float (&ev)[2][BK_R], const float (&rv)[2][BK_R],
+template <int C, int H, bool EDGE, bool TRACK, typename RV>
+__device__ __forceinline__ void blocked_half(float (&ev)[2][BK_R], const RV& rv, int lane,
                                              float up_row, float down_row, int s, int dx0,
                                              int dx1, int dyw, bool l0, bool l1, bool r0, bool r1,
                                              bool in0, bool in1, int Yw, int h, float om2,
@@ -342,14 +373,14 @@ __device__ __forceinline__ void blocked_half(float (&ev)[2][BK_R], const float (
             sum = left + up + right + down;
         }
         const float v = ev[C][k];
-        const float dl = oc * (sum - cnt * v - rv[C][k]);
+        const float dl = oc * (sum - cnt * v - rv.get(C, k, dyw, lane));
         ev[C][k] = upd ? v + dl : v;
         if (TRACK && upd && dx >= H && dy >= H) mx = fmaxf(mx, fabsf(dl));
     }
 }
 
-template <int C, int H, bool EDGE, bool TRACK>
-__device__ __forceinline__ void blocked_step(float (&ev)[2][BK_R], const float (&rv)[2][BK_R],
+template <int C, int H, bool EDGE, bool TRACK, typename RV>
+__device__ __forceinline__ void blocked_step(float (&ev)[2][BK_R], const RV& rv,
                                              float (*xch)[4][2][64], int lane, int wave, int s,
                                              int dx0, int dx1, int dyw, bool l0, bool l1, bool r0,
                                              bool r1, bool in0, bool in1, int Yw, int h,
@@ -360,12 +391,12 @@ __device__ __forceinline__ void blocked_step(float (&ev)[2][BK_R], const float (
     __syncthreads();
     const float up_row = xch[1 - C][wave > 0 ? wave - 1 : 0][1][lane];
     const float down_row = xch[1 - C][wave < 3 ? wave + 1 : 3][0][lane];
-    blocked_half<C, H, EDGE, TRACK>(ev, rv, up_row, down_row, s, dx0, dx1, dyw, l0, l1, r0, r1,
-                                    in0, in1, Yw, h, om2, om3, om4, mx);
+    blocked_half<C, H, EDGE, TRACK>(ev, rv, lane, up_row, down_row, s, dx0, dx1, dyw, l0, l1, r0,
+                                    r1, in0, in1, Yw, h, om2, om3, om4, mx);
 }
 
-template <int H, bool EDGE, bool TRACK>
-__device__ __forceinline__ void blocked_run(float (&ev)[2][BK_R], const float (&rv)[2][BK_R],
+template <int H, bool EDGE, bool TRACK, typename RV>
+__device__ __forceinline__ void blocked_run(float (&ev)[2][BK_R], const RV& rv,
                                             float (*xch)[4][2][64], int lane, int wave, int nhalf,
                                             int dx0, int dx1, int dyw, bool l0, bool l1, bool r0,
                                             bool r1, bool in0, bool in1, int Yw, int h, float om2,
@@ -421,7 +452,10 @@ __global__ __launch_bounds__(256) void k_rb_blocked(const float* __restrict__ ei
     // packed column of both cells, clamped into the row: every load below is
     // a uniform row pointer plus this one 32-bit lane offset
     const unsigned col = (unsigned)min(max(X0 / 2 + lane, 0), wp - 1);
-    float ev[2][BK_R], rv[2][BK_R];
+    float ev[2][BK_R];
+    __shared__ float rlds[BK_RV_LDS ? 2 : 1][BK_RV_LDS ? BK_RY : 1][64];
+    typename std::conditional<BK_RV_LDS, RhsLds, RhsRegs>::type rv;
+    if constexpr (BK_RV_LDS) rv.lds = reinterpret_cast<float (*)[BK_RY][64]>(&rlds[0][0][0]);
 #pragma unroll
     for (int k = 0; k < BK_R; ++k) {
         const int Y = Yw + k;
@@ -435,7 +469,7 @@ __global__ __launch_bounds__(256) void k_rb_blocked(const float* __restrict__ ei
             const float a = ein[row + col];
             const float b = r[row + col];
             ev[c][k] = in ? a : 0.f;
-            rv[c][k] = in ? b - rm : 0.f;
+            rv.set(c, k, dyw, lane, in ? b - rm : 0.f);
         }
     }
     float mxs[H / 2];
@@ -617,7 +651,7 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     const dim3 sblk(SWEEP_BX, SWEEP_BY);
     const dim3 sgrid(div_up(div_up(w, 2), SWEEP_BX), div_up(h, SWEEP_BY));
     // blocked geometry
-    constexpr int BH = 8;  // halo = half-sweeps per launch
+    constexpr int BH = CD_BK_H;  // halo = half-sweeps per launch
     const int bix = BK_RX - 2 * BH, biy = BK_RY - 2 * BH;
     const int tiles_x = div_up(w, bix), tiles_y = div_up(h, biy);
     const int ntiles = tiles_x * tiles_y;
