@@ -123,7 +123,16 @@ def device_copy(a) -> int:
     torch.cuda.synchronize()
     y.copy_(x)
     torch.cuda.synchronize()
-    print(json.dumps({"mode": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n}), flush=True)
+    reps = max(a.steps, 1)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        y.copy_(x)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    print(json.dumps({"mode": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n,
+                      "ms_per_copy": ms, "GBs": 8 * n / (ms * 1e-3) / 1e9}), flush=True)
     return 0
 
 

@@ -122,16 +122,50 @@ __global__ __launch_bounds__(256) void k_flush(double* __restrict__ phi, float* 
     }
 }
 
+// Bricked checkerboard layout of the blocked sweeps.  Colour c = (x+y)&1 and
+// packed column j = x>>1 split the grid into two planes of h x wp cells
+// (wp = (w+1)/2); each plane is stored in bricks of 8 rows x 64 packed columns
+// (2 KiB of FP32), bricks row-major.  A wave of the blocked kernel reads
+// 8 consecutive rows of one brick as one contiguous 2 KiB stream.
+struct Brick {
+    int wpb;       // packed columns rounded up to 64
+    int nbx;       // bricks per row of bricks
+    size_t plane;  // elements per colour plane (rows rounded up to 8)
+};
+
+inline Brick make_brick(int w, int h) {
+    Brick b;
+    const int wp = (w + 1) / 2;
+    b.wpb = (wp + 63) / 64 * 64;
+    b.nbx = b.wpb / 64;
+    b.plane = (size_t)((h + 7) / 8 * 8) * b.wpb;
+    return b;
+}
+
+__device__ __forceinline__ size_t brick_row(const Brick& b, int c, int Y) {
+    return (size_t)c * b.plane + ((size_t)(Y >> 3) * b.nbx) * 512 + (size_t)(Y & 7) * 64;
+}
+__device__ __forceinline__ unsigned brick_col(int j) { return (unsigned)((j >> 6) * 512 + (j & 63)); }
+
+// FP32 -> BF16 (round to nearest even) and back; finite inputs only
+__device__ __forceinline__ unsigned short f32_to_bf16(float x) {
+    const unsigned u = __float_as_uint(x);
+    return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(unsigned short v) {
+    return __uint_as_float((unsigned)v << 16);
+}
+
 // blocked path: fold the current ping-pong correction into phi64, zero both
 __global__ __launch_bounds__(256) void k_flush_packed(double* __restrict__ phi,
                                                       float* __restrict__ ea,
                                                       float* __restrict__ eb,
                                                       const SolveDev* __restrict__ st, int w,
-                                                      int h) {
+                                                      int h, Brick bk) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= w || y >= h) return;
-    const size_t pi = ((size_t)((x + y) & 1) * h + y) * ((w + 1) / 2) + (x >> 1);
+    const size_t pi = brick_row(bk, (x + y) & 1, y) + brick_col(x >> 1);
     const float ev = st->cur ? eb[pi] : ea[pi];
     phi[(size_t)y * w + x] += (double)ev;
     ea[pi] = 0.f;
@@ -142,20 +176,16 @@ __global__ __launch_bounds__(256) void k_flush_packed(double* __restrict__ phi,
 // FP64 residual r = (b - bmean) - A phi  (+ FP32 copy for the inner sweeps)
 // partials[block] = (sum r, sum r^2)
 // ---------------------------------------------------------------------------
-// Checkerboard-packed FP32 layout of the blocked sweeps: cell (X,Y) of colour
-// c = (X+Y)&1 lives at c*h*wp + Y*wp + (X>>1), wp = (w+1)/2, so every colour
-// is contiguous along a row both in HBM and in LDS.
-__device__ __forceinline__ size_t packed_index(int X, int Y, int h, int wp) {
-    return (size_t)((X + Y) & 1) * h * wp + (size_t)Y * wp + (X >> 1);
-}
-
-template <bool WRITE_R32, bool PACKED>
+// OUT: 0 norms only, 1 FP32 copy of r (natural layout), 2 BF16 copy of r
+// (bricked layout, blocked sweeps)
+template <int OUT>
 __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ phi,
                                                   const double* __restrict__ b,
                                                   const uint8_t* __restrict__ mask,
                                                   const SolveDev* __restrict__ st,
-                                                  float* __restrict__ r32, int w, int h,
-                                                  double* __restrict__ partials) {
+                                                  float* __restrict__ r32,
+                                                  unsigned short* __restrict__ r16, Brick bk,
+                                                  int w, int h, double* __restrict__ partials) {
     __shared__ double smem[16];
     int x = blockIdx.x * blockDim.x + threadIdx.x;
     int y = blockIdx.y * blockDim.y + threadIdx.y;
@@ -184,7 +214,8 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ phi
             rs = r;
             rq = r * r;
         }
-        if (WRITE_R32) r32[PACKED ? packed_index(x, y, h, (w + 1) / 2) : i] = (float)r;
+        if (OUT == 1) r32[i] = (float)r;
+        if (OUT == 2) r16[brick_row(bk, (x + y) & 1, y) + brick_col(x >> 1)] = f32_to_bf16((float)r);
     }
     double a = cd_block_sum(rs, smem);
     double q = cd_block_sum(rq, smem);
@@ -418,13 +449,13 @@ __device__ __forceinline__ void blocked_run(float (&ev)[2][BK_R], const RV& rv,
 template <int H, bool TRACK>
 __global__ __launch_bounds__(256) void k_rb_blocked(const float* __restrict__ ein,
                                                     float* __restrict__ eout,
-                                                    const float* __restrict__ r, int w, int h,
+                                                    const unsigned short* __restrict__ r,
+                                                    Brick bk, int w, int h,
                                                     int tiles_x, int ntiles, int nhalf,
                                                     float om2, float om3, float om4,
                                                     const float* __restrict__ rmean,
                                                     SolveDev* st, int out_index) {
     static_assert(H % 2 == 0 && H >= 2 && 2 * H < BK_R * 2 && BK_R % 2 == 0, "geometry");
-    if (st->done) return;
     __shared__ float xch[2][4][2][64];
     __shared__ float smax[4][H / 2];
     // XCD-aware tile order: gridDim.x is a multiple of 8 and consecutive
@@ -439,7 +470,6 @@ __global__ __launch_bounds__(256) void k_rb_blocked(const float* __restrict__ ei
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wp = (w + 1) / 2;
-    const float rm = *rmean;
     const int Yw = Y0 + BK_R * wave;  // first global row of this wave
     const int dyw = BK_R * wave;      // first region row of this wave
     // per-lane column facts for the two cells of packed column `lane`
@@ -450,26 +480,35 @@ __global__ __launch_bounds__(256) void k_rb_blocked(const float* __restrict__ ei
     const bool l0 = X0l > 0, l1 = X0l + 1 > 0;
     const bool r0 = X0l < w - 1, r1 = X0l + 1 < w - 1;
     // packed column of both cells, clamped into the row: every load below is
-    // a uniform row pointer plus this one 32-bit lane offset
-    const unsigned col = (unsigned)min(max(X0 / 2 + lane, 0), wp - 1);
+    // a uniform row offset plus this one 32-bit lane offset
+    const unsigned col = brick_col(min(max(X0 / 2 + lane, 0), wp - 1));
     float ev[2][BK_R];
+    unsigned short rr[2][BK_R];
+#pragma unroll
+    for (int k = 0; k < BK_R; ++k) {
+        const int Yc = min(max(Yw + k, 0), h - 1);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const size_t row = brick_row(bk, c, Yc);
+            ev[c][k] = ein[row + col];
+            rr[c][k] = r[row + col];
+        }
+    }
+    // the stop flag is read after the loads are in flight; a finished solve
+    // leaves the buffers untouched
+    if (st->done) return;
+    const float rm = *rmean;
     __shared__ float rlds[BK_RV_LDS ? 2 : 1][BK_RV_LDS ? BK_RY : 1][64];
     typename std::conditional<BK_RV_LDS, RhsLds, RhsRegs>::type rv;
     if constexpr (BK_RV_LDS) rv.lds = reinterpret_cast<float (*)[BK_RY][64]>(&rlds[0][0][0]);
 #pragma unroll
     for (int k = 0; k < BK_R; ++k) {
-        const int Y = Yw + k;
-        const bool iny = (unsigned)Y < (unsigned)h;
-        const int Yc = min(max(Y, 0), h - 1);
+        const bool iny = (unsigned)(Yw + k) < (unsigned)h;
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
-            const int p = (k + c) & 1;
-            const bool in = iny && (p ? in1 : in0);
-            const size_t row = ((size_t)c * h + Yc) * wp;
-            const float a = ein[row + col];
-            const float b = r[row + col];
-            ev[c][k] = in ? a : 0.f;
-            rv.set(c, k, dyw, lane, in ? b - rm : 0.f);
+            const bool in = iny && (((k + c) & 1) ? in1 : in0);
+            ev[c][k] = in ? ev[c][k] : 0.f;
+            rv.set(c, k, dyw, lane, in ? bf16_to_f32(rr[c][k]) - rm : 0.f);
         }
     }
     float mxs[H / 2];
@@ -492,7 +531,7 @@ __global__ __launch_bounds__(256) void k_rb_blocked(const float* __restrict__ ei
             const int p = (k + c) & 1;
             const int dx = p ? dx1 : dx0;
             if (dx >= H && dy >= H && (unsigned)Y < (unsigned)h && (p ? in1 : in0))
-                eout[((size_t)c * h + Y) * wp + col] = ev[c][k];
+                eout[brick_row(bk, c, Y) + col] = ev[c][k];
         }
     }
     if (TRACK) {
@@ -595,10 +634,12 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     // FP32 sweeps without holes take the temporally blocked kernel on the
     // packed layout; FP64 sweeps and hole masks take the per-half-sweep kernel.
     const bool blocked = mixed && mask == nullptr;
-    const size_t npk = 2 * (size_t)h * ((w + 1) / 2);
+    const Brick bk = make_brick(w, h);
+    const size_t npk = 2 * bk.plane;
     if (mixed) {
         CD_TRY(ctx->ensure(ctx->e32, sizeof(float) * (blocked ? npk : n)));
-        CD_TRY(ctx->ensure(ctx->r32, sizeof(float) * (blocked ? npk : n)));
+        // FP32 rhs (natural layout) or BF16 rhs (bricked layout)
+        CD_TRY(ctx->ensure(ctx->r32, blocked ? sizeof(unsigned short) * npk : sizeof(float) * n));
         if (blocked) CD_TRY(ctx->ensure(ctx->e32b, sizeof(float) * npk));
     }
     double* partials = ctx->partials.as<double>();
@@ -606,6 +647,8 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     float* e32 = mixed ? ctx->e32.as<float>() : nullptr;
     float* e32b = blocked ? ctx->e32b.as<float>() : nullptr;
     float* r32 = mixed ? ctx->r32.as<float>() : nullptr;
+    unsigned short* r16 = blocked ? ctx->r32.as<unsigned short>() : nullptr;
+    if (mixed && blocked) CD_TRY(cd_fill_f32(ctx, r32, npk / 2, 0.f));  // padding stays finite
 
     const double omega = cd_sor_omega(o->omega, w, h);
     const int kmax = o->check_every > 0 ? o->check_every : auto_check_every(w, h);
@@ -626,16 +669,16 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     auto enqueue_check = [&](int check_id) -> int {
         CD_TRY(ctx->timer_begin());
         if (blocked) {
-            hipLaunchKernelGGL(k_flush_packed, rgrid, rblk, 0, s, phi, e32, e32b, sd, w, h);
-            hipLaunchKernelGGL((k_residual<true, true>), rgrid, rblk, 0, s, phi, b, mask, sd, r32,
+            hipLaunchKernelGGL(k_flush_packed, rgrid, rblk, 0, s, phi, e32, e32b, sd, w, h, bk);
+            hipLaunchKernelGGL((k_residual<2>), rgrid, rblk, 0, s, phi, b, mask, sd, r32, r16, bk,
                                w, h, partials);
         } else if (mixed) {
             hipLaunchKernelGGL(k_flush, dim3(fblocks), dim3(CD_BLOCK), 0, s, phi, e32, n);
-            hipLaunchKernelGGL((k_residual<true, false>), rgrid, rblk, 0, s, phi, b, mask, sd, r32,
+            hipLaunchKernelGGL((k_residual<1>), rgrid, rblk, 0, s, phi, b, mask, sd, r32, r16, bk,
                                w, h, partials);
         } else {
-            hipLaunchKernelGGL((k_residual<false, false>), rgrid, rblk, 0, s, phi, b, mask, sd,
-                               r32, w, h, partials);
+            hipLaunchKernelGGL((k_residual<0>), rgrid, rblk, 0, s, phi, b, mask, sd, r32, r16, bk,
+                               w, h, partials);
         }
         hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, partials, nred, sd, count,
                            check_id, o->stop, o->tol);
@@ -668,13 +711,13 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
             float* eout = in ? e32 : e32b;
             if (track) {
                 hipLaunchKernelGGL((k_rb_blocked<BH, true>), dim3(bgrid), dim3(256), 0, s, ein,
-                                   eout, r32, w, h, tiles_x, ntiles, 2 * nsw, om2, om3, om4,
+                                   eout, r16, bk, w, h, tiles_x, ntiles, 2 * nsw, om2, om3, om4,
                                    &sd->rmean32, sd, 1 - in);
                 hipLaunchKernelGGL(k_maxcheck_blocked, dim3(1), dim3(64), 0, s, sd,
                                    first + done_sw, nsw, o->tol);
             } else {
                 hipLaunchKernelGGL((k_rb_blocked<BH, false>), dim3(bgrid), dim3(256), 0, s, ein,
-                                   eout, r32, w, h, tiles_x, ntiles, 2 * nsw, om2, om3, om4,
+                                   eout, r16, bk, w, h, tiles_x, ntiles, 2 * nsw, om2, om3, om4,
                                    &sd->rmean32, sd, 1 - in);
             }
             ++launches;

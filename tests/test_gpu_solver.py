@@ -34,9 +34,11 @@ def test_fp64_red_black_iterates_match_oracle(ctx, oracle, rng):
 @pytest.mark.parametrize("w,h,k,every", [(64, 64, 13, 5), (300, 130, 9, 4), (129, 77, 22, 7),
                                          (1000, 50, 6, 6)])
 def test_fp32_blocked_iterates_match_oracle(ctx, oracle, rng, w, h, k, every):
-    """Mixed precision (temporally blocked FP32 sweeps, FP64 flushes every
-    `every` sweeps): after k sweeps the iterate equals k red-black sweeps of the
-    oracle to FP32 accuracy, across launch and tile boundaries."""
+    """Mixed precision (temporally blocked FP32 sweeps on a BF16 right-hand side,
+    FP64 flushes every `every` sweeps): after k sweeps the iterate equals k
+    red-black sweeps of the oracle to the BF16 rounding of the right-hand side
+    (2^-9 relative), across launch and tile boundaries; an indexing or
+    ordering error would be O(1)."""
     b = rng.standard_normal((h, w)); b -= b.mean()
     p0 = 0.1 * rng.standard_normal((h, w))
     o = capi.default_opts(max_sweeps=k, stop=capi.STOP_MAX_UPDATE, tol=1e-300,
@@ -45,7 +47,7 @@ def test_fp32_blocked_iterates_match_oracle(ctx, oracle, rng, w, h, k, every):
     ref, _ = oracle.poisson_sor(b, p0, k, max_update_tol=1e-300, ordering=RED_BLACK)
     assert st.sweeps == k
     scale = np.abs(ref - p0).max()
-    assert np.abs(g - ref).max() <= 2e-5 * scale, (w, h, np.abs(g - ref).max(), scale)
+    assert np.abs(g - ref).max() <= 1e-2 * scale, (w, h, np.abs(g - ref).max(), scale)
 
 
 @pytest.mark.parametrize("precision", [capi.PREC_MIXED, capi.PREC_FP64])
