@@ -95,7 +95,8 @@ typedef enum {
     CD_KERNEL_RESIDUAL = 1,     /* FP64 residual + correction flush                  */
     CD_KERNEL_RASTER = 2,       /* mesh rasterization                                */
     CD_KERNEL_VERTEX = 3,       /* per-vertex kernels                                */
-    CD_KERNEL_COUNT = 4
+    CD_KERNEL_LEARNED = 4,      /* learned correction (restriction, network, prolongation) */
+    CD_KERNEL_COUNT = 5
 } cd_kernel_id;
 int cd_context_set_profiling(cd_context* ctx, int enable);
 /* total elapsed ms, number of launches, algorithmic bytes moved by those launches */
@@ -192,6 +193,57 @@ int cd_design_set(cd_design* d, int field, const double* host_in, size_t count);
 /* device pointer of a field's storage (NULL for composite fields) */
 const void* cd_design_device_ptr(cd_design* d, int field);
 int cd_design_last_solve_stats(const cd_design* d, cd_solve_stats* phi, cd_solve_stats* h);
+
+/* ---- learned initial correction (north_star; no reference counterpart) ----
+ * A bias-free convolutional encoder-decoder (U-Net: `levels` resolution
+ * levels, two 3x3 convolutions per level and per decoder stage, 2x2 average
+ * pooling, nearest upsampling, concatenated skips, ReLU) maps the residual of
+ * a Poisson solve, restricted to a grid `coarsen` times coarser and
+ * normalised to unit RMS, to the correction, normalised by RMS*(n/pi)^2
+ * (n = larger fine side).  BF16 weights and activations, FP32 accumulation
+ * (gfx950 MFMA), FP32 master weights and Adam.  Solves with learned_init = 1
+ * apply it at every convergence check, keep a correction only if it lowers the
+ * FP64 residual, smooth with red-black SOR sweeps, and fall back to plain SOR
+ * (which always finishes to the tolerance) once a correction is rejected. */
+typedef struct cd_net cd_net;
+typedef struct {
+    int levels;        /* resolution levels (>= 2)                          */
+    int channels[6];   /* channels per level, multiples of 8 (e.g. 16 32 64 64) */
+    int coarsen;       /* fine cells per coarse cell along each axis (4)     */
+    int smooth_sweeps; /* red-black sweeps between learned corrections (4)   */
+    double smooth_omega; /* over-relaxation of those sweeps (1.2)            */
+} cd_net_arch;
+
+void cd_net_arch_default(cd_net_arch* a);
+int cd_net_create(cd_context* ctx, const cd_net_arch* arch, cd_net** out);
+void cd_net_destroy(cd_net* net);
+size_t cd_net_num_params(const cd_net* net);
+int cd_net_init_params(cd_net* net, unsigned long long seed);      /* He-normal     */
+int cd_net_set_params(cd_net* net, const float* host, size_t n);  /* FP32 master   */
+int cd_net_get_params(cd_net* net, float* host, size_t n);
+/* Forward pass on a device batch: x, y are [batch][h][w] FP32 (normalised
+ * coarse residual in, normalised coarse correction out). */
+int cd_net_forward(cd_net* net, const float* d_x, float* d_y, int batch, int h, int w);
+/* Mean squared error of the forward pass against d_t and its gradient with
+ * respect to every parameter (FP32, host order of cd_net_get_params). */
+int cd_net_loss_grad(cd_net* net, const float* d_x, const float* d_t, int batch, int h, int w,
+                     double* loss, float* host_grad, size_t n);
+/* One Adam step (beta 0.9/0.999, eps 1e-8) on a device batch; *loss is the
+ * mean squared error before the step. */
+int cd_net_train_step(cd_net* net, const float* d_x, const float* d_t, int batch, int h, int w,
+                      float lr, double* loss);
+/* Attach (or detach with NULL) the network used by solves with learned_init. */
+int cd_context_set_net(cd_context* ctx, cd_net* net);
+
+/* Training pairs captured during solves: at every convergence check of a
+ * solve of a (coarsen*h) x (coarsen*w) grid the normalised coarse residual is
+ * appended to d_x and, when the solve ends, the normalised coarse difference
+ * between the final and the then-current solution to d_t ([capacity][h][w]
+ * FP32 each).  Pairs are taken every `every`-th check; capture stops at
+ * capacity.  d_x == NULL disables capture. */
+int cd_context_set_capture(cd_context* ctx, float* d_x, float* d_t, int capacity, int h, int w,
+                           int coarsen, int every);
+int cd_context_capture_count(const cd_context* ctx, int* count);
 
 #ifdef __cplusplus
 }

@@ -30,7 +30,7 @@ _D, _I, _P, _SZ, _LL = C.c_double, C.c_int, C.c_void_p, C.c_size_t, C.c_longlong
 OK, RASTER_MISS, FOLDED, NOT_CONVERGED, NAN_, BAD_ARG, HIP_ERROR, NO_DEVICE, OOM, STATE = range(10)
 PREC_MIXED, PREC_FP64 = 0, 1
 STOP_RESIDUAL, STOP_MAX_UPDATE = 0, 1
-KERNEL_SOR_SWEEP, KERNEL_RESIDUAL, KERNEL_RASTER, KERNEL_VERTEX = 0, 1, 2, 3
+KERNEL_SOR_SWEEP, KERNEL_RESIDUAL, KERNEL_RASTER, KERNEL_VERTEX, KERNEL_LEARNED = 0, 1, 2, 3, 4
 
 FIELD = {
     "target_points": 0, "source_points": 1, "phi": 2, "h": 3, "raster": 4, "errors": 5,
@@ -67,6 +67,11 @@ class RunStats(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class NetArch(C.Structure):
+    _fields_ = [("levels", _I), ("channels", _I * 6), ("coarsen", _I), ("smooth_sweeps", _I),
+                ("smooth_omega", _D)]
 
 
 class CDError(RuntimeError):
@@ -115,6 +120,19 @@ def load(path: str = LIB_PATH):
         "cd_design_set": ([_P, _I, _P, _SZ], _I),
         "cd_design_device_ptr": ([_P, _I], _P),
         "cd_design_last_solve_stats": ([_P, C.POINTER(SolveStats), C.POINTER(SolveStats)], _I),
+        "cd_net_arch_default": ([C.POINTER(NetArch)], None),
+        "cd_net_create": ([_P, C.POINTER(NetArch), C.POINTER(_P)], _I),
+        "cd_net_destroy": ([_P], None),
+        "cd_net_num_params": ([_P], _SZ),
+        "cd_net_init_params": ([_P, C.c_ulonglong], _I),
+        "cd_net_set_params": ([_P, _P, _SZ], _I),
+        "cd_net_get_params": ([_P, _P, _SZ], _I),
+        "cd_net_forward": ([_P, _P, _P, _I, _I, _I], _I),
+        "cd_net_loss_grad": ([_P, _P, _P, _I, _I, _I, C.POINTER(_D), _P, _SZ], _I),
+        "cd_net_train_step": ([_P, _P, _P, _I, _I, _I, C.c_float, C.POINTER(_D)], _I),
+        "cd_context_set_net": ([_P, _P], _I),
+        "cd_context_set_capture": ([_P, _P, _P, _I, _I, _I, _I, _I], _I),
+        "cd_context_capture_count": ([_P, C.POINTER(_I)], _I),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -133,7 +151,11 @@ def exported_symbols():
         "cd_design_create", "cd_design_destroy", "cd_design_set_solve_opts", "cd_design_init",
         "cd_design_init_device", "cd_design_transport_step", "cd_design_height_step",
         "cd_design_run", "cd_design_field_size", "cd_design_get", "cd_design_set",
-        "cd_design_device_ptr", "cd_design_last_solve_stats",
+        "cd_design_device_ptr", "cd_design_last_solve_stats", "cd_net_arch_default",
+        "cd_net_create", "cd_net_destroy", "cd_net_num_params", "cd_net_init_params",
+        "cd_net_set_params", "cd_net_get_params", "cd_net_forward", "cd_net_loss_grad",
+        "cd_net_train_step", "cd_context_set_net", "cd_context_set_capture",
+        "cd_context_capture_count",
     ]
 
 
@@ -208,6 +230,22 @@ class Context:
                                      C.byref(opts or default_opts()), C.byref(st))
         self.check(rc, "cd_poisson_solve", allow)
         return phi, st
+
+    def set_net(self, net: Optional["Net"]):
+        self.check(self.L.cd_context_set_net(self.h, net.h if net else None), "set_net")
+
+    def set_capture(self, d_x: int, d_t: int, capacity: int, h: int, w: int, coarsen: int,
+                    every: int = 1):
+        self.check(self.L.cd_context_set_capture(self.h, d_x, d_t, capacity, h, w, coarsen,
+                                                  every), "set_capture")
+
+    def clear_capture(self):
+        self.check(self.L.cd_context_set_capture(self.h, None, None, 0, 0, 0, 0, 0), "set_capture")
+
+    def capture_count(self) -> int:
+        n = _I()
+        self.check(self.L.cd_context_capture_count(self.h, C.byref(n)), "capture_count")
+        return n.value
 
     def poisson_solve_device(self, d_rhs: int, d_phi: int, width: int, height: int,
                              opts: Optional[SolveOpts] = None, allow=(OK, NOT_CONVERGED)):
@@ -379,3 +417,66 @@ class CausticDesign:
     pixels = property(lambda self: self.get("pixels"))
     normals = property(lambda self: self.get("normals"))
     vertex_gradient = property(lambda self: self.get("vertex_gradient").T.copy())
+
+
+def default_arch(**kw) -> NetArch:
+    a = NetArch()
+    load().cd_net_arch_default(C.byref(a))
+    for k, v in kw.items():
+        if k == "channels":
+            for i, c in enumerate(v):
+                a.channels[i] = c
+        else:
+            setattr(a, k, v)
+    return a
+
+
+class Net:
+    """The learned-init network (cd_net) on one context."""
+
+    def __init__(self, ctx: Context, arch: Optional[NetArch] = None):
+        self.ctx, self.L = ctx, ctx.L
+        self.arch = arch or default_arch()
+        h = _P()
+        ctx.check(self.L.cd_net_create(ctx.h, C.byref(self.arch), C.byref(h)), "cd_net_create")
+        self.h = h
+        self.n = self.L.cd_net_num_params(h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cd_net_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init_params(self, seed: int = 0):
+        self.ctx.check(self.L.cd_net_init_params(self.h, seed), "init_params")
+
+    def get_params(self) -> np.ndarray:
+        p = np.empty(self.n, dtype=np.float32)
+        self.ctx.check(self.L.cd_net_get_params(self.h, p.ctypes.data, self.n), "get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self.ctx.check(self.L.cd_net_set_params(self.h, p.ctypes.data, p.size), "set_params")
+
+    def forward(self, d_x: int, d_y: int, batch: int, h: int, w: int):
+        self.ctx.check(self.L.cd_net_forward(self.h, d_x, d_y, batch, h, w), "net forward")
+
+    def loss_grad(self, d_x: int, d_t: int, batch: int, h: int, w: int):
+        loss = _D()
+        g = np.empty(self.n, dtype=np.float32)
+        self.ctx.check(self.L.cd_net_loss_grad(self.h, d_x, d_t, batch, h, w, C.byref(loss),
+                                               g.ctypes.data, self.n), "net loss_grad")
+        return loss.value, g
+
+    def train_step(self, d_x: int, d_t: int, batch: int, h: int, w: int, lr: float) -> float:
+        loss = _D()
+        self.ctx.check(self.L.cd_net_train_step(self.h, d_x, d_t, batch, h, w, lr, C.byref(loss)),
+                       "net train_step")
+        return loss.value

@@ -182,6 +182,12 @@ void cd_context_destroy(cd_context* ctx) {
     ctx->partials.release();
     ctx->solve_dev.release();
     ctx->fill.release();
+    ctx->lc_r.release();
+    ctx->lc_x.release();
+    ctx->lc_y.release();
+    ctx->lc_stats.release();
+    ctx->lc_backup.release();
+    ctx->cap_phi.release();
     if (ctx->host_status) (void)hipHostFree(ctx->host_status);
     for (auto& p : ctx->pending) {
         ctx->event_pool.push_back(p.a);

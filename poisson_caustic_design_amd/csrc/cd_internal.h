@@ -100,8 +100,15 @@ struct cd_context {
     SolveDev* host_status = nullptr;  // pinned
     int ws_w = 0, ws_h = 0;
 
-    // learned-init network handle (opaque, owned elsewhere)
-    void* net = nullptr;
+    // learned-init network (owned by the caller) and its solver workspace
+    cd_net* net = nullptr;
+    DevBuf lc_r, lc_x, lc_y, lc_stats, lc_backup;
+
+    // training-pair capture (device buffers owned by the caller)
+    float* cap_x = nullptr;
+    float* cap_t = nullptr;
+    int cap_capacity = 0, cap_count = 0, cap_h = 0, cap_w = 0, cap_coarsen = 0, cap_every = 1;
+    DevBuf cap_phi;  // FP64 restricted iterates of the pairs of the current solve
 
     void set_error(const std::string& s) { last_error = s; }
     int ensure(DevBuf& b, size_t bytes);
@@ -171,7 +178,6 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
                              long long valid_cells, int w, int h, const cd_solve_opts* o,
                              cd_solve_stats* st);
 
-// Learned-init hook (implemented in cd_learned.hip when a net is attached):
-// predicts a correction e (FP32, w*h) from the FP32 residual r.
-int cd_learned_predict(cd_context* ctx, const float* r32, const float* rmean, float* e32, int w,
-                       int h);
+// Learned network forward on one coarse field (cd_learned.hip).
+int cd_net_forward_internal(cd_net* n, const float* d_x, float* d_y, int h, int w);
+const cd_net_arch* cd_net_arch_of(const cd_net* n);

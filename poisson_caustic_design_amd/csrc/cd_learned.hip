@@ -1,7 +1,825 @@
-// Learned initial correction (placeholder until the network kernels land).
+// Learned initial correction: a bias-free convolutional encoder-decoder with
+// hand-written gfx950 kernels (BF16 MFMA, FP32 accumulation) for inference and
+// training.  No counterpart in the reference; see cd_api.h for the contract.
+//
+// Tensors are NHWC.  Activations are BF16 with channel counts that are
+// multiples of 8 (16-byte vectors per pixel); gradients are FP32.  A 3x3
+// convolution is an implicit GEMM: M = pixels, N = output channels,
+// K = 9 * input channels (tap-major: k = tap * Cin + ci, tap = 3*dy + dx),
+// computed with v_mfma_f32_16x16x32_bf16.  Lane layouts of that instruction:
+//   A (16x32): lane l holds A[l % 16][8*(l/16) + j], j = 0..7
+//   B (32x16): lane l holds B[8*(l/16) + j][l % 16]
+//   D (16x16): lane l holds D[4*(l/16) + r][l % 16], r = 0..3
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
 #include "cd_internal.h"
 
-int cd_learned_predict(cd_context* ctx, const float*, const float*, float*, int, int) {
-    if (ctx) ctx->set_error("no learned-init network attached");
-    return CD_ERR_STATE;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16;
+
+namespace {
+
+inline int div_up(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+__device__ __forceinline__ u16 to_bf16(float x) {
+    const unsigned u = __float_as_uint(x);
+    return (u16)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
 }
+__device__ __forceinline__ float from_bf16(u16 v) { return __uint_as_float((unsigned)v << 16); }
+
+union V8 {
+    bf16x8 v;
+    uint4 u;
+    u16 h[8];
+};
+
+// ---------------------------------------------------------------------------
+// 3x3 convolution, implicit GEMM on MFMA.
+//   IN 0: BF16 input = channel concatenation of x1 (c1 ch) and x2 (c2 ch)
+//   IN 1: FP32 input g (c1 ch), multiplied by (mask > 0) when mask != NULL
+//   OUT 0: BF16 output (cy ch), optional ReLU
+//   OUT 1: FP32 output split by channel into y1 (cy ch) and y2 (cy2 ch),
+//          optionally accumulated
+// wt: BF16 weights [nb*16][kpad], kpad a multiple of 32, zero padded.
+// One workgroup = 4 waves = 64 consecutive pixels; a wave computes 16 pixels
+// by NB*16 output channels.
+// ---------------------------------------------------------------------------
+template <int NB, int IN, int OUT>
+__global__ __launch_bounds__(256) void k_conv3x3(const u16* __restrict__ x1,
+                                                 const u16* __restrict__ x2, int c1, int c2,
+                                                 const float* __restrict__ g,
+                                                 const u16* __restrict__ mask,
+                                                 const u16* __restrict__ wt, int kpad, int B,
+                                                 int H, int W, u16* __restrict__ ybf,
+                                                 float* __restrict__ y1, float* __restrict__ y2,
+                                                 int cy, int cy2, int relu, int accumulate) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long long npix = (long long)B * H * W;
+    const long long p0 = (long long)blockIdx.x * 64 + wave * 16;
+    const int cin = IN == 0 ? c1 + c2 : c1;
+    const int K = 9 * cin;
+    const long long pa = p0 + (lane & 15);
+    const bool pvalid = pa < npix;
+    int xb = 0, yb = 0, bb = 0;
+    if (pvalid) {
+        xb = (int)(pa % W);
+        const long long t = pa / W;
+        yb = (int)(t % H);
+        bb = (int)(t / H);
+    }
+    floatx4 acc[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) acc[nb] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const int kq = 8 * (lane >> 4);
+    const u16* wrow = wt + (size_t)(lane & 15) * kpad + kq;
+    for (int kc = 0; kc < kpad; kc += 32) {
+        const int k = kc + kq;
+        V8 a;
+        a.u = make_uint4(0, 0, 0, 0);
+        if (pvalid && k < K) {
+            const int tap = k / cin, ci = k - tap * cin;
+            const int yy = yb + tap / 3 - 1, xx = xb + tap % 3 - 1;
+            if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
+                const long long q = ((long long)bb * H + yy) * W + xx;
+                if (IN == 0) {
+                    const u16* src = ci < c1 ? x1 + q * c1 + ci : x2 + q * c2 + (ci - c1);
+                    a.u = *reinterpret_cast<const uint4*>(src);
+                } else {
+                    const float4 g0 = *reinterpret_cast<const float4*>(g + q * c1 + ci);
+                    const float4 g1 = *reinterpret_cast<const float4*>(g + q * c1 + ci + 4);
+                    float gv[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
+                    if (mask) {
+                        V8 m;
+                        m.u = *reinterpret_cast<const uint4*>(mask + q * c1 + ci);
+#pragma unroll
+                        for (int j = 0; j < 8; ++j)
+                            if (!(from_bf16(m.h[j]) > 0.f)) gv[j] = 0.f;
+                    }
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) a.h[j] = to_bf16(gv[j]);
+                }
+            }
+        }
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            V8 bw;
+            bw.u = *reinterpret_cast<const uint4*>(wrow + (size_t)nb * 16 * kpad + kc);
+            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, bw.v, acc[nb], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const long long p = p0 + 4 * (lane >> 4) + r;
+            const int ch = nb * 16 + (lane & 15);
+            if (p >= npix) continue;
+            float v = acc[nb][r];
+            if (OUT == 0) {
+                if (ch < cy) {
+                    if (relu) v = fmaxf(v, 0.f);
+                    ybf[p * cy + ch] = to_bf16(v);
+                }
+            } else {
+                float* dst = nullptr;
+                if (ch < cy) dst = y1 + p * cy + ch;
+                else if (ch - cy < cy2) dst = y2 + p * cy2 + (ch - cy);
+                if (dst) *dst = accumulate ? *dst + v : v;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient dW[co][k] += sum_p gm[p][co] * xs[p][k], gm = g*(mask>0),
+// xs the 3x3-shifted (concatenated) input: MFMA with M = output channels,
+// N = k, reduction over pixels in steps of 32.  grid: (co blocks, k blocks,
+// pixel chunks); each wave reduces `ppw` pixels and adds its 16x16 tile with
+// FP32 atomics.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_wgrad(const u16* __restrict__ x1,
+                                               const u16* __restrict__ x2, int c1, int c2,
+                                               const float* __restrict__ g,
+                                               const u16* __restrict__ mask, int cg, int B,
+                                               int H, int W, float* __restrict__ dw, int kpad,
+                                               int ppw) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int co = blockIdx.x * 16 + (lane & 15);
+    const int kcol = blockIdx.y * 16 + (lane & 15);
+    const int cin = c1 + c2;
+    const int K = 9 * cin;
+    const long long npix = (long long)B * H * W;
+    const long long pbeg = ((long long)blockIdx.z * 4 + wave) * ppw;
+    int tap = 0, ci = 0, dy = 0, dx = 0;
+    const bool kvalid = kcol < K;
+    if (kvalid) {
+        tap = kcol / cin;
+        ci = kcol - tap * cin;
+        dy = tap / 3 - 1;
+        dx = tap % 3 - 1;
+    }
+    const bool cvalid = co < cg;
+    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (long long ps = pbeg; ps < pbeg + ppw && ps < npix; ps += 32) {
+        V8 a, b;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const long long p = ps + 8 * (lane >> 4) + j;
+            float gv = 0.f;
+            u16 xv = 0;
+            if (p < npix) {
+                if (cvalid) {
+                    gv = g[p * cg + co];
+                    if (mask && !(from_bf16(mask[p * cg + co]) > 0.f)) gv = 0.f;
+                }
+                if (kvalid) {
+                    const int xx = (int)(p % W) + dx;
+                    const long long t = p / W;
+                    const int yy = (int)(t % H) + dy;
+                    if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
+                        const long long q = (t / H * H + yy) * W + xx;
+                        xv = ci < c1 ? x1[q * c1 + ci] : x2[q * c2 + (ci - c1)];
+                    }
+                }
+            }
+            a.h[j] = to_bf16(gv);
+            b.h[j] = xv;
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b.v, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int m = blockIdx.x * 16 + 4 * (lane >> 4) + r;  // output channel row of D
+        if (m < cg && kvalid) atomicAdd(dw + (size_t)m * kpad + kcol, acc[r]);
+    }
+}
+
+// ---- elementwise / pooling ---------------------------------------------------
+__global__ void k_prep_input(const float* __restrict__ x, u16* __restrict__ x8, long long npix) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    V8 v;
+    v.u = make_uint4(0, 0, 0, 0);
+    v.h[0] = to_bf16(x[p]);
+    *reinterpret_cast<uint4*>(x8 + p * 8) = v.u;
+}
+
+__global__ void k_avgpool2(const u16* __restrict__ x, u16* __restrict__ y, int B, int H, int W,
+                           int C) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int Ho = H / 2, Wo = W / 2;
+    const long long n = (long long)B * Ho * Wo * C;
+    if (i >= n) return;
+    const int c = (int)(i % C);
+    long long t = i / C;
+    const int xo = (int)(t % Wo);
+    t /= Wo;
+    const int yo = (int)(t % Ho);
+    const int b = (int)(t / Ho);
+    const long long base = (((long long)b * H + 2 * yo) * W + 2 * xo) * C + c;
+    const float s = from_bf16(x[base]) + from_bf16(x[base + C]) + from_bf16(x[base + (long long)W * C]) +
+                    from_bf16(x[base + (long long)W * C + C]);
+    y[i] = to_bf16(0.25f * s);
+}
+
+// dx[fine] += dy[coarse] / 4
+__global__ void k_avgpool2_bwd(const float* __restrict__ dy, float* __restrict__ dx, int B, int H,
+                               int W, int C) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = (long long)B * H * W * C;
+    if (i >= n) return;
+    const int c = (int)(i % C);
+    long long t = i / C;
+    const int x = (int)(t % W);
+    t /= W;
+    const int y = (int)(t % H);
+    const int b = (int)(t / H);
+    dx[i] += 0.25f * dy[(((long long)b * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c];
+}
+
+// nearest 2x upsampling of x [B][H/2][W/2][C] to y [B][H][W][C]
+__global__ void k_upsample2(const u16* __restrict__ x, u16* __restrict__ y, int B, int H, int W,
+                            int C) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = (long long)B * H * W * C;
+    if (i >= n) return;
+    const int c = (int)(i % C);
+    long long t = i / C;
+    const int xx = (int)(t % W);
+    t /= W;
+    const int yy = (int)(t % H);
+    const int b = (int)(t / H);
+    y[i] = x[(((long long)b * (H / 2) + yy / 2) * (W / 2) + xx / 2) * C + c];
+}
+
+// dx[coarse] += sum of the 4 fine gradients
+__global__ void k_upsample2_bwd(const float* __restrict__ dy, float* __restrict__ dx, int B, int H,
+                                int W, int C) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int Ho = H / 2, Wo = W / 2;
+    const long long n = (long long)B * Ho * Wo * C;
+    if (i >= n) return;
+    const int c = (int)(i % C);
+    long long t = i / C;
+    const int xo = (int)(t % Wo);
+    t /= Wo;
+    const int yo = (int)(t % Ho);
+    const int b = (int)(t / Ho);
+    const long long base = (((long long)b * H + 2 * yo) * W + 2 * xo) * C + c;
+    dx[i] += dy[base] + dy[base + C] + dy[base + (long long)W * C] + dy[base + (long long)W * C + C];
+}
+
+// head output channel 0 -> y; loss partials and head gradient
+__global__ __launch_bounds__(256) void k_loss(const float* __restrict__ head, const float* __restrict__ t,
+                                              float* __restrict__ ghead, long long npix,
+                                              double* __restrict__ partials) {
+    __shared__ double smem[16];
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double e2 = 0.0;
+    if (p < npix) {
+        const float d = head[p * 8] - t[p];
+        e2 = (double)d * d;
+        if (ghead) {
+            ghead[p * 8] = 2.f * d / (float)npix;
+#pragma unroll
+            for (int c = 1; c < 8; ++c) ghead[p * 8 + c] = 0.f;
+        }
+    }
+    const double s = cd_block_sum(e2, smem);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ void k_head_out(const float* __restrict__ head, float* __restrict__ y, long long npix) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < npix) y[p] = head[p * 8];
+}
+
+// FP32 master -> BF16 forward weights [coutpad][kpad] (same layout)
+__global__ void k_cast_bf16(const float* __restrict__ w, u16* __restrict__ o, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) o[i] = to_bf16(w[i]);
+}
+
+// transposed, tap-flipped BF16 weights for the input gradient:
+// wt[ci][tap' * coutA + co] = w[co][(8 - tap') * cin + ci]
+__global__ void k_transpose_w(const float* __restrict__ w, u16* __restrict__ wt, int cout, int coutA,
+                              int cin, int kpad, int cinpad, int kpadT) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)cinpad * kpadT) return;
+    const int ci = (int)(i / kpadT);
+    const int kk = (int)(i - (long long)ci * kpadT);
+    float v = 0.f;
+    if (ci < cin && kk < 9 * coutA) {
+        const int tp = kk / coutA, co = kk - tp * coutA;
+        if (co < cout) v = w[(size_t)co * kpad + (8 - tp) * cin + ci];
+    }
+    wt[i] = to_bf16(v);
+}
+
+__global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
+                       const float* __restrict__ g, long long n, float lr, float b1, float b2,
+                       float c1, float c2, float eps) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+template <typename T>
+__global__ void k_zero(T* __restrict__ p, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = T(0);
+}
+
+}  // namespace
+
+// ===========================================================================
+// host side
+// ===========================================================================
+struct NetConv {
+    int c1, c2, cout, coutA;  // inputs (concatenated), logical and allocated outputs
+    int relu;
+    int K, kpad, coutpad;     // forward GEMM dims
+    int cinpad, kpadT;        // input-gradient GEMM dims
+    size_t woff;              // offset in the flat parameter arrays
+    size_t nparam;            // coutpad * kpad
+    int level;                // resolution level
+    // tensors (indices into NetState::acts)
+    int in1, in2, out;
+};
+
+struct NetTensor {
+    int level, C;             // channels (multiple of 8)
+    DevBuf act;               // BF16 (or FP32 for the head)
+    DevBuf grad;              // FP32
+};
+
+struct cd_net {
+    cd_context* ctx = nullptr;
+    cd_net_arch arch;
+    std::vector<NetConv> convs;
+    size_t nparam = 0;
+    DevBuf w, m, v, gw, wbf, wtbf, partials, head, ghead, x8;
+    size_t wtoff_total = 0;
+    std::vector<size_t> wtoff;
+    long long step = 0;
+    // activation tensors for the current (batch, h, w)
+    std::vector<NetTensor> t;
+    int batch = 0, bh = 0, bw = 0;
+    bool weights_dirty = true;
+};
+
+namespace {
+
+int launch_conv(cd_net* n, const NetConv& cv, int B, int H, int W, bool grad_input) {
+    cd_context* c = n->ctx;
+    hipStream_t s = c->stream;
+    const long long npix = (long long)B * H * W;
+    const int blocks = div_up(npix, 64);
+    if (!grad_input) {
+        const u16* x1 = cv.in1 < 0 ? n->x8.as<u16>() : n->t[cv.in1].act.as<u16>();
+        const u16* x2 = cv.in2 < 0 ? nullptr : n->t[cv.in2].act.as<u16>();
+        const u16* wt = n->wbf.as<u16>() + cv.woff;
+        const int nb = cv.coutpad / 16;
+        if (cv.out < 0) {  // head: FP32 output [npix][8]
+#define CONV_F(NB) hipLaunchKernelGGL((k_conv3x3<NB, 0, 1>), dim3(blocks), dim3(256), 0, s, x1, x2, cv.c1, cv.c2, (const float*)nullptr, (const u16*)nullptr, wt, cv.kpad, B, H, W, (u16*)nullptr, n->head.as<float>(), (float*)nullptr, 8, 0, 0, 0)
+            if (nb == 1) CONV_F(1);
+            else return CD_ERR_BAD_ARG;
+#undef CONV_F
+        } else {
+            u16* y = n->t[cv.out].act.as<u16>();
+#define CONV_B(NB) hipLaunchKernelGGL((k_conv3x3<NB, 0, 0>), dim3(blocks), dim3(256), 0, s, x1, x2, cv.c1, cv.c2, (const float*)nullptr, (const u16*)nullptr, wt, cv.kpad, B, H, W, y, (float*)nullptr, (float*)nullptr, cv.coutA, 0, cv.relu, 0)
+            switch (nb) {
+                case 1: CONV_B(1); break;
+                case 2: CONV_B(2); break;
+                case 3: CONV_B(3); break;
+                case 4: CONV_B(4); break;
+                case 6: CONV_B(6); break;
+                case 8: CONV_B(8); break;
+                default: return CD_ERR_BAD_ARG;
+            }
+#undef CONV_B
+        }
+    } else {
+        // input gradient: conv of the masked output gradient with transposed weights
+        const float* g = cv.out < 0 ? n->ghead.as<float>() : n->t[cv.out].grad.as<float>();
+        const u16* mask = (cv.out >= 0 && cv.relu) ? n->t[cv.out].act.as<u16>() : nullptr;
+        const u16* wt = n->wtbf.as<u16>() + n->wtoff[&cv - &n->convs[0]];
+        float* d1 = n->t[cv.in1].grad.as<float>();
+        float* d2 = cv.in2 < 0 ? nullptr : n->t[cv.in2].grad.as<float>();
+        const int nb = cv.cinpad / 16;
+#define CONV_G(NB) hipLaunchKernelGGL((k_conv3x3<NB, 1, 1>), dim3(blocks), dim3(256), 0, s, (const u16*)nullptr, (const u16*)nullptr, cv.coutA, 0, g, mask, wt, cv.kpadT, B, H, W, (u16*)nullptr, d1, d2, cv.c1, cv.c2, 0, 1)
+        switch (nb) {
+            case 1: CONV_G(1); break;
+            case 2: CONV_G(2); break;
+            case 3: CONV_G(3); break;
+            case 4: CONV_G(4); break;
+            case 6: CONV_G(6); break;
+            case 8: CONV_G(8); break;
+            default: return CD_ERR_BAD_ARG;
+        }
+#undef CONV_G
+    }
+    CD_CHECK_LAUNCH(c);
+    return CD_OK;
+}
+
+int launch_wgrad(cd_net* n, const NetConv& cv, int B, int H, int W) {
+    cd_context* c = n->ctx;
+    const u16* x1 = cv.in1 < 0 ? n->x8.as<u16>() : n->t[cv.in1].act.as<u16>();
+    const u16* x2 = cv.in2 < 0 ? nullptr : n->t[cv.in2].act.as<u16>();
+    const float* g = cv.out < 0 ? n->ghead.as<float>() : n->t[cv.out].grad.as<float>();
+    const u16* mask = (cv.out >= 0 && cv.relu) ? n->t[cv.out].act.as<u16>() : nullptr;
+    const long long npix = (long long)B * H * W;
+    const int ppw = 1024;
+    dim3 grid(div_up(cv.coutA, 16), div_up(cv.K, 16), div_up(npix, 4LL * ppw));
+    hipLaunchKernelGGL(k_wgrad, grid, dim3(256), 0, c->stream, x1, x2, cv.c1, cv.c2, g, mask,
+                       cv.coutA, B, H, W, n->gw.as<float>() + cv.woff, cv.kpad, ppw);
+    CD_CHECK_LAUNCH(c);
+    return CD_OK;
+}
+
+template <typename T>
+int zero(cd_context* c, T* p, long long n) {
+    if (n <= 0) return CD_OK;
+    hipLaunchKernelGGL(k_zero<T>, dim3(div_up(n, 256)), dim3(256), 0, c->stream, p, n);
+    CD_CHECK_LAUNCH(c);
+    return CD_OK;
+}
+
+// Build the layer list (independent of batch size).
+void build_layers(cd_net* n) {
+    const int L = n->arch.levels;
+    const int* C = n->arch.channels;
+    n->convs.clear();
+    n->t.clear();
+    auto tensor = [&](int level, int ch) {
+        NetTensor t;
+        t.level = level;
+        t.C = ch;
+        n->t.push_back(std::move(t));
+        return (int)n->t.size() - 1;
+    };
+    auto conv = [&](int in1, int c1, int in2, int c2, int out, int cout, int coutA, int relu,
+                    int level) {
+        NetConv cv;
+        cv.c1 = c1; cv.c2 = c2; cv.cout = cout; cv.coutA = coutA; cv.relu = relu;
+        cv.K = 9 * (c1 + c2);
+        cv.kpad = (cv.K + 31) / 32 * 32;
+        cv.coutpad = (coutA + 15) / 16 * 16;
+        cv.cinpad = (c1 + c2 + 15) / 16 * 16;
+        cv.kpadT = (9 * coutA + 31) / 32 * 32;
+        cv.woff = n->nparam;
+        cv.nparam = (size_t)cv.coutpad * cv.kpad;
+        n->nparam += cv.nparam;
+        cv.level = level;
+        cv.in1 = in1; cv.in2 = in2; cv.out = out;
+        n->convs.push_back(cv);
+    };
+    std::vector<int> skip(L);
+    int prev = -1, prevC = 8;  // -1: the 8-channel input tensor
+    for (int l = 0; l < L; ++l) {
+        int in = prev, inC = prevC;
+        if (l > 0) {
+            const int p = tensor(l, prevC);  // pooled
+            n->t[p].level = l;
+            in = p;
+        }
+        const int a = tensor(l, C[l]);
+        conv(in, inC, -1, 0, a, C[l], C[l], 1, l);
+        const int s = tensor(l, C[l]);
+        conv(a, C[l], -1, 0, s, C[l], C[l], 1, l);
+        skip[l] = s;
+        prev = s;
+        prevC = C[l];
+    }
+    int d = skip[L - 1], dC = C[L - 1];
+    for (int l = L - 2; l >= 0; --l) {
+        const int u = tensor(l, dC);  // upsampled
+        const int b = tensor(l, C[l]);
+        conv(u, dC, skip[l], C[l], b, C[l], C[l], 1, l);
+        const int e = tensor(l, C[l]);
+        conv(b, C[l], -1, 0, e, C[l], C[l], 1, l);
+        d = e;
+        dC = C[l];
+    }
+    conv(d, dC, -1, 0, -1, 1, 8, 0, 0);  // head: 1 output channel stored with stride 8
+    n->wtoff.clear();
+    size_t off = 0;
+    for (auto& cv : n->convs) {
+        n->wtoff.push_back(off);
+        off += (size_t)cv.cinpad * cv.kpadT;
+    }
+    n->wtoff_total = off;
+}
+
+int ensure_batch(cd_net* n, int B, int H, int W) {
+    cd_context* c = n->ctx;
+    const int L = n->arch.levels;
+    if (H % (1 << (L - 1)) || W % (1 << (L - 1))) {
+        c->set_error("net input size must be divisible by 2^(levels-1)");
+        return CD_ERR_BAD_ARG;
+    }
+    if (n->batch == B && n->bh == H && n->bw == W) return CD_OK;
+    for (auto& t : n->t) {
+        const long long px = (long long)B * (H >> t.level) * (W >> t.level);
+        CD_TRY(c->ensure(t.act, sizeof(u16) * px * t.C));
+        CD_TRY(c->ensure(t.grad, sizeof(float) * px * t.C));
+    }
+    const long long px = (long long)B * H * W;
+    CD_TRY(c->ensure(n->x8, sizeof(u16) * px * 8));
+    CD_TRY(c->ensure(n->head, sizeof(float) * px * 8));
+    CD_TRY(c->ensure(n->ghead, sizeof(float) * px * 8));
+    CD_TRY(c->ensure(n->partials, sizeof(double) * (size_t)div_up(px, 256)));
+    n->batch = B;
+    n->bh = H;
+    n->bw = W;
+    return CD_OK;
+}
+
+int refresh_weights(cd_net* n) {
+    if (!n->weights_dirty) return CD_OK;
+    cd_context* c = n->ctx;
+    hipLaunchKernelGGL(k_cast_bf16, dim3(div_up((long long)n->nparam, 256)), dim3(256), 0,
+                       c->stream, n->w.as<float>(), n->wbf.as<u16>(), (long long)n->nparam);
+    for (size_t i = 0; i < n->convs.size(); ++i) {
+        const NetConv& cv = n->convs[i];
+        const long long cnt = (long long)cv.cinpad * cv.kpadT;
+        hipLaunchKernelGGL(k_transpose_w, dim3(div_up(cnt, 256)), dim3(256), 0, c->stream,
+                           n->w.as<float>() + cv.woff, n->wtbf.as<u16>() + n->wtoff[i], cv.cout,
+                           cv.coutA, cv.c1 + cv.c2, cv.kpad, cv.cinpad, cv.kpadT);
+    }
+    CD_CHECK_LAUNCH(c);
+    n->weights_dirty = false;
+    return CD_OK;
+}
+
+// forward pass from the 8-channel input already in n->x8; head output in n->head
+int forward(cd_net* n, int B, int H, int W) {
+    cd_context* c = n->ctx;
+    CD_TRY(refresh_weights(n));
+    for (const NetConv& cv : n->convs) {
+        const int Hl = H >> cv.level, Wl = W >> cv.level;
+        CD_TRY(launch_conv(n, cv, B, Hl, Wl, false));
+        // after the second conv of an encoder level below the bottleneck: pool
+        const int idx = (int)(&cv - &n->convs[0]);
+        const int L = n->arch.levels;
+        if (idx < 2 * L && idx % 2 == 1 && cv.level < L - 1) {
+            const NetConv& nx = n->convs[idx + 1];  // next level's first conv reads the pooled tensor
+            const long long cnt = (long long)B * (Hl / 2) * (Wl / 2) * n->t[nx.in1].C;
+            hipLaunchKernelGGL(k_avgpool2, dim3(div_up(cnt, 256)), dim3(256), 0, c->stream,
+                               n->t[cv.out].act.as<u16>(), n->t[nx.in1].act.as<u16>(), B, Hl, Wl,
+                               n->t[nx.in1].C);
+        }
+        // before a decoder concat conv: upsample the previous decoder output
+        if (idx + 1 < (int)n->convs.size()) {
+            const NetConv& nx = n->convs[idx + 1];
+            if (nx.in2 >= 0) {
+                const int Hn = H >> nx.level, Wn = W >> nx.level;
+                const long long cnt = (long long)B * Hn * Wn * n->t[nx.in1].C;
+                hipLaunchKernelGGL(k_upsample2, dim3(div_up(cnt, 256)), dim3(256), 0, c->stream,
+                                   n->t[cv.out].act.as<u16>(), n->t[nx.in1].act.as<u16>(), B, Hn,
+                                   Wn, n->t[nx.in1].C);
+            }
+        }
+        CD_CHECK_LAUNCH(c);
+    }
+    return CD_OK;
+}
+
+// backward pass: n->ghead holds dL/dhead; accumulates n->gw
+int backward(cd_net* n, int B, int H, int W) {
+    cd_context* c = n->ctx;
+    for (auto& t : n->t) {
+        const long long px = (long long)B * (H >> t.level) * (W >> t.level);
+        CD_TRY(zero(c, t.grad.as<float>(), px * t.C));
+    }
+    const int L = n->arch.levels;
+    for (int idx = (int)n->convs.size() - 1; idx >= 0; --idx) {
+        const NetConv& cv = n->convs[idx];
+        const int Hl = H >> cv.level, Wl = W >> cv.level;
+        // gradient into this conv's output is complete here: its consumers
+        // (later convs, pooling, upsampling) were processed before it
+        CD_TRY(launch_wgrad(n, cv, B, Hl, Wl));
+        if (cv.in1 >= 0) CD_TRY(launch_conv(n, cv, B, Hl, Wl, true));
+        // route input gradients through upsampling / pooling
+        if (cv.in2 >= 0) {
+            // in1 is the upsampled previous decoder output (level+1)
+            const NetConv& pv = n->convs[idx - 1];
+            const long long cnt = (long long)B * (Hl / 2) * (Wl / 2) * n->t[cv.in1].C;
+            hipLaunchKernelGGL(k_upsample2_bwd, dim3(div_up(cnt, 256)), dim3(256), 0, c->stream,
+                               n->t[cv.in1].grad.as<float>(), n->t[pv.out].grad.as<float>(), B, Hl,
+                               Wl, n->t[cv.in1].C);
+        }
+        if (idx < 2 * L && idx % 2 == 0 && cv.level > 0) {
+            // in1 is the pooled skip of the level above
+            const NetConv& pv = n->convs[idx - 1];
+            const long long cnt = (long long)B * (Hl * 2) * (Wl * 2) * n->t[cv.in1].C;
+            hipLaunchKernelGGL(k_avgpool2_bwd, dim3(div_up(cnt, 256)), dim3(256), 0, c->stream,
+                               n->t[cv.in1].grad.as<float>(), n->t[pv.out].grad.as<float>(), B,
+                               Hl * 2, Wl * 2, n->t[cv.in1].C);
+        }
+        CD_CHECK_LAUNCH(c);
+    }
+    return CD_OK;
+}
+
+int loss_and_head_grad(cd_net* n, const float* d_t, int B, int H, int W, double* loss,
+                       bool want_grad) {
+    cd_context* c = n->ctx;
+    const long long px = (long long)B * H * W;
+    const int nb = div_up(px, 256);
+    hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, c->stream, n->head.as<float>(), d_t,
+                       want_grad ? n->ghead.as<float>() : nullptr, px, n->partials.as<double>());
+    CD_CHECK_LAUNCH(c);
+    CD_TRY(c->ensure(c->fill, sizeof(double) * 4));
+    CD_TRY(cd_reduce_partials(c, n->partials.as<double>(), nb, 1, c->fill.as<double>()));
+    double s = 0.0;
+    CD_HIP(c, hipMemcpyAsync(&s, c->fill.ptr, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CD_HIP(c, hipStreamSynchronize(c->stream));
+    if (loss) *loss = s / (double)px;
+    return CD_OK;
+}
+
+int prep(cd_net* n, const float* d_x, int B, int H, int W) {
+    CD_TRY(ensure_batch(n, B, H, W));
+    const long long px = (long long)B * H * W;
+    hipLaunchKernelGGL(k_prep_input, dim3(div_up(px, 256)), dim3(256), 0, n->ctx->stream, d_x,
+                       n->x8.as<u16>(), px);
+    CD_CHECK_LAUNCH(n->ctx);
+    return CD_OK;
+}
+
+}  // namespace
+
+// solver-side entry: forward on one coarse field (batch 1), output FP32 [h][w]
+int cd_net_forward_internal(cd_net* n, const float* d_x, float* d_y, int h, int w) {
+    CD_TRY(prep(n, d_x, 1, h, w));
+    CD_TRY(forward(n, 1, h, w));
+    hipLaunchKernelGGL(k_head_out, dim3(div_up((long long)h * w, 256)), dim3(256), 0,
+                       n->ctx->stream, n->head.as<float>(), d_y, (long long)h * w);
+    CD_CHECK_LAUNCH(n->ctx);
+    return CD_OK;
+}
+
+const cd_net_arch* cd_net_arch_of(const cd_net* n) { return &n->arch; }
+
+extern "C" {
+
+void cd_net_arch_default(cd_net_arch* a) {
+    if (!a) return;
+    std::memset(a, 0, sizeof(*a));
+    a->levels = 4;
+    a->channels[0] = 16;
+    a->channels[1] = 32;
+    a->channels[2] = 64;
+    a->channels[3] = 64;
+    a->coarsen = 4;
+    a->smooth_sweeps = 4;
+    a->smooth_omega = 1.2;
+}
+
+int cd_net_create(cd_context* ctx, const cd_net_arch* arch, cd_net** out) {
+    if (!ctx || !arch || !out) return CD_ERR_BAD_ARG;
+    *out = nullptr;
+    if (arch->levels < 2 || arch->levels > 6 || arch->coarsen < 1) return CD_ERR_BAD_ARG;
+    for (int l = 0; l < arch->levels; ++l)
+        if (arch->channels[l] <= 0 || arch->channels[l] % 8 || arch->channels[l] > 128)
+            return CD_ERR_BAD_ARG;
+    CD_HIP(ctx, hipSetDevice(ctx->device));
+    auto* n = new cd_net();
+    n->ctx = ctx;
+    n->arch = *arch;
+    build_layers(n);
+    int st = CD_OK;
+    if ((st = ctx->ensure(n->w, sizeof(float) * n->nparam)) != CD_OK ||
+        (st = ctx->ensure(n->m, sizeof(float) * n->nparam)) != CD_OK ||
+        (st = ctx->ensure(n->v, sizeof(float) * n->nparam)) != CD_OK ||
+        (st = ctx->ensure(n->gw, sizeof(float) * n->nparam)) != CD_OK ||
+        (st = ctx->ensure(n->wbf, sizeof(u16) * n->nparam)) != CD_OK ||
+        (st = ctx->ensure(n->wtbf, sizeof(u16) * n->wtoff_total)) != CD_OK ||
+        (st = zero(ctx, n->w.as<float>(), (long long)n->nparam)) != CD_OK ||
+        (st = zero(ctx, n->m.as<float>(), (long long)n->nparam)) != CD_OK ||
+        (st = zero(ctx, n->v.as<float>(), (long long)n->nparam)) != CD_OK) {
+        delete n;
+        return st;
+    }
+    *out = n;
+    return CD_OK;
+}
+
+void cd_net_destroy(cd_net* n) {
+    if (!n) return;
+    (void)hipStreamSynchronize(n->ctx->stream);
+    if (n->ctx->net == n) n->ctx->net = nullptr;
+    delete n;
+}
+
+size_t cd_net_num_params(const cd_net* n) { return n ? n->nparam : 0; }
+
+int cd_net_set_params(cd_net* n, const float* host, size_t count) {
+    if (!n || !host || count != n->nparam) return CD_ERR_BAD_ARG;
+    cd_context* c = n->ctx;
+    CD_HIP(c, hipMemcpyAsync(n->w.ptr, host, sizeof(float) * count, hipMemcpyHostToDevice,
+                             c->stream));
+    CD_TRY(zero(c, n->m.as<float>(), (long long)count));
+    CD_TRY(zero(c, n->v.as<float>(), (long long)count));
+    n->step = 0;
+    n->weights_dirty = true;
+    CD_HIP(c, hipStreamSynchronize(c->stream));
+    return CD_OK;
+}
+
+int cd_net_get_params(cd_net* n, float* host, size_t count) {
+    if (!n || !host || count != n->nparam) return CD_ERR_BAD_ARG;
+    cd_context* c = n->ctx;
+    CD_HIP(c, hipMemcpyAsync(host, n->w.ptr, sizeof(float) * count, hipMemcpyDeviceToHost,
+                             c->stream));
+    CD_HIP(c, hipStreamSynchronize(c->stream));
+    return CD_OK;
+}
+
+int cd_net_init_params(cd_net* n, unsigned long long seed) {
+    if (!n) return CD_ERR_BAD_ARG;
+    std::vector<float> w(n->nparam, 0.f);
+    std::mt19937_64 rng(seed);
+    for (const NetConv& cv : n->convs) {
+        const int fan_in = 9 * (cv.c1 + cv.c2);
+        std::normal_distribution<float> nd(0.f, std::sqrt(2.f / (float)fan_in));
+        const float scale = cv.out < 0 ? 0.1f : 1.f;  // small head
+        for (int co = 0; co < cv.cout; ++co)
+            for (int k = 0; k < cv.K; ++k) {
+                const int ci = k % (cv.c1 + cv.c2);
+                // the first layer only sees input channel 0
+                if (cv.in1 < 0 && &cv == &n->convs[0] && ci != 0) continue;
+                w[cv.woff + (size_t)co * cv.kpad + k] = scale * nd(rng);
+            }
+    }
+    return cd_net_set_params(n, w.data(), w.size());
+}
+
+int cd_net_forward(cd_net* n, const float* d_x, float* d_y, int batch, int h, int w) {
+    if (!n || !d_x || !d_y || batch < 1) return CD_ERR_BAD_ARG;
+    CD_HIP(n->ctx, hipSetDevice(n->ctx->device));
+    CD_TRY(prep(n, d_x, batch, h, w));
+    CD_TRY(forward(n, batch, h, w));
+    const long long px = (long long)batch * h * w;
+    hipLaunchKernelGGL(k_head_out, dim3(div_up(px, 256)), dim3(256), 0, n->ctx->stream,
+                       n->head.as<float>(), d_y, px);
+    CD_CHECK_LAUNCH(n->ctx);
+    CD_HIP(n->ctx, hipStreamSynchronize(n->ctx->stream));
+    return CD_OK;
+}
+
+int cd_net_loss_grad(cd_net* n, const float* d_x, const float* d_t, int batch, int h, int w,
+                     double* loss, float* host_grad, size_t count) {
+    if (!n || !d_x || !d_t || batch < 1) return CD_ERR_BAD_ARG;
+    if (host_grad && count != n->nparam) return CD_ERR_BAD_ARG;
+    cd_context* c = n->ctx;
+    CD_HIP(c, hipSetDevice(c->device));
+    CD_TRY(prep(n, d_x, batch, h, w));
+    CD_TRY(forward(n, batch, h, w));
+    CD_TRY(loss_and_head_grad(n, d_t, batch, h, w, loss, true));
+    CD_TRY(zero(c, n->gw.as<float>(), (long long)n->nparam));
+    CD_TRY(backward(n, batch, h, w));
+    if (host_grad) {
+        CD_HIP(c, hipMemcpyAsync(host_grad, n->gw.ptr, sizeof(float) * n->nparam,
+                                 hipMemcpyDeviceToHost, c->stream));
+    }
+    CD_HIP(c, hipStreamSynchronize(c->stream));
+    return CD_OK;
+}
+
+int cd_net_train_step(cd_net* n, const float* d_x, const float* d_t, int batch, int h, int w,
+                      float lr, double* loss) {
+    CD_TRY(cd_net_loss_grad(n, d_x, d_t, batch, h, w, loss, nullptr, 0));
+    cd_context* c = n->ctx;
+    n->step += 1;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - std::pow(b1, (float)n->step), c2 = 1.f - std::pow(b2, (float)n->step);
+    hipLaunchKernelGGL(k_adam, dim3(div_up((long long)n->nparam, 256)), dim3(256), 0, c->stream,
+                       n->w.as<float>(), n->m.as<float>(), n->v.as<float>(), n->gw.as<float>(),
+                       (long long)n->nparam, lr, b1, b2, c1, c2, 1e-8f);
+    CD_CHECK_LAUNCH(c);
+    n->weights_dirty = true;
+    return CD_OK;
+}
+
+int cd_context_set_net(cd_context* ctx, cd_net* net) {
+    if (!ctx) return CD_ERR_BAD_ARG;
+    if (net && net->ctx != ctx) return CD_ERR_BAD_ARG;
+    ctx->net = net;
+    return CD_OK;
+}
+
+}  // extern "C"

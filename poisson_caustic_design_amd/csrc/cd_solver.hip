@@ -612,6 +612,106 @@ int auto_check_every(int w, int h) {
     return std::max(8, std::min(k, 2048));
 }
 
+
+// ---------------------------------------------------------------------------
+// Learned correction and training-pair capture (no hole mask, positional
+// Neumann counts).  Coarse cell (I,J) averages the f x f fine cells
+// [f*I, f*I+f) x [f*J, f*J+f).
+// ---------------------------------------------------------------------------
+// coarse residual rc = block mean of (b - bmean) - A phi; partials (sum, sum^2)
+__global__ __launch_bounds__(256) void k_restrict_residual(const double* __restrict__ phi,
+                                                           const double* __restrict__ b,
+                                                           const SolveDev* __restrict__ st,
+                                                           int w, int h, int f, int wc, int hc,
+                                                           float* __restrict__ rc,
+                                                           double* __restrict__ partials) {
+    __shared__ double smem[16];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (i < wc * hc) {
+        const int J = i / wc, I = i - J * wc;
+        const double bm = st->bmean;
+        double acc = 0.0;
+        for (int yy = J * f; yy < J * f + f; ++yy)
+            for (int xx = I * f; xx < I * f + f; ++xx) {
+                const size_t c = (size_t)yy * w + xx;
+                double s = 0.0, cnt = 0.0;
+                if (xx > 0) { cnt += 1.0; s += phi[c - 1]; }
+                if (yy > 0) { cnt += 1.0; s += phi[c - w]; }
+                if (xx < w - 1) { cnt += 1.0; s += phi[c + 1]; }
+                if (yy < h - 1) { cnt += 1.0; s += phi[c + w]; }
+                acc += (b[c] - bm) - (s - cnt * phi[c]);
+            }
+        v = acc / (double)(f * f);
+        rc[i] = (float)v;
+    }
+    const double a = cd_block_sum(v, smem);
+    const double q = cd_block_sum(v * v, smem);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = a;
+        partials[2 * blockIdx.x + 1] = q;
+    }
+}
+
+// stats = (sum, sum^2) over n coarse cells -> x = (rc - mean) / rms
+__global__ void k_normalize_coarse(const float* __restrict__ rc, const double* __restrict__ stats,
+                                   int n, float* __restrict__ x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double mean = stats[0] / n;
+    const double var = stats[1] / n - mean * mean;
+    const double sig = sqrt(var > 1e-300 ? var : 1e-300);
+    x[i] = (float)(((double)rc[i] - mean) / sig);
+}
+
+// phi += sign * rms * (nmax/pi)^2 * bilinear(y) at cell centres
+__global__ __launch_bounds__(256) void k_prolong_add(const float* __restrict__ y,
+                                                     const double* __restrict__ stats, int nc,
+                                                     int w, int h, int f, int wc, int hc,
+                                                     double nscale, double sign,
+                                                     double* __restrict__ phi) {
+    const int xx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int yy = blockIdx.y;
+    if (xx >= w || yy >= h) return;
+    const double mean = stats[0] / nc;
+    const double var = stats[1] / nc - mean * mean;
+    const double sig = sqrt(var > 1e-300 ? var : 1e-300);
+    const float u = (xx + 0.5f) / f - 0.5f, v = (yy + 0.5f) / f - 0.5f;
+    int i0 = (int)floorf(u), j0 = (int)floorf(v);
+    const float tu = u - i0, tv = v - j0;
+    const int i1 = min(max(i0 + 1, 0), wc - 1), j1 = min(max(j0 + 1, 0), hc - 1);
+    i0 = min(max(i0, 0), wc - 1);
+    j0 = min(max(j0, 0), hc - 1);
+    const float e = (y[j0 * wc + i0] * (1.f - tu) + y[j0 * wc + i1] * tu) * (1.f - tv) +
+                    (y[j1 * wc + i0] * (1.f - tu) + y[j1 * wc + i1] * tu) * tv;
+    phi[(size_t)yy * w + xx] += sign * sig * nscale * (double)e;
+}
+
+// coarse FP64 block mean of phi
+__global__ void k_restrict_phi(const double* __restrict__ phi, int w, int f, int wc, int hc,
+                               double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= wc * hc) return;
+    const int J = i / wc, I = i - J * wc;
+    double acc = 0.0;
+    for (int yy = J * f; yy < J * f + f; ++yy)
+        for (int xx = I * f; xx < I * f + f; ++xx) acc += phi[(size_t)yy * w + xx];
+    out[i] = acc / (double)(f * f);
+}
+
+// target of one captured pair: (phi*_c - phi_k_c) / (rms * (nmax/pi)^2)
+__global__ void k_capture_finish(const double* __restrict__ final_c,
+                                 const double* __restrict__ phik_c,
+                                 const double* __restrict__ stats, int nc, double nscale,
+                                 float* __restrict__ t) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    const double mean = stats[0] / nc;
+    const double var = stats[1] / nc - mean * mean;
+    const double sig = sqrt(var > 1e-300 ? var : 1e-300);
+    t[i] = (float)((final_c[i] - phik_c[i]) / (sig * nscale));
+}
+
 }  // namespace
 
 int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, const uint8_t* mask,
@@ -701,7 +801,8 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     const int bgrid = div_up(ntiles, 8) * 8;
     const float om = (float)omega;
     const float om2 = om / 2.f, om3 = om / 3.f, om4 = om / 4.f;
-    auto enqueue_blocked = [&](int first, int count_sweeps) -> int {
+    auto enqueue_blocked_om = [&](int first, int count_sweeps, float om2, float om3,
+                                  float om4) -> int {
         CD_TRY(ctx->timer_begin());
         int launches = 0;
         for (int done_sw = 0; done_sw < count_sweeps;) {
@@ -729,6 +830,9 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         CD_TRY(ctx->timer_end(CD_KERNEL_SOR_SWEEP, launches, 12.0 * (double)n * count_sweeps));
         return CD_OK;
     };
+    auto enqueue_blocked = [&](int first, int count_sweeps) -> int {
+        return enqueue_blocked_om(first, count_sweeps, om2, om3, om4);
+    };
     auto enqueue_sweeps = [&](int first, int count_sweeps) -> int {
         if (blocked) return enqueue_blocked(first, count_sweeps);
         CD_TRY(ctx->timer_begin());
@@ -751,15 +855,123 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         return CD_OK;
     };
 
+    // ---- learned correction / capture geometry ------------------------------
+    const cd_net_arch* arch = ctx->net ? cd_net_arch_of(ctx->net) : nullptr;
+    const double nscale = std::pow((double)std::max(w, h) / M_PI, 2.0);
+    auto coarse_ok = [&](int f, int lv) {
+        const int m = f << (lv - 1);
+        return f > 0 && w % f == 0 && h % f == 0 && (w / f) % (1 << (lv - 1)) == 0 &&
+               (h / f) % (1 << (lv - 1)) == 0 && m > 0;
+    };
+    bool learned = o->learned_init && arch && blocked && o->stop == CD_STOP_RESIDUAL &&
+                   coarse_ok(arch->coarsen, arch->levels);
+    const int lf = arch ? arch->coarsen : 1;
+    const int lwc = w / std::max(lf, 1), lhc = h / std::max(lf, 1);
+    const bool capture = ctx->cap_x && ctx->cap_coarsen > 0 && w == ctx->cap_coarsen * ctx->cap_w &&
+                         h == ctx->cap_coarsen * ctx->cap_h && blocked;
+    const int cwc = capture ? ctx->cap_w : 0, chc = capture ? ctx->cap_h : 0;
+    constexpr int kMaxPairsPerSolve = 64;
+    std::vector<int> cap_slots;
+    if (learned) {
+        CD_TRY(ctx->ensure(ctx->lc_r, sizeof(float) * (size_t)lwc * lhc));
+        CD_TRY(ctx->ensure(ctx->lc_x, sizeof(float) * (size_t)lwc * lhc));
+        CD_TRY(ctx->ensure(ctx->lc_y, sizeof(float) * (size_t)lwc * lhc));
+        CD_TRY(ctx->ensure(ctx->lc_backup, sizeof(double) * n));  // phi before a correction
+    }
+    if (learned || capture)
+        CD_TRY(ctx->ensure(ctx->lc_stats, sizeof(double) * 2 * (kMaxPairsPerSolve + 2)));
+    if (capture) {
+        CD_TRY(ctx->ensure(ctx->cap_phi, sizeof(double) * (size_t)cwc * chc * (kMaxPairsPerSolve + 1)));
+        CD_TRY(ctx->ensure(ctx->lc_r, sizeof(float) * (size_t)std::max(cwc * chc, lwc * lhc)));
+    }
+    // restrict the current residual to (wc x hc), normalise into xout; stats slot
+    auto restrict_normalize = [&](int f, int wc, int hc, float* xout, double* stats) -> int {
+        const int nc = wc * hc;
+        const int nb = div_up(nc, 256);  // <= nred: the partials buffer is large enough
+        hipLaunchKernelGGL(k_restrict_residual, dim3(nb), dim3(256), 0, s, phi, b, sd, w, h, f,
+                           wc, hc, ctx->lc_r.as<float>(), ctx->partials.as<double>());
+        CD_CHECK_LAUNCH(ctx);
+        CD_TRY(cd_reduce_partials(ctx, ctx->partials.as<double>(), nb, 2, stats));
+        hipLaunchKernelGGL(k_normalize_coarse, dim3(nb), dim3(256), 0, s, ctx->lc_r.as<float>(),
+                           stats, nc, xout);
+        CD_CHECK_LAUNCH(ctx);
+        return CD_OK;
+    };
+    auto capture_pair = [&]() -> int {
+        if (ctx->cap_count >= ctx->cap_capacity || (int)cap_slots.size() >= kMaxPairsPerSolve)
+            return CD_OK;
+        const int slot = ctx->cap_count++;
+        const int i = (int)cap_slots.size();
+        const int nc = cwc * chc;
+        double* stats = ctx->lc_stats.as<double>() + 2 * (2 + i);
+        CD_TRY(restrict_normalize(ctx->cap_coarsen, cwc, chc, ctx->cap_x + (size_t)slot * nc, stats));
+        hipLaunchKernelGGL(k_restrict_phi, dim3(div_up(nc, 256)), dim3(256), 0, s, phi, w,
+                           ctx->cap_coarsen, cwc, chc, ctx->cap_phi.as<double>() + (size_t)(i + 1) * nc);
+        CD_CHECK_LAUNCH(ctx);
+        cap_slots.push_back(slot);
+        return CD_OK;
+    };
+    // the learned correction: phi += sign * P net(R r)
+    auto learned_correction = [&](double sign) -> int {
+        double* stats = ctx->lc_stats.as<double>();
+        if (sign > 0) {
+            CD_TRY(restrict_normalize(lf, lwc, lhc, ctx->lc_x.as<float>(), stats));
+            CD_TRY(cd_net_forward_internal(ctx->net, ctx->lc_x.as<float>(), ctx->lc_y.as<float>(),
+                                           lhc, lwc));
+        }
+        hipLaunchKernelGGL(k_prolong_add, dim3(div_up(w, 256), h), dim3(256), 0, s,
+                           ctx->lc_y.as<float>(), stats, lwc * lhc, w, h, lf, lwc, lhc, nscale,
+                           sign, phi);
+        CD_CHECK_LAUNCH(ctx);
+        return CD_OK;
+    };
+    const float oms = arch ? (float)arch->smooth_omega : om;
+    const int nsmooth = arch ? std::max(1, arch->smooth_sweeps) : 4;
+    int learned_applied = 0, learned_rejected = 0;
+
     std::vector<int> cum_at_check;
     int check_id = 0;
     int cum = 0;
     CD_TRY(enqueue_check(check_id));
     cum_at_check.push_back(0);
+    if (capture && !ctx->host_status->done) CD_TRY(capture_pair());
     double prev_rel = ctx->host_status->rel;
     int k = kmax;
     if (adaptive) k = std::min(kmax, std::max(8, kmax / 4));
     bool done = ctx->host_status->done != 0;
+    while (!done && learned && cum < o->max_sweeps) {
+        // one learned cycle: correction, residual, smoothing sweeps, residual;
+        // kept only if the FP64 residual after the cycle is lower than before
+        const double rel_before = ctx->host_status->rel;
+        CD_HIP(ctx, hipMemcpyAsync(ctx->lc_backup.ptr, phi, sizeof(double) * n,
+                                   hipMemcpyDeviceToDevice, s));
+        CD_TRY(ctx->timer_begin());
+        CD_TRY(learned_correction(+1.0));
+        CD_TRY(ctx->timer_end(CD_KERNEL_LEARNED, 1, 0.0));
+        CD_TRY(enqueue_check(++check_id));
+        cum_at_check.push_back(cum);
+        if (ctx->host_status->done) { ++learned_applied; done = true; break; }
+        const int nsw = std::min(nsmooth, o->max_sweeps - cum);
+        CD_TRY(enqueue_blocked_om(cum, nsw, oms / 2.f, oms / 3.f, oms / 4.f));
+        cum += nsw;
+        CD_TRY(enqueue_check(++check_id));
+        cum_at_check.push_back(cum);
+        const SolveDev& hs = *ctx->host_status;
+        if (hs.done) { ++learned_applied; done = true; break; }
+        if (std::isfinite(hs.rel) && hs.rel < rel_before) {
+            ++learned_applied;
+            continue;
+        }
+        // rejected: restore the iterate before the correction; plain SOR finishes
+        ++learned_rejected;
+        learned = false;
+        CD_HIP(ctx, hipMemcpyAsync(phi, ctx->lc_backup.ptr, sizeof(double) * n,
+                                   hipMemcpyDeviceToDevice, s));
+        CD_TRY(enqueue_check(++check_id));
+        cum_at_check.push_back(cum);
+        done = ctx->host_status->done != 0;
+        prev_rel = ctx->host_status->rel;
+    }
     while (!done && cum < o->max_sweeps) {
         const int nsw = std::min(k, o->max_sweeps - cum);
         CD_TRY(enqueue_sweeps(cum, nsw));
@@ -770,6 +982,8 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         const SolveDev& hs = *ctx->host_status;
         done = hs.done != 0;
         if (!std::isfinite(hs.rel)) break;
+        if (capture && !done && check_id % std::max(ctx->cap_every, 1) == 0)
+            CD_TRY(capture_pair());
         if (adaptive && !done) {
             // geometric rate estimate over the last batch -> sweeps still needed
             double rel = hs.rel;
@@ -782,6 +996,17 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
             k = std::max(8, std::min(kmax, want));
             prev_rel = rel;
         }
+    }
+    if (capture && !cap_slots.empty()) {
+        const int nc = cwc * chc;
+        double* fin = ctx->cap_phi.as<double>();  // slot 0: the final solution
+        hipLaunchKernelGGL(k_restrict_phi, dim3(div_up(nc, 256)), dim3(256), 0, s, phi, w,
+                           ctx->cap_coarsen, cwc, chc, fin);
+        for (size_t i = 0; i < cap_slots.size(); ++i)
+            hipLaunchKernelGGL(k_capture_finish, dim3(div_up(nc, 256)), dim3(256), 0, s, fin,
+                               fin + (i + 1) * nc, ctx->lc_stats.as<double>() + 2 * (2 + i), nc,
+                               nscale, ctx->cap_t + (size_t)cap_slots[i] * nc);
+        CD_CHECK_LAUNCH(ctx);
     }
     const SolveDev hs = *ctx->host_status;
     if (mask && o->max_sweeps > 0) {
@@ -805,6 +1030,8 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         st_out->rel_residual = hs.rel;
         st_out->initial_rel_residual = hs.rel0;
         st_out->last_max_update = hs.last_max_update;
+        st_out->learned_applied = learned_applied;
+        st_out->learned_rejected = learned_rejected;
         st_out->elapsed_ms = std::chrono::duration<double, std::milli>(
                                  std::chrono::steady_clock::now() - t_start).count();
     }
@@ -838,6 +1065,27 @@ int cd_poisson_solve_device(cd_context* ctx, const double* d_rhs, double* d_phi,
     const uint8_t* mask = holes > 0 ? ctx->mask.as<uint8_t>() : nullptr;
     return cd_solve_device_internal(ctx, d_rhs, d_phi, mask, (long long)(n - (size_t)holes), width,
                                     height, opts, stats);
+}
+
+int cd_context_set_capture(cd_context* ctx, float* d_x, float* d_t, int capacity, int h, int w,
+                           int coarsen, int every) {
+    if (!ctx) return CD_ERR_BAD_ARG;
+    if (d_x && (!d_t || capacity < 1 || h < 1 || w < 1 || coarsen < 1)) return CD_ERR_BAD_ARG;
+    ctx->cap_x = d_x;
+    ctx->cap_t = d_t;
+    ctx->cap_capacity = d_x ? capacity : 0;
+    ctx->cap_count = 0;
+    ctx->cap_h = h;
+    ctx->cap_w = w;
+    ctx->cap_coarsen = coarsen;
+    ctx->cap_every = every < 1 ? 1 : every;
+    return CD_OK;
+}
+
+int cd_context_capture_count(const cd_context* ctx, int* count) {
+    if (!ctx || !count) return CD_ERR_BAD_ARG;
+    *count = ctx->cap_count;
+    return CD_OK;
 }
 
 int cd_poisson_solve(cd_context* ctx, const double* rhs, double* phi, int width, int height,

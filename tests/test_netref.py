@@ -1,0 +1,44 @@
+"""Pins the FP64 network restatement (oracle/netref.py) by central finite
+differences; it is then the checker of the GPU network kernels."""
+import numpy as np
+
+from oracle.netref import NetRef, bf16
+
+
+def test_netref_gradient_finite_differences():
+    net = NetRef(levels=3, channels=(8, 16, 16))
+    p = net.init_params(3).astype(np.float64)
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((2, 8, 8))
+    t = rng.standard_normal((2, 8, 8)) * 0.1
+    loss, g = net.loss_grad(p, x, t)
+    nz = np.flatnonzero(p)
+    idx = rng.choice(nz, 25, replace=False)
+    eps = 1e-6
+    for i in idx:
+        pp, pm = p.copy(), p.copy()
+        pp[i] += eps
+        pm[i] -= eps
+        fd = (net.loss_grad(pp, x, t)[0] - net.loss_grad(pm, x, t)[0]) / (2 * eps)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (i, fd, g[i])
+
+
+def test_netref_shapes_and_layout():
+    net = NetRef()
+    assert len(net.convs) == 4 * 4 - 1
+    p = net.init_params(0)
+    assert p.size == net.nparam
+    y, _ = net.forward(p.astype(np.float64), np.zeros((1, 16, 16)))
+    assert y.shape == (1, 16, 16) and np.all(y == 0)  # bias-free: zero in, zero out
+    # positive homogeneity of a bias-free ReLU network
+    x = np.random.default_rng(1).standard_normal((1, 16, 16))
+    y1, _ = net.forward(p.astype(np.float64), x)
+    y2, _ = net.forward(p.astype(np.float64), 3.0 * x)
+    assert np.allclose(y2, 3.0 * y1)
+
+
+def test_bf16_rounding():
+    v = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 2 ** -9, 1.0 + 3 * 2 ** -9, -2.5])
+    r = bf16(v)
+    assert r[0] == 1.0 and r[1] == 1.0  # exact tie -> even mantissa
+    assert r[2] == 1.0 and r[3] == 1.0 + 2 ** -7 and r[4] == -2.5
