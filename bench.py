@@ -136,6 +136,55 @@ def device_copy(a) -> int:
     return 0
 
 
+def learned_leg(ctx, cd, one_design, wpath, a, plain_value) -> dict:
+    """The same designs with the learned coarse solve in every Poisson solve
+    (same stop rule and tolerance); timed like the plain leg."""
+    from poisson_caustic_design_amd import capi, learned
+
+    net = learned.load_net(ctx, wpath)
+    ctx.set_net(net)
+    o = capi.default_opts(tol=a.tol, learned_init=1)
+    cd.set_solve_opts(o, o)
+    try:
+        for i in range(a.warmup):
+            one_design()
+        ctx.synchronize()
+        runs = []
+        t0 = time.perf_counter()
+        for i in range(a.steps):
+            rs = one_design()
+            runs.append(rs.as_dict())
+            log(f"learned step {i}: {rs.elapsed_ms:.1f} ms, {rs.transport_steps} OT steps, "
+                f"{rs.phi_sweeps}+{rs.h_sweeps} sweeps, {rs.learned_applied} learned "
+                f"iterations, {rs.learned_rejected} fallbacks, max residual "
+                f"{max(rs.max_phi_residual, rs.max_h_residual):.2e}")
+        ctx.synchronize()
+        elapsed = time.perf_counter() - t0
+    finally:
+        ctx.set_net(None)
+        o = capi.default_opts(tol=a.tol)
+        cd.set_solve_opts(o, o)
+        net.close()
+    for r in runs:
+        if r["status"] != capi.OK:
+            raise SystemExit(f"learned design failed: status {r['status']}")
+    value = a.steps / elapsed
+    n = len(runs)
+    return {
+        "value": value,
+        "ms_per_step": 1e3 * elapsed / a.steps,
+        "speedup_vs_plain": value / plain_value,
+        "weights": os.path.relpath(wpath, ROOT),
+        "per_design": {
+            "ot_steps": sum(r["transport_steps"] for r in runs) / n,
+            "sweeps": sum(r["phi_sweeps"] + r["h_sweeps"] for r in runs) / n,
+            "learned_iterations": sum(r["learned_applied"] for r in runs) / n,
+            "fallbacks": sum(r["learned_rejected"] for r in runs) / n,
+            "max_residual": max(max(r["max_phi_residual"], r["max_h_residual"]) for r in runs),
+        },
+    }
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,6 +201,10 @@ def main(argv=None):
                     help="design: the benchmark; sweeps: a fixed number of SOR sweeps at the "
                          "workload grid (profiling); copy: a device copy of known size "
                          "(calibrates rocprofv3 byte counters)")
+    ap.add_argument("--learned", choices=["auto", "off"], default="auto",
+                    help="auto: when weights for the grid exist, also time the same designs "
+                         "with the learned coarse solve and report them beside the plain ones")
+    ap.add_argument("--weights", default=None)
     ap.add_argument("--sweeps", type=int, default=64)
     ap.add_argument("--mib", type=int, default=256)
     a = ap.parse_args(argv)
@@ -269,6 +322,10 @@ def main(argv=None):
         "time_split_ms": {"sor": sor_ms, "residual_checks": res_ms, "raster": ras_ms,
                           "vertex": vtx_ms, "total": elapsed * 1e3},
     }
+    wpath = a.weights or os.path.join(ROOT, "poisson_caustic_design_amd", "weights",
+                                      f"learned_grid{N}.npz")
+    if a.learned == "auto" and os.path.exists(wpath):
+        out["learned"] = learned_leg(ctx, cd, one_design, wpath, a, value)
     cb = None
     if not a.no_cpu_baseline:
         cb = cpu_baseline(a.config, a.seed, per_design, a.cpu_timeout)

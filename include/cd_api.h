@@ -194,17 +194,68 @@ int cd_design_set(cd_design* d, int field, const double* host_in, size_t count);
 const void* cd_design_device_ptr(cd_design* d, int field);
 int cd_design_last_solve_stats(const cd_design* d, cd_solve_stats* phi, cd_solve_stats* h);
 
-/* ---- learned initial correction (north_star; no reference counterpart) ----
+/* ---- input and output files (main.cpp, src/utils.cpp) -------------------- */
+/* Parameters a design was created with. */
+int cd_design_get_params(const cd_design* d, cd_design_params* p);
+
+/* PNG -> RGBA8 rows, top row first, with the conversions main.cpp:56-83 asks
+ * libpng for (16-bit stripped, palette and low-bit gray expanded, tRNS to
+ * alpha, filler 0xFF).  Gray and gray-alpha images give (g, g, g, a).  With
+ * rgba == NULL only *width, *height are set; otherwise capacity must be at
+ * least 4*width*height.  Replaces the decoding half of image_to_grid
+ * (main.cpp:29-109). */
+int cd_png_read_rgba8(const char* path, int* width, int* height, unsigned char* rgba,
+                      size_t capacity);
+/* Gray conversion 0.299 r + 0.587 g + 0.114 b of bytes/255 (main.cpp:95-98)
+ * at the pixels a nearest resize to out_w x out_h samples (resize_image,
+ * main.cpp:13-27), row-major FP64 out.  Host and device-pointer variants. */
+int cd_image_gray_resize(cd_context* ctx, const unsigned char* rgba, int width, int height,
+                         int out_w, int out_h, double* out);
+int cd_image_gray_resize_device(cd_context* ctx, const unsigned char* d_rgba, int width,
+                                int height, int out_w, int out_h, double* d_out);
+
+/* Solidified lens OBJ (save_solid_obj, src/utils.cpp:198-260): front points
+ * V x 3 (x, y, height), back points V x 3 (x, y used), res_x x res_y mesh. */
+int cd_obj_write_solid(const char* path, const double* front, const double* back, int res_x,
+                       int res_y, double thickness, double width, double height);
+/* heightmap.json of a row-major res_y x res_x grid (main.cpp:111-135). */
+int cd_json_write_heightmap(const char* path, const double* h, int res_x, int res_y);
+/* SVG of a point grid (export_grid_to_svg, src/utils.cpp:262-307) or of the
+ * structured triangles (export_triangles_to_svg, src/utils.cpp:309-344);
+ * points have `stride` doubles each, x and y first. */
+int cd_svg_write_grid(const char* path, const double* points, int stride, int res_x, int res_y,
+                      double width, double height, double stroke_width);
+int cd_svg_write_triangles(const char* path, const double* points, int stride, int res_x,
+                           int res_y, double width, double height, double stroke_width);
+
+enum { CD_OBJ_SOURCE = 0, CD_OBJ_TARGET = 1 };
+/* save_solid_obj_source / save_solid_obj_target (src/caustic_design.cpp:55-61). */
+int cd_design_save_solid_obj(cd_design* d, const char* path, int front);
+/* save_heightmap_as_json(caustic_design.h, ...) (main.cpp:269). */
+int cd_design_save_heightmap_json(cd_design* d, const char* path);
+enum { CD_SVG_PARAMETERIZATION = 0, CD_SVG_INVERTED_MAP = 1, CD_SVG_TRIANGLES = 2 };
+/* export_paramererization_to_svg, export_inverted_transport_map, export_to_svg
+ * (src/caustic_design.cpp:20-26, src/mesh.cpp:101-103,411-414,608-610). */
+int cd_design_export_svg(cd_design* d, const char* path, int what, double stroke_width);
+
+/* ---- learned coarse solve (no reference counterpart) ----------------------
  * A bias-free convolutional encoder-decoder (U-Net: `levels` resolution
  * levels, two 3x3 convolutions per level and per decoder stage, 2x2 average
- * pooling, nearest upsampling, concatenated skips, ReLU) maps the residual of
- * a Poisson solve, restricted to a grid `coarsen` times coarser and
- * normalised to unit RMS, to the correction, normalised by RMS*(n/pi)^2
- * (n = larger fine side).  BF16 weights and activations, FP32 accumulation
- * (gfx950 MFMA), FP32 master weights and Adam.  Solves with learned_init = 1
- * apply it at every convergence check, keep a correction only if it lowers the
- * FP64 residual, smooth with red-black SOR sweeps, and fall back to plain SOR
- * (which always finishes to the tolerance) once a correction is rejected. */
+ * pooling, nearest upsampling, concatenated skips; linear, or ReLU on the
+ * hidden convolutions) approximates the inverse of the Neumann 5-point
+ * operator on a grid `coarsen` times coarser than the solve: its input is a
+ * coarse residual normalised to unit RMS, its output the coarse solution
+ * divided by (n/pi)^2 (n = larger coarse side).  BF16 weights and
+ * activations, FP32 accumulation (gfx950 MFMA), FP32 master weights and Adam.
+ *
+ * Solves with learned_init = 1 (FP32-mixed precision, no holes) start with
+ * flexible conjugate-gradient iterations preconditioned by one two-grid
+ * cycle: restrict the residual (block mean), apply the network, prolong
+ * (bilinear) and run `smooth_sweeps` red-black SOR sweeps.  The first cycle
+ * is the learned initial correction.  Every iteration recomputes the FP64
+ * residual; when eight iterations bring no halving of it the solve continues
+ * with plain SOR from the current iterate, so the stop rule and tolerance are
+ * the same as without the network. */
 typedef struct cd_net cd_net;
 typedef struct {
     int levels;        /* resolution levels (>= 2)                          */
@@ -212,6 +263,7 @@ typedef struct {
     int coarsen;       /* fine cells per coarse cell along each axis (4)     */
     int smooth_sweeps; /* red-black sweeps between learned corrections (4)   */
     double smooth_omega; /* over-relaxation of those sweeps (1.2)            */
+    int linear;        /* 1: no activation (a linear operator, default)      */
 } cd_net_arch;
 
 void cd_net_arch_default(cd_net_arch* a);
@@ -232,6 +284,16 @@ int cd_net_loss_grad(cd_net* net, const float* d_x, const float* d_t, int batch,
  * mean squared error before the step. */
 int cd_net_train_step(cd_net* net, const float* d_x, const float* d_t, int batch, int h, int w,
                       float lr, double* loss);
+/* Operator loss of the coarse solve: with y the forward output, Lc the coarse
+ * Neumann 5-point operator and cs = (n/pi)^2, the mean of (cs*Lc y - x)^2 and
+ * its parameter gradient.  Needs no targets; it measures how far Lc times the
+ * network is from the identity, the quantity that sets the convergence of the
+ * preconditioned iterations. */
+int cd_net_loss_grad_operator(cd_net* net, const float* d_x, int batch, int h, int w,
+                              double* loss, float* host_grad, size_t n);
+/* One Adam step on the operator loss; *loss is the loss before the step. */
+int cd_net_train_step_operator(cd_net* net, const float* d_x, int batch, int h, int w, float lr,
+                               double* loss);
 /* Attach (or detach with NULL) the network used by solves with learned_init. */
 int cd_context_set_net(cd_context* ctx, cd_net* net);
 

@@ -303,10 +303,23 @@ class RefLib:
         L.ref_set_grid.argtypes = [_P, _I, _P]
         L.ref_get_vector.argtypes = [_P, _I, _P]
         L.ref_inverted_map.argtypes = [_P, _P]
+        L.ref_save_solid_obj.argtypes = [C.c_char_p, _P, _P, _I, _I, _D, _D, _D]
+        L.ref_export_grid_svg.argtypes = [C.c_char_p, _P, _I, _I, _I, _D, _D, _D]
+        L.ref_export_triangles_svg.argtypes = [C.c_char_p, _P, _I, _I, _I, _D, _D, _D]
 
     @staticmethod
     def available(path: str = REF_SO) -> bool:
         return os.path.exists(path)
+
+    def save_solid_obj(self, path, front, back, res_x, res_y, thickness, width, height):
+        f, b = _f64(front), _f64(back)
+        self.L.ref_save_solid_obj(path.encode(), f.ctypes.data, b.ctypes.data, res_x, res_y,
+                                  thickness, width, height)
+
+    def export_svg(self, path, pts, res_x, res_y, width, height, stroke, triangles=False):
+        p = _f64(pts)
+        fn = self.L.ref_export_triangles_svg if triangles else self.L.ref_export_grid_svg
+        fn(path.encode(), p.ctypes.data, p.shape[1], res_x, res_y, width, height, stroke)
 
     def poisson_solver(self, D, phi, max_iterations, threshold, threads=1):
         D = _f64(D)

@@ -34,9 +34,10 @@ class Conv:
 
 
 class NetRef:
-    def __init__(self, levels=4, channels=(16, 32, 64, 64)):
+    def __init__(self, levels=4, channels=(16, 32, 64, 64), linear=False):
         self.L = levels
         self.C = list(channels)[:levels]
+        self.act = 0 if linear else 1
         self.convs: list[Conv] = []
         self.tensors: list[tuple[int, int]] = []  # (level, channels)
         self.nparam = 0
@@ -60,18 +61,18 @@ class NetRef:
             if lv > 0:
                 inp = self._tensor(lv, prevC)
             a = self._tensor(lv, C[lv])
-            self._conv(inp, inC, -1, 0, a, C[lv], C[lv], 1, lv)
+            self._conv(inp, inC, -1, 0, a, C[lv], C[lv], self.act, lv)
             s = self._tensor(lv, C[lv])
-            self._conv(a, C[lv], -1, 0, s, C[lv], C[lv], 1, lv)
+            self._conv(a, C[lv], -1, 0, s, C[lv], C[lv], self.act, lv)
             skip[lv] = s
             prev, prevC = s, C[lv]
         d, dC = skip[L - 1], C[L - 1]
         for lv in range(L - 2, -1, -1):
             u = self._tensor(lv, dC)
             b = self._tensor(lv, C[lv])
-            self._conv(u, dC, skip[lv], C[lv], b, C[lv], C[lv], 1, lv)
+            self._conv(u, dC, skip[lv], C[lv], b, C[lv], C[lv], self.act, lv)
             e = self._tensor(lv, C[lv])
-            self._conv(b, C[lv], -1, 0, e, C[lv], C[lv], 1, lv)
+            self._conv(b, C[lv], -1, 0, e, C[lv], C[lv], self.act, lv)
             d, dC = e, C[lv]
         self._conv(d, dC, -1, 0, -1, 1, 8, 0, 0)
 
@@ -90,7 +91,11 @@ class NetRef:
         p = np.zeros(self.nparam, dtype=np.float32)
         for i, cv in enumerate(self.convs):
             cin = cv.c1 + cv.c2
-            std = np.sqrt(2.0 / (9 * cin)) * (0.1 if cv.out < 0 else 1.0)
+            fan = 9 * (1 if i == 0 else cin)
+            if self.act:
+                std = np.sqrt(2.0 / fan) * (0.1 if cv.out < 0 else 1.0)
+            else:
+                std = np.sqrt(1.0 / (3.0 * fan))
             w = np.zeros((cv.coutpad, cv.kpad), dtype=np.float32)
             vals = rng.standard_normal((cv.cout, cv.K)).astype(np.float32) * std
             if i == 0:
@@ -174,14 +179,35 @@ class NetRef:
         return y, (acts, cache)
 
     def loss_grad(self, params, x, t, emulate_bf16=False):
-        y, (acts, cache) = self.forward(params, x, emulate_bf16)
-        n = y.size
+        """Mean squared error against targets t and its parameter gradient."""
+        y, cache = self.forward(params, x, emulate_bf16)
         loss = float(np.mean((y - t) ** 2))
-        grads = [np.zeros((self.tensors[i][1],)) for i in range(0)]  # placeholder
-        g_t = [None] * len(self.tensors)
-        for i, (lv, ch) in enumerate(self.tensors):
-            a = acts[i]
-            g_t[i] = np.zeros_like(a) if a is not None else None
+        return loss, self._backward(cache, 2.0 * (y - t) / y.size)
+
+    @staticmethod
+    def lap(v):
+        """Neumann 5-point operator (sum of in-grid neighbours - count * centre)
+        on the last two axes."""
+        out = np.zeros_like(v)
+        cnt = np.zeros(v.shape[-2:])
+        out[..., :, 1:] += v[..., :, :-1]; cnt[:, 1:] += 1
+        out[..., :, :-1] += v[..., :, 1:]; cnt[:, :-1] += 1
+        out[..., 1:, :] += v[..., :-1, :]; cnt[1:, :] += 1
+        out[..., :-1, :] += v[..., 1:, :]; cnt[:-1, :] += 1
+        return out - cnt * v
+
+    def loss_grad_operator(self, params, x, emulate_bf16=False):
+        """Operator loss mean((cs * Lc y - x)^2), cs = (max(H, W)/pi)^2, and its
+        parameter gradient ((2 cs / n) Lc u into the head; Lc is symmetric)."""
+        y, cache = self.forward(params, x, emulate_bf16)
+        cs = (max(x.shape[1], x.shape[2]) / np.pi) ** 2
+        u = cs * self.lap(y) - x
+        loss = float(np.mean(u ** 2))
+        return loss, self._backward(cache, 2.0 * cs / y.size * self.lap(u))
+
+    def _backward(self, fwd_cache, g_y):
+        acts, cache = fwd_cache
+        g_t = [np.zeros_like(a) if a is not None else None for a in acts]
         gflat = np.zeros(self.nparam)
         L = self.L
         for idx in range(len(self.convs) - 1, -1, -1):
@@ -189,7 +215,7 @@ class NetRef:
             xin, wt, out = cache[idx]
             if cv.out < 0:
                 g = np.zeros(out.shape)
-                g[..., 0] = 2.0 * (y - t) / n
+                g[..., 0] = g_y
             else:
                 g = g_t[cv.out] * (out > 0 if cv.relu else 1.0)
             dx, dw = self.conv_bwd(xin, wt, g)
@@ -207,5 +233,4 @@ class NetRef:
                 if idx < 2 * L and idx % 2 == 0 and cv.level > 0:
                     pv = self.convs[idx - 1]
                     g_t[pv.out] += self.pool_bwd(g_t[cv.in1])
-        del grads
-        return loss, gflat
+        return gflat

@@ -18,6 +18,9 @@
 //   bilinearInterpolation       src/caustic_design.cpp:156-188
 //   integrate_cell_intensities  src/polygon_utils.cpp:311-366
 //   Mesh::calculate_inverted_transport_map  src/mesh.cpp:348-409
+//   save_solid_obj              src/utils.cpp:198-260
+//   export_grid_to_svg          src/utils.cpp:262-307
+//   export_triangles_to_svg     src/utils.cpp:309-344
 
 #include <unistd.h>
 #include <fcntl.h>
@@ -249,6 +252,41 @@ int ref_inverted_map(void* h, double* xy) {
     }
     for (size_t i = 0; i < inv.size(); ++i) { xy[2 * i] = inv[i][0]; xy[2 * i + 1] = inv[i][1]; }
     return (int)inv.size();
+}
+
+// points as V x `stride` doubles -> the reference's {x, y, z} vectors
+static std::vector<std::vector<double>> to_points(const double* a, int n, int stride) {
+    std::vector<std::vector<double>> p(n, std::vector<double>(3, 0.0));
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < stride && k < 3; ++k) p[i][k] = a[(size_t)i * stride + k];
+    return p;
+}
+
+// The writers run through the public Mesh methods, which supply the
+// reference's own (private) structured triangulation (mesh.cpp:57-63).
+void ref_save_solid_obj(const char* path, const double* front, const double* back, int res_x,
+                        int res_y, double thickness, double width, double height) {
+    Quiet q;
+    Mesh m(width, height, res_x, res_y);
+    m.target_points = to_points(front, res_x * res_y, 3);
+    m.source_points = to_points(back, res_x * res_y, 3);
+    m.save_solid_obj_target(thickness, path);  // mesh.cpp:757-759
+}
+
+void ref_export_grid_svg(const char* path, const double* pts, int stride, int res_x, int res_y,
+                         double width, double height, double stroke) {
+    Quiet q;
+    Mesh m(width, height, res_x, res_y);
+    m.target_points = to_points(pts, res_x * res_y, stride);
+    m.export_paramererization_to_svg(path, stroke);  // mesh.cpp:608-610
+}
+
+void ref_export_triangles_svg(const char* path, const double* pts, int stride, int res_x,
+                              int res_y, double width, double height, double stroke) {
+    Quiet q;
+    Mesh m(width, height, res_x, res_y);
+    m.target_points = to_points(pts, res_x * res_y, stride);
+    m.export_to_svg(path, stroke);  // mesh.cpp:101-103
 }
 
 }  // extern "C"

@@ -24,7 +24,7 @@ BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "libcaustic_amd.so")
 
 HIP_SOURCES = ["cd_context.hip", "cd_solver.hip", "cd_design.hip", "cd_learned.hip"]
-CXX_SOURCES = ["dropin_solver.cpp", "dropin_caustic_design.cpp"]
+CXX_SOURCES = ["dropin_solver.cpp", "dropin_caustic_design.cpp", "cd_io.cpp"]
 
 ARCH = os.environ.get("CD_OFFLOAD_ARCH", "gfx950")
 
@@ -72,7 +72,7 @@ def build(verbose: bool = False, defines=(), out: str = LIB) -> str:
     objs = [j[-1] for j in jobs]
     tmp = out + ".tmp"
     _run([hipcc, "-shared", "-fPIC", "--offload-arch=" + ARCH, *objs, "-o", tmp,
-          "-Wl,-soname,libcaustic_amd.so"])
+          "-Wl,-soname,libcaustic_amd.so", "-lz"])
     os.replace(tmp, out)
     return out
 

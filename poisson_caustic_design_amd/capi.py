@@ -71,7 +71,7 @@ class RunStats(C.Structure):
 
 class NetArch(C.Structure):
     _fields_ = [("levels", _I), ("channels", _I * 6), ("coarsen", _I), ("smooth_sweeps", _I),
-                ("smooth_omega", _D)]
+                ("smooth_omega", _D), ("linear", _I)]
 
 
 class CDError(RuntimeError):
@@ -120,6 +120,17 @@ def load(path: str = LIB_PATH):
         "cd_design_set": ([_P, _I, _P, _SZ], _I),
         "cd_design_device_ptr": ([_P, _I], _P),
         "cd_design_last_solve_stats": ([_P, C.POINTER(SolveStats), C.POINTER(SolveStats)], _I),
+        "cd_design_get_params": ([_P, C.POINTER(DesignParams)], _I),
+        "cd_png_read_rgba8": ([C.c_char_p, C.POINTER(_I), C.POINTER(_I), _P, _SZ], _I),
+        "cd_image_gray_resize": ([_P, _P, _I, _I, _I, _I, _P], _I),
+        "cd_image_gray_resize_device": ([_P, _P, _I, _I, _I, _I, _P], _I),
+        "cd_obj_write_solid": ([C.c_char_p, _P, _P, _I, _I, _D, _D, _D], _I),
+        "cd_json_write_heightmap": ([C.c_char_p, _P, _I, _I], _I),
+        "cd_svg_write_grid": ([C.c_char_p, _P, _I, _I, _I, _D, _D, _D], _I),
+        "cd_svg_write_triangles": ([C.c_char_p, _P, _I, _I, _I, _D, _D, _D], _I),
+        "cd_design_save_solid_obj": ([_P, C.c_char_p, _I], _I),
+        "cd_design_save_heightmap_json": ([_P, C.c_char_p], _I),
+        "cd_design_export_svg": ([_P, C.c_char_p, _I, _D], _I),
         "cd_net_arch_default": ([C.POINTER(NetArch)], None),
         "cd_net_create": ([_P, C.POINTER(NetArch), C.POINTER(_P)], _I),
         "cd_net_destroy": ([_P], None),
@@ -130,6 +141,8 @@ def load(path: str = LIB_PATH):
         "cd_net_forward": ([_P, _P, _P, _I, _I, _I], _I),
         "cd_net_loss_grad": ([_P, _P, _P, _I, _I, _I, C.POINTER(_D), _P, _SZ], _I),
         "cd_net_train_step": ([_P, _P, _P, _I, _I, _I, C.c_float, C.POINTER(_D)], _I),
+        "cd_net_loss_grad_operator": ([_P, _P, _I, _I, _I, C.POINTER(_D), _P, _SZ], _I),
+        "cd_net_train_step_operator": ([_P, _P, _I, _I, _I, C.c_float, C.POINTER(_D)], _I),
         "cd_context_set_net": ([_P, _P], _I),
         "cd_context_set_capture": ([_P, _P, _P, _I, _I, _I, _I, _I], _I),
         "cd_context_capture_count": ([_P, C.POINTER(_I)], _I),
@@ -154,8 +167,12 @@ def exported_symbols():
         "cd_design_device_ptr", "cd_design_last_solve_stats", "cd_net_arch_default",
         "cd_net_create", "cd_net_destroy", "cd_net_num_params", "cd_net_init_params",
         "cd_net_set_params", "cd_net_get_params", "cd_net_forward", "cd_net_loss_grad",
-        "cd_net_train_step", "cd_context_set_net", "cd_context_set_capture",
-        "cd_context_capture_count",
+        "cd_net_train_step", "cd_net_loss_grad_operator", "cd_net_train_step_operator",
+        "cd_context_set_net", "cd_context_set_capture",
+        "cd_context_capture_count", "cd_design_get_params", "cd_png_read_rgba8",
+        "cd_image_gray_resize", "cd_image_gray_resize_device", "cd_obj_write_solid",
+        "cd_json_write_heightmap", "cd_svg_write_grid", "cd_svg_write_triangles",
+        "cd_design_save_solid_obj", "cd_design_save_heightmap_json", "cd_design_export_svg",
     ]
 
 
@@ -376,6 +393,31 @@ class CausticDesign:
         self.ctx.check(rc, "cd_design_run", allow)
         return rs
 
+    # file outputs (caustic_design.cpp:20-26,55-61; main.cpp:269)
+    def save_solid_obj_source(self, filename: str):
+        self.ctx.check(self.L.cd_design_save_solid_obj(self.d, filename.encode(), 0),
+                       "save_solid_obj_source")
+
+    def save_solid_obj_target(self, filename: str):
+        self.ctx.check(self.L.cd_design_save_solid_obj(self.d, filename.encode(), 1),
+                       "save_solid_obj_target")
+
+    def save_heightmap_json(self, filename: str):
+        self.ctx.check(self.L.cd_design_save_heightmap_json(self.d, filename.encode()),
+                       "save_heightmap_json")
+
+    def export_paramererization_to_svg(self, filename: str, line_width: float):
+        self.ctx.check(self.L.cd_design_export_svg(self.d, filename.encode(), 0, line_width),
+                       "export_paramererization_to_svg")
+
+    def export_inverted_transport_map(self, filename: str, stroke_width: float):
+        self.ctx.check(self.L.cd_design_export_svg(self.d, filename.encode(), 1, stroke_width),
+                       "export_inverted_transport_map")
+
+    def export_to_svg(self, filename: str, stroke_width: float):
+        self.ctx.check(self.L.cd_design_export_svg(self.d, filename.encode(), 2, stroke_width),
+                       "export_to_svg")
+
     def get(self, name: str) -> np.ndarray:
         f = FIELD[name]
         n = self.L.cd_design_field_size(self.d, f)
@@ -417,6 +459,55 @@ class CausticDesign:
     pixels = property(lambda self: self.get("pixels"))
     normals = property(lambda self: self.get("normals"))
     vertex_gradient = property(lambda self: self.get("vertex_gradient").T.copy())
+
+
+def png_read_rgba8(path: str) -> np.ndarray:
+    """PNG -> [h][w][4] uint8 with the reference's libpng conversions."""
+    L = load()
+    w, h = _I(), _I()
+    rc = L.cd_png_read_rgba8(path.encode(), C.byref(w), C.byref(h), None, 0)
+    if rc != OK:
+        raise CDError(rc, f"cd_png_read_rgba8({path}): {status_string(rc)}")
+    out = np.empty((h.value, w.value, 4), dtype=np.uint8)
+    rc = L.cd_png_read_rgba8(path.encode(), C.byref(w), C.byref(h), out.ctypes.data, out.size)
+    if rc != OK:
+        raise CDError(rc, f"cd_png_read_rgba8({path}): {status_string(rc)}")
+    return out
+
+
+def image_gray_resize(ctx: Context, rgba: np.ndarray, out_w: int, out_h: int) -> np.ndarray:
+    """Gray + nearest resize on the device (main.cpp:13-27, 95-98)."""
+    rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
+    h, w = rgba.shape[:2]
+    out = np.empty((out_h, out_w))
+    ctx.check(ctx.L.cd_image_gray_resize(ctx.h, rgba.ctypes.data, w, h, out_w, out_h,
+                                         out.ctypes.data), "cd_image_gray_resize")
+    return out
+
+
+def obj_write_solid(path: str, front, back, res_x: int, res_y: int, thickness: float,
+                    width: float, height: float):
+    f, b = _f64(front), _f64(back)
+    rc = load().cd_obj_write_solid(path.encode(), f.ctypes.data, b.ctypes.data, res_x, res_y,
+                                   thickness, width, height)
+    if rc != OK:
+        raise CDError(rc, f"cd_obj_write_solid({path})")
+
+
+def json_write_heightmap(path: str, h):
+    h = _f64(h)
+    rc = load().cd_json_write_heightmap(path.encode(), h.ctypes.data, h.shape[1], h.shape[0])
+    if rc != OK:
+        raise CDError(rc, f"cd_json_write_heightmap({path})")
+
+
+def svg_write(path: str, points, res_x: int, res_y: int, width: float, height: float,
+              stroke_width: float, triangles: bool = False):
+    p = _f64(points)
+    fn = load().cd_svg_write_triangles if triangles else load().cd_svg_write_grid
+    rc = fn(path.encode(), p.ctypes.data, p.shape[1], res_x, res_y, width, height, stroke_width)
+    if rc != OK:
+        raise CDError(rc, f"svg write ({path})")
 
 
 def default_arch(**kw) -> NetArch:
@@ -479,4 +570,18 @@ class Net:
         loss = _D()
         self.ctx.check(self.L.cd_net_train_step(self.h, d_x, d_t, batch, h, w, lr, C.byref(loss)),
                        "net train_step")
+        return loss.value
+
+    def loss_grad_operator(self, d_x: int, batch: int, h: int, w: int):
+        loss = _D()
+        g = np.empty(self.n, dtype=np.float32)
+        self.ctx.check(self.L.cd_net_loss_grad_operator(self.h, d_x, batch, h, w, C.byref(loss),
+                                                        g.ctypes.data, self.n),
+                       "net loss_grad_operator")
+        return loss.value, g
+
+    def train_step_operator(self, d_x: int, batch: int, h: int, w: int, lr: float) -> float:
+        loss = _D()
+        self.ctx.check(self.L.cd_net_train_step_operator(self.h, d_x, batch, h, w, lr,
+                                                         C.byref(loss)), "net train_step_operator")
         return loss.value

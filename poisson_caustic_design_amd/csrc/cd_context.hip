@@ -186,7 +186,9 @@ void cd_context_destroy(cd_context* ctx) {
     ctx->lc_x.release();
     ctx->lc_y.release();
     ctx->lc_stats.release();
-    ctx->lc_backup.release();
+    ctx->fcg_p.release();
+    ctx->fcg_q.release();
+    ctx->fcg_dev.release();
     ctx->cap_phi.release();
     if (ctx->host_status) (void)hipHostFree(ctx->host_status);
     for (auto& p : ctx->pending) {

@@ -76,6 +76,22 @@ __device__ __forceinline__ double block_max(double v, double* smem) {
 // initialisation (caustic_design.cpp:334-364)
 // ---------------------------------------------------------------------------
 // min/max over non-NaN pixels (utils.cpp:104-115)
+// image ingest (main.cpp:13-27, 95-98, 222): gray of the RGBA8 source pixel
+// a nearest resize samples, src = (x * old_w / new_w, y * old_h / new_h) in
+// integer arithmetic; gray = 0.299 r + 0.587 g + 0.114 b of the bytes / 255
+// (same operation order, no contraction: bit-equal to the reference)
+__global__ __launch_bounds__(256) void k_gray_resize(const unsigned char* __restrict__ rgba,
+                                                     int w, int h, int ow, int oh,
+                                                     double* __restrict__ out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)ow * oh) return;
+    const int y = (int)(i / ow), x = (int)(i - (long long)y * ow);
+    const int sx = (int)((long long)x * w / ow), sy = (int)((long long)y * h / oh);
+    const unsigned char* px = rgba + ((size_t)sy * w + sx) * 4;
+    const double r = px[0] / 255.0, g = px[1] / 255.0, b = px[2] / 255.0;
+    out[i] = (0.299 * r) + (0.587 * g) + (0.114 * b);
+}
+
 __global__ __launch_bounds__(256) void k_minmax(const double* __restrict__ img, size_t n,
                                                 double* __restrict__ partials) {
     __shared__ double smem[16];
@@ -884,6 +900,42 @@ int cd_design_run(cd_design* d, int max_transport_steps, double conv_tres, int h
     r.elapsed_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (rs) *rs = r;
     return status;
+}
+
+int cd_image_gray_resize_device(cd_context* c, const unsigned char* d_rgba, int width,
+                                int height, int out_w, int out_h, double* d_out) {
+    if (!c || !d_rgba || !d_out || width < 1 || height < 1 || out_w < 1 || out_h < 1)
+        return CD_ERR_BAD_ARG;
+    CD_HIP(c, hipSetDevice(c->device));
+    const long long n = (long long)out_w * out_h;
+    hipLaunchKernelGGL(k_gray_resize, dim3(div_up(n, 256)), dim3(256), 0, c->stream, d_rgba,
+                       width, height, out_w, out_h, d_out);
+    CD_CHECK_LAUNCH(c);
+    return CD_OK;
+}
+
+int cd_image_gray_resize(cd_context* c, const unsigned char* rgba, int width, int height,
+                         int out_w, int out_h, double* out) {
+    if (!c || !rgba || !out || width < 1 || height < 1 || out_w < 1 || out_h < 1)
+        return CD_ERR_BAD_ARG;
+    CD_HIP(c, hipSetDevice(c->device));
+    const size_t in_bytes = (size_t)width * height * 4;
+    const size_t n = (size_t)out_w * out_h;
+    DevBuf din, dout;
+    CD_TRY(c->ensure(din, in_bytes));
+    CD_TRY(c->ensure(dout, sizeof(double) * n));
+    CD_HIP(c, hipMemcpyAsync(din.ptr, rgba, in_bytes, hipMemcpyHostToDevice, c->stream));
+    CD_TRY(cd_image_gray_resize_device(c, din.as<unsigned char>(), width, height, out_w, out_h,
+                                       dout.as<double>()));
+    CD_HIP(c, hipMemcpyAsync(out, dout.ptr, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    CD_HIP(c, hipStreamSynchronize(c->stream));
+    return CD_OK;
+}
+
+int cd_design_get_params(const cd_design* d, cd_design_params* p) {
+    if (!d || !p) return CD_ERR_BAD_ARG;
+    *p = d->p;
+    return CD_OK;
 }
 
 size_t cd_design_field_size(const cd_design* d, int field) {

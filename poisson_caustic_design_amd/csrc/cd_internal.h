@@ -102,7 +102,8 @@ struct cd_context {
 
     // learned-init network (owned by the caller) and its solver workspace
     cd_net* net = nullptr;
-    DevBuf lc_r, lc_x, lc_y, lc_stats, lc_backup;
+    DevBuf lc_r, lc_x, lc_y, lc_stats;
+    DevBuf fcg_p, fcg_q, fcg_dev;  // learned FCG: direction, L direction, coefficients
 
     // training-pair capture (device buffers owned by the caller)
     float* cap_x = nullptr;

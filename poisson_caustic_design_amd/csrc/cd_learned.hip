@@ -295,6 +295,48 @@ __global__ __launch_bounds__(256) void k_loss(const float* __restrict__ head, co
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
+// operator (residual) loss of the coarse solve: with y the head output and
+// Lc the coarse Neumann 5-point operator, u = cs * Lc y - x and
+// loss = mean(u^2); the head gradient is (2 cs / npix) Lc u (Lc symmetric)
+__device__ __forceinline__ float lap_nhwc8(const float* __restrict__ v, int stride, long long p,
+                                           int x, int y, int H, int W) {
+    const float c = v[p * stride];
+    float s = 0.f, cnt = 0.f;
+    if (x > 0) { s += v[(p - 1) * stride]; cnt += 1.f; }
+    if (x < W - 1) { s += v[(p + 1) * stride]; cnt += 1.f; }
+    if (y > 0) { s += v[(p - W) * stride]; cnt += 1.f; }
+    if (y < H - 1) { s += v[(p + W) * stride]; cnt += 1.f; }
+    return s - cnt * c;
+}
+
+__global__ __launch_bounds__(256) void k_op_residual(const float* __restrict__ head,
+                                                     const float* __restrict__ x,
+                                                     float* __restrict__ u, int B, int H, int W,
+                                                     float cs, double* __restrict__ partials) {
+    __shared__ double smem[16];
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double e2 = 0.0;
+    if (p < (long long)B * H * W) {
+        const int xx = (int)(p % W), yy = (int)((p / W) % H);
+        const float d = cs * lap_nhwc8(head, 8, p, xx, yy, H, W) - x[p];
+        u[p] = d;
+        e2 = (double)d * d;
+    }
+    const double s = cd_block_sum(e2, smem);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_op_grad(const float* __restrict__ u,
+                                                 float* __restrict__ ghead, int B, int H, int W,
+                                                 float scale) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (long long)B * H * W) return;
+    const int xx = (int)(p % W), yy = (int)((p / W) % H);
+    ghead[p * 8] = scale * lap_nhwc8(u, 1, p, xx, yy, H, W);
+#pragma unroll
+    for (int c = 1; c < 8; ++c) ghead[p * 8 + c] = 0.f;
+}
+
 __global__ void k_head_out(const float* __restrict__ head, float* __restrict__ y, long long npix) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p < npix) y[p] = head[p * 8];
@@ -370,6 +412,7 @@ struct cd_net {
     std::vector<NetConv> convs;
     size_t nparam = 0;
     DevBuf w, m, v, gw, wbf, wtbf, partials, head, ghead, x8;
+    DevBuf ures;  // operator-loss residual [B][H][W]
     size_t wtoff_total = 0;
     std::vector<size_t> wtoff;
     long long step = 0;
@@ -486,6 +529,7 @@ void build_layers(cd_net* n) {
         cv.in1 = in1; cv.in2 = in2; cv.out = out;
         n->convs.push_back(cv);
     };
+    const int act = n->arch.linear ? 0 : 1;  // ReLU on hidden convolutions
     std::vector<int> skip(L);
     int prev = -1, prevC = 8;  // -1: the 8-channel input tensor
     for (int l = 0; l < L; ++l) {
@@ -496,9 +540,9 @@ void build_layers(cd_net* n) {
             in = p;
         }
         const int a = tensor(l, C[l]);
-        conv(in, inC, -1, 0, a, C[l], C[l], 1, l);
+        conv(in, inC, -1, 0, a, C[l], C[l], act, l);
         const int s = tensor(l, C[l]);
-        conv(a, C[l], -1, 0, s, C[l], C[l], 1, l);
+        conv(a, C[l], -1, 0, s, C[l], C[l], act, l);
         skip[l] = s;
         prev = s;
         prevC = C[l];
@@ -507,9 +551,9 @@ void build_layers(cd_net* n) {
     for (int l = L - 2; l >= 0; --l) {
         const int u = tensor(l, dC);  // upsampled
         const int b = tensor(l, C[l]);
-        conv(u, dC, skip[l], C[l], b, C[l], C[l], 1, l);
+        conv(u, dC, skip[l], C[l], b, C[l], C[l], act, l);
         const int e = tensor(l, C[l]);
-        conv(b, C[l], -1, 0, e, C[l], C[l], 1, l);
+        conv(b, C[l], -1, 0, e, C[l], C[l], act, l);
         d = e;
         dC = C[l];
     }
@@ -651,12 +695,54 @@ int loss_and_head_grad(cd_net* n, const float* d_t, int B, int H, int W, double*
     return CD_OK;
 }
 
+// coarse operator scale (n/pi)^2, n the larger side: makes outputs O(1)
+inline float op_scale(int H, int W) {
+    const double m = (double)std::max(H, W) / M_PI;
+    return (float)(m * m);
+}
+
+int op_loss_and_head_grad(cd_net* n, const float* d_x, int B, int H, int W, double* loss,
+                          bool want_grad) {
+    cd_context* c = n->ctx;
+    const long long px = (long long)B * H * W;
+    const int nb = div_up(px, 256);
+    CD_TRY(c->ensure(n->ures, sizeof(float) * px));
+    const float cs = op_scale(H, W);
+    hipLaunchKernelGGL(k_op_residual, dim3(nb), dim3(256), 0, c->stream, n->head.as<float>(), d_x,
+                       n->ures.as<float>(), B, H, W, cs, n->partials.as<double>());
+    if (want_grad)
+        hipLaunchKernelGGL(k_op_grad, dim3(nb), dim3(256), 0, c->stream, n->ures.as<float>(),
+                           n->ghead.as<float>(), B, H, W, 2.f * cs / (float)px);
+    CD_CHECK_LAUNCH(c);
+    CD_TRY(c->ensure(c->fill, sizeof(double) * 4));
+    CD_TRY(cd_reduce_partials(c, n->partials.as<double>(), nb, 1, c->fill.as<double>()));
+    double s = 0.0;
+    CD_HIP(c, hipMemcpyAsync(&s, c->fill.ptr, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CD_HIP(c, hipStreamSynchronize(c->stream));
+    if (loss) *loss = s / (double)px;
+    return CD_OK;
+}
+
 int prep(cd_net* n, const float* d_x, int B, int H, int W) {
     CD_TRY(ensure_batch(n, B, H, W));
     const long long px = (long long)B * H * W;
     hipLaunchKernelGGL(k_prep_input, dim3(div_up(px, 256)), dim3(256), 0, n->ctx->stream, d_x,
                        n->x8.as<u16>(), px);
     CD_CHECK_LAUNCH(n->ctx);
+    return CD_OK;
+}
+
+// one Adam step (beta 0.9/0.999, eps 1e-8) from the accumulated gradient
+int adam_step(cd_net* n, float lr) {
+    cd_context* c = n->ctx;
+    n->step += 1;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - std::pow(b1, (float)n->step), c2 = 1.f - std::pow(b2, (float)n->step);
+    hipLaunchKernelGGL(k_adam, dim3(div_up((long long)n->nparam, 256)), dim3(256), 0, c->stream,
+                       n->w.as<float>(), n->m.as<float>(), n->v.as<float>(), n->gw.as<float>(),
+                       (long long)n->nparam, lr, b1, b2, c1, c2, 1e-8f);
+    CD_CHECK_LAUNCH(c);
+    n->weights_dirty = true;
     return CD_OK;
 }
 
@@ -687,6 +773,7 @@ void cd_net_arch_default(cd_net_arch* a) {
     a->coarsen = 4;
     a->smooth_sweeps = 4;
     a->smooth_omega = 1.2;
+    a->linear = 1;
 }
 
 int cd_net_create(cd_context* ctx, const cd_net_arch* arch, cd_net** out) {
@@ -754,14 +841,19 @@ int cd_net_init_params(cd_net* n, unsigned long long seed) {
     std::vector<float> w(n->nparam, 0.f);
     std::mt19937_64 rng(seed);
     for (const NetConv& cv : n->convs) {
-        const int fan_in = 9 * (cv.c1 + cv.c2);
-        std::normal_distribution<float> nd(0.f, std::sqrt(2.f / (float)fan_in));
-        const float scale = cv.out < 0 ? 0.1f : 1.f;  // small head
+        const bool first = cv.in1 < 0 && &cv == &n->convs[0];  // sees input channel 0 only
+        const int fan_in = 9 * (first ? 1 : cv.c1 + cv.c2);
+        // ReLU: He-normal with a small head.  Linear: variance 1/(3 fan_in)
+        // per layer, so the initial operator is close to zero and the first
+        // steps of the operator loss are well conditioned.
+        const float sd = n->arch.linear ? std::sqrt(1.f / (3.f * fan_in))
+                                        : std::sqrt(2.f / (float)fan_in);
+        std::normal_distribution<float> nd(0.f, sd);
+        const float scale = (!n->arch.linear && cv.out < 0) ? 0.1f : 1.f;
         for (int co = 0; co < cv.cout; ++co)
             for (int k = 0; k < cv.K; ++k) {
                 const int ci = k % (cv.c1 + cv.c2);
-                // the first layer only sees input channel 0
-                if (cv.in1 < 0 && &cv == &n->convs[0] && ci != 0) continue;
+                if (first && ci != 0) continue;
                 w[cv.woff + (size_t)co * cv.kpad + k] = scale * nd(rng);
             }
     }
@@ -800,19 +892,35 @@ int cd_net_loss_grad(cd_net* n, const float* d_x, const float* d_t, int batch, i
     return CD_OK;
 }
 
+int cd_net_loss_grad_operator(cd_net* n, const float* d_x, int batch, int h, int w,
+                              double* loss, float* host_grad, size_t count) {
+    if (!n || !d_x || batch < 1) return CD_ERR_BAD_ARG;
+    if (host_grad && count != n->nparam) return CD_ERR_BAD_ARG;
+    cd_context* c = n->ctx;
+    CD_HIP(c, hipSetDevice(c->device));
+    CD_TRY(prep(n, d_x, batch, h, w));
+    CD_TRY(forward(n, batch, h, w));
+    CD_TRY(op_loss_and_head_grad(n, d_x, batch, h, w, loss, true));
+    CD_TRY(zero(c, n->gw.as<float>(), (long long)n->nparam));
+    CD_TRY(backward(n, batch, h, w));
+    if (host_grad) {
+        CD_HIP(c, hipMemcpyAsync(host_grad, n->gw.ptr, sizeof(float) * n->nparam,
+                                 hipMemcpyDeviceToHost, c->stream));
+    }
+    CD_HIP(c, hipStreamSynchronize(c->stream));
+    return CD_OK;
+}
+
 int cd_net_train_step(cd_net* n, const float* d_x, const float* d_t, int batch, int h, int w,
                       float lr, double* loss) {
     CD_TRY(cd_net_loss_grad(n, d_x, d_t, batch, h, w, loss, nullptr, 0));
-    cd_context* c = n->ctx;
-    n->step += 1;
-    const float b1 = 0.9f, b2 = 0.999f;
-    const float c1 = 1.f - std::pow(b1, (float)n->step), c2 = 1.f - std::pow(b2, (float)n->step);
-    hipLaunchKernelGGL(k_adam, dim3(div_up((long long)n->nparam, 256)), dim3(256), 0, c->stream,
-                       n->w.as<float>(), n->m.as<float>(), n->v.as<float>(), n->gw.as<float>(),
-                       (long long)n->nparam, lr, b1, b2, c1, c2, 1e-8f);
-    CD_CHECK_LAUNCH(c);
-    n->weights_dirty = true;
-    return CD_OK;
+    return adam_step(n, lr);
+}
+
+int cd_net_train_step_operator(cd_net* n, const float* d_x, int batch, int h, int w, float lr,
+                               double* loss) {
+    CD_TRY(cd_net_loss_grad_operator(n, d_x, batch, h, w, loss, nullptr, 0));
+    return adam_step(n, lr);
 }
 
 int cd_context_set_net(cd_context* ctx, cd_net* net) {

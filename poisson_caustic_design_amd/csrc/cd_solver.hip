@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <type_traits>
 
@@ -618,31 +619,29 @@ int auto_check_every(int w, int h) {
 // Neumann counts).  Coarse cell (I,J) averages the f x f fine cells
 // [f*I, f*I+f) x [f*J, f*J+f).
 // ---------------------------------------------------------------------------
-// coarse residual rc = block mean of (b - bmean) - A phi; partials (sum, sum^2)
-__global__ __launch_bounds__(256) void k_restrict_residual(const double* __restrict__ phi,
-                                                           const double* __restrict__ b,
-                                                           const SolveDev* __restrict__ st,
-                                                           int w, int h, int f, int wc, int hc,
-                                                           float* __restrict__ rc,
-                                                           double* __restrict__ partials) {
+// bricked index of cell (x, y)
+__device__ __forceinline__ size_t brick_at(const Brick& bk, int x, int y) {
+    return brick_row(bk, (x + y) & 1, y) + brick_col(x >> 1);
+}
+
+// coarse residual rc = block mean of the projected BF16 residual the blocked
+// sweeps see (r16 - rmean32); partials (sum, sum^2)
+__global__ __launch_bounds__(256) void k_restrict_r16(const unsigned short* __restrict__ r16,
+                                                      Brick bk, const SolveDev* __restrict__ st,
+                                                      int f, int wc, int hc,
+                                                      float* __restrict__ rc,
+                                                      double* __restrict__ partials) {
     __shared__ double smem[16];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double v = 0.0;
     if (i < wc * hc) {
         const int J = i / wc, I = i - J * wc;
-        const double bm = st->bmean;
-        double acc = 0.0;
+        const float rm = st->rmean32;
+        float acc = 0.f;
         for (int yy = J * f; yy < J * f + f; ++yy)
-            for (int xx = I * f; xx < I * f + f; ++xx) {
-                const size_t c = (size_t)yy * w + xx;
-                double s = 0.0, cnt = 0.0;
-                if (xx > 0) { cnt += 1.0; s += phi[c - 1]; }
-                if (yy > 0) { cnt += 1.0; s += phi[c - w]; }
-                if (xx < w - 1) { cnt += 1.0; s += phi[c + 1]; }
-                if (yy < h - 1) { cnt += 1.0; s += phi[c + w]; }
-                acc += (b[c] - bm) - (s - cnt * phi[c]);
-            }
-        v = acc / (double)(f * f);
+            for (int xx = I * f; xx < I * f + f; ++xx)
+                acc += bf16_to_f32(r16[brick_at(bk, xx, yy)]) - rm;
+        v = (double)acc / (double)(f * f);
         rc[i] = (float)v;
     }
     const double a = cd_block_sum(v, smem);
@@ -664,12 +663,13 @@ __global__ void k_normalize_coarse(const float* __restrict__ rc, const double* _
     x[i] = (float)(((double)rc[i] - mean) / sig);
 }
 
-// phi += sign * rms * (nmax/pi)^2 * bilinear(y) at cell centres
-__global__ __launch_bounds__(256) void k_prolong_add(const float* __restrict__ y,
-                                                     const double* __restrict__ stats, int nc,
-                                                     int w, int h, int f, int wc, int hc,
-                                                     double nscale, double sign,
-                                                     double* __restrict__ phi) {
+// e = rms * (nmax/pi)^2 * bilinear(y) at cell centres, written in the
+// bricked layout of the blocked sweeps (the start of the smoothing)
+__global__ __launch_bounds__(256) void k_prolong_bricked(const float* __restrict__ y,
+                                                         const double* __restrict__ stats, int nc,
+                                                         int w, int h, int f, int wc, int hc,
+                                                         double nscale, Brick bk,
+                                                         float* __restrict__ e) {
     const int xx = blockIdx.x * blockDim.x + threadIdx.x;
     const int yy = blockIdx.y;
     if (xx >= w || yy >= h) return;
@@ -682,9 +682,121 @@ __global__ __launch_bounds__(256) void k_prolong_add(const float* __restrict__ y
     const int i1 = min(max(i0 + 1, 0), wc - 1), j1 = min(max(j0 + 1, 0), hc - 1);
     i0 = min(max(i0, 0), wc - 1);
     j0 = min(max(j0, 0), hc - 1);
-    const float e = (y[j0 * wc + i0] * (1.f - tu) + y[j0 * wc + i1] * tu) * (1.f - tv) +
-                    (y[j1 * wc + i0] * (1.f - tu) + y[j1 * wc + i1] * tu) * tv;
-    phi[(size_t)yy * w + xx] += sign * sig * nscale * (double)e;
+    const float ev = (y[j0 * wc + i0] * (1.f - tu) + y[j0 * wc + i1] * tu) * (1.f - tv) +
+                     (y[j1 * wc + i0] * (1.f - tu) + y[j1 * wc + i1] * tu) * tv;
+    e[brick_at(bk, xx, yy)] = (float)(sig * nscale * (double)ev);
+}
+
+// ---------------------------------------------------------------------------
+// Flexible conjugate gradients (FCG(1)) preconditioned by the network plus
+// smoothing sweeps.  With L the operator (negative semi-definite), r the
+// projected residual and z = M(r) the preconditioned residual:
+//   beta = -(z . Lp_old) / (p_old . Lp_old)   p = z + beta p_old
+//   alpha = (p . r) / (p . Lp)                phi += alpha p
+// alpha minimises the energy 1/2 phi'(-L)phi + b'phi along p, so every step
+// lowers the energy norm of the error.  Lp = Lz + beta Lp_old needs no second
+// stencil pass; p . Lp expands into the dot products taken below.
+// ---------------------------------------------------------------------------
+struct FcgDev {
+    double beta, alpha;
+    double pq;     // p . Lp of the current direction
+    int have;      // a previous direction exists
+    int bad;       // the last step had no descent direction
+};
+
+__device__ __forceinline__ float lap_bricked(const float* __restrict__ e, const Brick& bk, int x,
+                                             int y, int w, int h, float c) {
+    float s = 0.f, cnt = 0.f;
+    if (x > 0) { s += e[brick_at(bk, x - 1, y)]; cnt += 1.f; }
+    if (y > 0) { s += e[brick_at(bk, x, y - 1)]; cnt += 1.f; }
+    if (x < w - 1) { s += e[brick_at(bk, x + 1, y)]; cnt += 1.f; }
+    if (y < h - 1) { s += e[brick_at(bk, x, y + 1)]; cnt += 1.f; }
+    return s - cnt * c;
+}
+
+// partials[4 * block + k]: z.q_old, z.r, p_old.r, z.Lz
+__global__ __launch_bounds__(256) void k_fcg_dots(const float* __restrict__ z,
+                                                  const float* __restrict__ p,
+                                                  const float* __restrict__ q,
+                                                  const unsigned short* __restrict__ r16,
+                                                  const SolveDev* __restrict__ st, Brick bk,
+                                                  int w, int h, double* __restrict__ partials) {
+    __shared__ double smem[16];
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
+    if (x < w && y < h) {
+        const size_t i = brick_at(bk, x, y);
+        const float zc = z[i];
+        const float lz = lap_bricked(z, bk, x, y, w, h, zc);
+        const float rr = bf16_to_f32(r16[i]) - st->rmean32;
+        d0 = (double)zc * q[i];
+        d1 = (double)zc * rr;
+        d2 = (double)p[i] * rr;
+        d3 = (double)zc * lz;
+    }
+    d0 = cd_block_sum(d0, smem);
+    d1 = cd_block_sum(d1, smem);
+    d2 = cd_block_sum(d2, smem);
+    d3 = cd_block_sum(d3, smem);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        const size_t bid = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+        partials[4 * bid] = d0;
+        partials[4 * bid + 1] = d1;
+        partials[4 * bid + 2] = d2;
+        partials[4 * bid + 3] = d3;
+    }
+}
+
+// single block: step coefficients from the dot products (fixed summation order)
+__global__ void k_fcg_coef(const double* __restrict__ partials, int nblocks, FcgDev* fd) {
+    __shared__ double smem[16];
+    double d[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+        for (int k = 0; k < 4; ++k) d[k] += partials[4 * b + k];
+    for (int k = 0; k < 4; ++k) d[k] = cd_block_sum(d[k], smem);
+    if (threadIdx.x == 0) {
+        const double zq = d[0], zr = d[1], pr = d[2], zlz = d[3];
+        double beta = 0.0;
+        if (fd->have && fd->pq < 0.0) beta = -zq / fd->pq;
+        double pq = zlz + 2.0 * beta * zq + beta * beta * (fd->have ? fd->pq : 0.0);
+        double prn = zr + beta * (fd->have ? pr : 0.0);
+        double alpha = 0.0;
+        fd->bad = 0;
+        if (pq < 0.0 && isfinite(pq) && isfinite(prn)) {
+            alpha = prn / pq;
+        } else {
+            // no usable direction: restart from z alone next time
+            fd->bad = 1;
+            beta = 0.0;
+            pq = zlz;
+        }
+        fd->beta = beta;
+        fd->alpha = alpha;
+        fd->pq = pq;
+        fd->have = fd->bad ? 0 : 1;
+    }
+}
+
+// p = z + beta p; q = Lz + beta q; eo = alpha p (the correction the next check
+// folds into phi64); st->cur names eo's buffer
+__global__ __launch_bounds__(256) void k_fcg_update(const float* __restrict__ z,
+                                                    float* __restrict__ p, float* __restrict__ q,
+                                                    float* __restrict__ eo,
+                                                    const FcgDev* __restrict__ fd, Brick bk,
+                                                    int w, int h, SolveDev* st, int eo_index) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x == 0 && y == 0) st->cur = eo_index;
+    if (x >= w || y >= h) return;
+    const size_t i = brick_at(bk, x, y);
+    const float beta = (float)fd->beta, alpha = (float)fd->alpha;
+    const float zc = z[i];
+    const float lz = lap_bricked(z, bk, x, y, w, h, zc);
+    const float pn = zc + beta * p[i];
+    p[i] = pn;
+    q[i] = lz + beta * q[i];
+    eo[i] = alpha * pn;
 }
 
 // coarse FP64 block mean of phi
@@ -729,7 +841,8 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     const dim3 rblk(RED_BX, RED_BY);
     const dim3 rgrid(div_up(w, RED_BX), div_up(h, RED_BY));
     const int nred = rgrid.x * rgrid.y;
-    CD_TRY(ctx->ensure(ctx->partials, sizeof(double) * 2 * (size_t)nred));
+    // (sum, sum^2) per block for checks, four dot products per block for FCG
+    CD_TRY(ctx->ensure(ctx->partials, sizeof(double) * 4 * (size_t)nred));
     CD_TRY(ctx->ensure(ctx->solve_dev, sizeof(SolveDev)));
     // FP32 sweeps without holes take the temporally blocked kernel on the
     // packed layout; FP64 sweeps and hole masks take the per-half-sweep kernel.
@@ -876,7 +989,9 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         CD_TRY(ctx->ensure(ctx->lc_r, sizeof(float) * (size_t)lwc * lhc));
         CD_TRY(ctx->ensure(ctx->lc_x, sizeof(float) * (size_t)lwc * lhc));
         CD_TRY(ctx->ensure(ctx->lc_y, sizeof(float) * (size_t)lwc * lhc));
-        CD_TRY(ctx->ensure(ctx->lc_backup, sizeof(double) * n));  // phi before a correction
+        CD_TRY(ctx->ensure(ctx->fcg_p, sizeof(float) * npk));
+        CD_TRY(ctx->ensure(ctx->fcg_q, sizeof(float) * npk));
+        CD_TRY(ctx->ensure(ctx->fcg_dev, sizeof(FcgDev)));
     }
     if (learned || capture)
         CD_TRY(ctx->ensure(ctx->lc_stats, sizeof(double) * 2 * (kMaxPairsPerSolve + 2)));
@@ -884,14 +999,15 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         CD_TRY(ctx->ensure(ctx->cap_phi, sizeof(double) * (size_t)cwc * chc * (kMaxPairsPerSolve + 1)));
         CD_TRY(ctx->ensure(ctx->lc_r, sizeof(float) * (size_t)std::max(cwc * chc, lwc * lhc)));
     }
-    // restrict the current residual to (wc x hc), normalise into xout; stats slot
+    // restrict the residual of the last check to (wc x hc), normalise into
+    // xout; stats slot receives (sum, sum^2) of the coarse residual
     auto restrict_normalize = [&](int f, int wc, int hc, float* xout, double* stats) -> int {
         const int nc = wc * hc;
         const int nb = div_up(nc, 256);  // <= nred: the partials buffer is large enough
-        hipLaunchKernelGGL(k_restrict_residual, dim3(nb), dim3(256), 0, s, phi, b, sd, w, h, f,
-                           wc, hc, ctx->lc_r.as<float>(), ctx->partials.as<double>());
+        hipLaunchKernelGGL(k_restrict_r16, dim3(nb), dim3(256), 0, s, r16, bk, sd, f, wc, hc,
+                           ctx->lc_r.as<float>(), partials);
         CD_CHECK_LAUNCH(ctx);
-        CD_TRY(cd_reduce_partials(ctx, ctx->partials.as<double>(), nb, 2, stats));
+        CD_TRY(cd_reduce_partials(ctx, partials, nb, 2, stats));
         hipLaunchKernelGGL(k_normalize_coarse, dim3(nb), dim3(256), 0, s, ctx->lc_r.as<float>(),
                            stats, nc, xout);
         CD_CHECK_LAUNCH(ctx);
@@ -911,22 +1027,14 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         cap_slots.push_back(slot);
         return CD_OK;
     };
-    // the learned correction: phi += sign * P net(R r)
-    auto learned_correction = [&](double sign) -> int {
-        double* stats = ctx->lc_stats.as<double>();
-        if (sign > 0) {
-            CD_TRY(restrict_normalize(lf, lwc, lhc, ctx->lc_x.as<float>(), stats));
-            CD_TRY(cd_net_forward_internal(ctx->net, ctx->lc_x.as<float>(), ctx->lc_y.as<float>(),
-                                           lhc, lwc));
-        }
-        hipLaunchKernelGGL(k_prolong_add, dim3(div_up(w, 256), h), dim3(256), 0, s,
-                           ctx->lc_y.as<float>(), stats, lwc * lhc, w, h, lf, lwc, lhc, nscale,
-                           sign, phi);
-        CD_CHECK_LAUNCH(ctx);
-        return CD_OK;
-    };
     const float oms = arch ? (float)arch->smooth_omega : om;
     const int nsmooth = arch ? std::max(1, arch->smooth_sweeps) : 4;
+    // the smoothing launches start from e32; after `nl` launches the result
+    // sits in e32 (nl even) or e32b (nl odd)
+    const int smooth_launches = div_up(nsmooth, BH / 2);
+    float* zbuf = (smooth_launches & 1) ? e32b : e32;
+    float* obuf = (smooth_launches & 1) ? e32 : e32b;
+    const int obuf_index = (smooth_launches & 1) ? 0 : 1;
     int learned_applied = 0, learned_rejected = 0;
 
     std::vector<int> cum_at_check;
@@ -939,38 +1047,65 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     int k = kmax;
     if (adaptive) k = std::min(kmax, std::max(8, kmax / 4));
     bool done = ctx->host_status->done != 0;
+    if (learned && !done) {
+        CD_TRY(cd_fill_f32(ctx, ctx->fcg_p.as<float>(), npk, 0.f));
+        CD_TRY(cd_fill_f32(ctx, ctx->fcg_q.as<float>(), npk, 0.f));
+        CD_TRY(cd_fill_f64(ctx, ctx->fcg_dev.as<double>(), sizeof(FcgDev) / sizeof(double), 0.0));
+    }
+    // learned FCG iterations; abandoned for plain SOR when the residual stops
+    // falling (the iterate is kept: every step lowered the error's energy norm)
+    double best_rel = ctx->host_status->rel;
+    int stall = 0;
     while (!done && learned && cum < o->max_sweeps) {
-        // one learned cycle: correction, residual, smoothing sweeps, residual;
-        // kept only if the FP64 residual after the cycle is lower than before
-        const double rel_before = ctx->host_status->rel;
-        CD_HIP(ctx, hipMemcpyAsync(ctx->lc_backup.ptr, phi, sizeof(double) * n,
-                                   hipMemcpyDeviceToDevice, s));
+        FcgDev* fd = ctx->fcg_dev.as<FcgDev>();
         CD_TRY(ctx->timer_begin());
-        CD_TRY(learned_correction(+1.0));
-        CD_TRY(ctx->timer_end(CD_KERNEL_LEARNED, 1, 0.0));
-        CD_TRY(enqueue_check(++check_id));
-        cum_at_check.push_back(cum);
-        if (ctx->host_status->done) { ++learned_applied; done = true; break; }
+        double* stats = ctx->lc_stats.as<double>();
+        CD_TRY(restrict_normalize(lf, lwc, lhc, ctx->lc_x.as<float>(), stats));
+        CD_TRY(cd_net_forward_internal(ctx->net, ctx->lc_x.as<float>(), ctx->lc_y.as<float>(), lhc,
+                                       lwc));
+        hipLaunchKernelGGL(k_prolong_bricked, dim3(div_up(w, 256), h), dim3(256), 0, s,
+                           ctx->lc_y.as<float>(), stats, lwc * lhc, w, h, lf, lwc, lhc, nscale, bk,
+                           e32);
+        CD_CHECK_LAUNCH(ctx);
+        CD_TRY(ctx->timer_end(CD_KERNEL_LEARNED, 4, 0.0));
         const int nsw = std::min(nsmooth, o->max_sweeps - cum);
         CD_TRY(enqueue_blocked_om(cum, nsw, oms / 2.f, oms / 3.f, oms / 4.f));
         cum += nsw;
+        if (div_up(nsw, BH / 2) != smooth_launches) { learned = false; break; }  // sweep cap hit
+        CD_TRY(ctx->timer_begin());
+        hipLaunchKernelGGL(k_fcg_dots, rgrid, rblk, 0, s, zbuf, ctx->fcg_p.as<float>(),
+                           ctx->fcg_q.as<float>(), r16, sd, bk, w, h, partials);
+        hipLaunchKernelGGL(k_fcg_coef, dim3(1), dim3(1024), 0, s, partials, nred, fd);
+        hipLaunchKernelGGL(k_fcg_update, rgrid, rblk, 0, s, zbuf, ctx->fcg_p.as<float>(),
+                           ctx->fcg_q.as<float>(), obuf, fd, bk, w, h, sd, obuf_index);
+        CD_CHECK_LAUNCH(ctx);
+        CD_TRY(ctx->timer_end(CD_KERNEL_LEARNED, 3, 26.0 * (double)n));
         CD_TRY(enqueue_check(++check_id));
         cum_at_check.push_back(cum);
+        ++learned_applied;
         const SolveDev& hs = *ctx->host_status;
-        if (hs.done) { ++learned_applied; done = true; break; }
-        if (std::isfinite(hs.rel) && hs.rel < rel_before) {
-            ++learned_applied;
-            continue;
+        done = hs.done != 0;
+        if (!std::isfinite(hs.rel)) break;
+        if (capture && !done && check_id % std::max(ctx->cap_every, 1) == 0)
+            CD_TRY(capture_pair());
+#ifdef CD_TRACE_LEARNED
+        {
+            FcgDev hf;
+            CD_HIP(ctx, hipMemcpy(&hf, fd, sizeof(hf), hipMemcpyDeviceToHost));
+            fprintf(stderr, "fcg %d: rel %.3e alpha %.3e beta %.3e pq %.3e bad %d\n",
+                    learned_applied, hs.rel, hf.alpha, hf.beta, hf.pq, hf.bad);
         }
-        // rejected: restore the iterate before the correction; plain SOR finishes
-        ++learned_rejected;
-        learned = false;
-        CD_HIP(ctx, hipMemcpyAsync(phi, ctx->lc_backup.ptr, sizeof(double) * n,
-                                   hipMemcpyDeviceToDevice, s));
-        CD_TRY(enqueue_check(++check_id));
-        cum_at_check.push_back(cum);
-        done = ctx->host_status->done != 0;
-        prev_rel = ctx->host_status->rel;
+#endif
+        // the residual of CG-type iterations may rise for a few steps before
+        // it falls; give up only when 8 iterations bring no halving
+        if (hs.rel < 0.5 * best_rel) {
+            best_rel = hs.rel;
+            stall = 0;
+        } else if (++stall >= 8) {
+            ++learned_rejected;
+            learned = false;
+        }
+        prev_rel = hs.rel;
     }
     while (!done && cum < o->max_sweeps) {
         const int nsw = std::min(k, o->max_sweeps - cum);
@@ -1052,7 +1187,7 @@ int cd_poisson_solve_device(cd_context* ctx, const double* d_rhs, double* d_phi,
     const dim3 rblk(RED_BX, RED_BY);
     const dim3 rgrid(div_up(width, RED_BX), div_up(height, RED_BY));
     const int nred = rgrid.x * rgrid.y;
-    CD_TRY(ctx->ensure(ctx->partials, sizeof(double) * 2 * (size_t)nred));
+    CD_TRY(ctx->ensure(ctx->partials, sizeof(double) * 4 * (size_t)nred));  // as the solve
     CD_TRY(ctx->ensure(ctx->fill, sizeof(double) * 4));
     hipLaunchKernelGGL(k_build_mask, rgrid, rblk, 0, ctx->stream, d_rhs, ctx->mask.as<uint8_t>(),
                        width, height, ctx->partials.as<double>());

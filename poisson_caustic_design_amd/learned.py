@@ -1,19 +1,24 @@
-"""Training and use of the learned initial correction (cd_net).
+"""Training and use of the learned coarse solve (cd_net).
 
-Pairs come from the plain solver itself: designs over seeded synthetic
-densities run with the FP32/FP64 red-black SOR solver while the context
-captures, at every convergence check of every Poisson solve (transport
-potential and height), the normalised coarse residual and, once the solve has
-finished, the normalised coarse difference between the converged solution and
-the iterate at that check.  The first check of a cold solve gives exactly the
-(density mismatch, converged solution) pair.
+The network approximates the inverse of the coarse Neumann operator; solves
+with learned_init use it inside a two-grid preconditioner of flexible
+conjugate gradients.  It is trained on the operator loss
+mean((cs * Lc net(x) - x)^2), which needs inputs only:
+
+* coarse residuals captured from the solver itself: designs over seeded
+  synthetic densities run while the context records, at every convergence
+  check of every Poisson solve, the normalised coarse residual (with plain SOR
+  first, then again with the network in use, so that the inputs it meets in
+  its own iterations are covered);
+* random fields of several smoothness scales and white noise, so that the
+  operator is learned on every frequency.
 
 Training runs on the GPU through the library's own BF16-MFMA forward/backward
-kernels and Adam (cd_net_train_step).  Device buffers for the pairs are torch
-tensors (memory plumbing only).
+kernels and Adam (cd_net_train_step_operator).  Device buffers for the inputs
+are torch tensors (memory plumbing only).
 
-    python -m poisson_caustic_design_amd.learned --mesh-res 512 --designs 200 \
-        --out poisson_caustic_design_amd/weights/learned_2048.npz
+    python -m poisson_caustic_design_amd.learned --mesh-res 512 --designs 16 \
+        --out poisson_caustic_design_amd/weights/learned_grid2048.npz
 """
 from __future__ import annotations
 
@@ -44,7 +49,7 @@ def design_for(ctx: capi.Context, img: np.ndarray, prm: dict, opts: capi.SolveOp
 
 def generate_pairs(ctx: capi.Context, seeds, mesh_res: int, config: str = "mixture",
                    capacity: int = 4096, coarsen: int = 4, every: int = 1, tol: float = 1e-6,
-                   max_ot: int = 50, log=print):
+                   max_ot: int = 50, log=print, learned_init: int = 0):
     """Capture (x, t) pairs from plain-solver designs; returns torch tensors."""
     import torch
 
@@ -54,7 +59,7 @@ def generate_pairs(ctx: capi.Context, seeds, mesh_res: int, config: str = "mixtu
     t = torch.zeros_like(x)
     torch.cuda.synchronize()
     ctx.set_capture(x.data_ptr(), t.data_ptr(), capacity, hc, wc, coarsen, every)
-    opts = capi.default_opts(tol=tol)
+    opts = capi.default_opts(tol=tol, learned_init=learned_init)
     t0 = time.time()
     try:
         for i, seed in enumerate(seeds):
@@ -71,6 +76,67 @@ def generate_pairs(ctx: capi.Context, seeds, mesh_res: int, config: str = "mixtu
         count = ctx.capture_count()
         ctx.clear_capture()
     return x[:count], t[:count]
+
+
+def random_fields(count: int, h: int, w: int, seed: int = 0) -> np.ndarray:
+    """Mean-free unit-RMS random fields: Gaussian-filtered noise over scales
+    0.5 .. 32 coarse cells, and white noise."""
+    from scipy.ndimage import gaussian_filter
+
+    rng = np.random.default_rng(seed)
+    scales = [0.0, 0.5, 1.0, 2.0, 4.0, 8.0, 16.0, 32.0]
+    out = np.empty((count, h, w), dtype=np.float32)
+    for i in range(count):
+        g = rng.standard_normal((h, w))
+        sc = scales[i % len(scales)]
+        if sc > 0:
+            g = gaussian_filter(g, sc, mode="reflect")
+        g -= g.mean()
+        out[i] = g / max(np.sqrt(np.mean(g * g)), 1e-30)
+    return out
+
+
+def capture_inputs(ctx: capi.Context, seeds, mesh_res: int, config: str = "mixture",
+                   capacity: int = 4096, coarsen: int = 4, net: capi.Net = None,
+                   tol: float = 1e-6, max_ot: int = 50, log=print):
+    """Coarse residuals met by solves of seeded designs (with `net` in use when
+    given); returns a torch tensor [count][h][w] on the device."""
+    if net is not None:
+        ctx.set_net(net)
+    try:
+        x, _ = generate_pairs(ctx, seeds, mesh_res, config, capacity, coarsen, 1, tol, max_ot,
+                              log=log, learned_init=1 if net is not None else 0)
+    finally:
+        if net is not None:
+            ctx.set_net(None)
+    return x
+
+
+def train_operator(net: capi.Net, x, steps: int = 3000, batch: int = 16, lr: float = 2e-3,
+                   seed: int = 0, log=print, log_every: int = 500):
+    """Adam on the operator loss over random mini-batches of the inputs x
+    (torch tensor [n][h][w] on the device); returns the per-step losses."""
+    import torch
+
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    n, hc, wc = x.shape
+    losses = []
+    t0 = time.time()
+    for step in range(steps):
+        idx = torch.randint(0, n, (batch,), generator=g).to(x.device)
+        xb = x.index_select(0, idx)
+        # flips keep the operator's symmetries in the data
+        if step % 2:
+            xb = xb.flip(2)
+        if step % 4 >= 2:
+            xb = xb.flip(1)
+        xb = xb.contiguous()
+        rate = lr * 0.5 * (1 + np.cos(np.pi * step / steps))  # cosine schedule
+        losses.append(net.train_step_operator(xb.data_ptr(), batch, hc, wc, rate))
+        if log and (step % log_every == 0 or step == steps - 1):
+            log(f"train: step {step} loss {np.mean(losses[-log_every:]):.4e} "
+                f"lr {rate:.2e} {time.time() - t0:.1f}s")
+    return np.array(losses)
 
 
 def train(net: capi.Net, x, t, steps: int = 2000, batch: int = 8, lr: float = 1e-3,
@@ -99,19 +165,47 @@ def train(net: capi.Net, x, t, steps: int = 2000, batch: int = 8, lr: float = 1e
     return np.array(losses)
 
 
+def fit(ctx: capi.Context, arch: capi.NetArch, mesh_res: int, seeds, config: str = "mixture",
+        random_count: int = 1000, capacity: int = 4096, steps: int = 3000, batch: int = 16,
+        lr: float = 2e-3, rounds: int = 1, seed: int = 0, log=print):
+    """Train a network for solves of (4*mesh_res)^2 grids: solver residuals of
+    plain-SOR designs plus random fields, then `rounds` more passes that add
+    the residuals met with the network in use.  Returns (net, losses, inputs)."""
+    import torch
+
+    n = 4 * mesh_res
+    hc = wc = n // arch.coarsen
+    xs = [capture_inputs(ctx, seeds, mesh_res, config, capacity, arch.coarsen, log=log)]
+    xs.append(torch.from_numpy(random_fields(random_count, hc, wc, seed)).to(xs[0].device))
+    pool = torch.cat(xs)
+    if log:
+        log(f"inputs: {pool.shape[0]} ({xs[0].shape[0]} from solves)")
+    net = capi.Net(ctx, arch)
+    net.init_params(seed)
+    losses = [train_operator(net, pool, steps, batch, lr, seed, log=log)]
+    for r in range(rounds):
+        more = capture_inputs(ctx, seeds, mesh_res, config, capacity, arch.coarsen, net=net,
+                              log=log)
+        pool = torch.cat([pool, more])
+        if log:
+            log(f"round {r + 1}: +{more.shape[0]} inputs met with the network in use")
+        losses.append(train_operator(net, pool, steps // 2, batch, lr / 2, seed + 1 + r, log=log))
+    return net, np.concatenate(losses), pool
+
+
 def save_params(net: capi.Net, path: str, meta: dict):
     a = net.arch
     np.savez_compressed(path, params=net.get_params(), levels=a.levels,
                         channels=np.array(list(a.channels)), coarsen=a.coarsen,
                         smooth_sweeps=a.smooth_sweeps, smooth_omega=a.smooth_omega,
-                        meta=json.dumps(meta))
+                        linear=a.linear, meta=json.dumps(meta))
 
 
 def load_net(ctx: capi.Context, path: str) -> capi.Net:
     d = np.load(path, allow_pickle=False)
     arch = capi.default_arch(levels=int(d["levels"]), channels=[int(c) for c in d["channels"]],
                              coarsen=int(d["coarsen"]), smooth_sweeps=int(d["smooth_sweeps"]),
-                             smooth_omega=float(d["smooth_omega"]))
+                             smooth_omega=float(d["smooth_omega"]), linear=int(d["linear"]))
     net = capi.Net(ctx, arch)
     net.set_params(d["params"])
     return net
@@ -119,14 +213,19 @@ def load_net(ctx: capi.Context, path: str) -> capi.Net:
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--mesh-res", type=int, default=256)
-    ap.add_argument("--designs", type=int, default=200)
+    ap.add_argument("--mesh-res", type=int, default=512)
+    ap.add_argument("--designs", type=int, default=16)
     ap.add_argument("--seed0", type=int, default=1000)
     ap.add_argument("--config", default="mixture")
-    ap.add_argument("--capacity", type=int, default=6000)
+    ap.add_argument("--levels", type=int, default=5)
+    ap.add_argument("--channels", default="16,16,32,32,32")
+    ap.add_argument("--relu", action="store_true")
+    ap.add_argument("--random", type=int, default=1000)
+    ap.add_argument("--capacity", type=int, default=4000)
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--rounds", type=int, default=1)
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
 
@@ -134,20 +233,20 @@ def main(argv=None):
         print(*m, file=sys.stderr, flush=True)
 
     ctx = capi.Context(0)
-    x, t = generate_pairs(ctx, range(a.seed0, a.seed0 + a.designs), a.mesh_res, a.config,
-                          a.capacity, log=log)
-    log(f"pairs: {x.shape[0]}")
-    net = capi.Net(ctx)
-    net.init_params(a.seed0)
-    losses = train(net, x, t, a.steps, a.batch, a.lr, log=log)
+    arch = capi.default_arch(levels=a.levels, channels=[int(c) for c in a.channels.split(",")],
+                             linear=0 if a.relu else 1)
+    t0 = time.time()
+    net, losses, pool = fit(ctx, arch, a.mesh_res, range(a.seed0, a.seed0 + a.designs), a.config,
+                            a.random, a.capacity, a.steps, a.batch, a.lr, a.rounds, a.seed0,
+                            log=log)
     out = a.out or os.path.join(WEIGHTS_DIR, f"learned_grid{4 * a.mesh_res}.npz")
     os.makedirs(os.path.dirname(out), exist_ok=True)
-    save_params(net, out, {"mesh_res": a.mesh_res, "designs": a.designs, "seed0": a.seed0,
-                           "config": a.config, "pairs": int(x.shape[0]), "steps": a.steps,
-                           "final_loss": float(np.mean(losses[-100:]))})
-    print(json.dumps({"out": out, "pairs": int(x.shape[0]),
-                      "loss_first": float(np.mean(losses[:50])),
-                      "loss_last": float(np.mean(losses[-100:]))}), flush=True)
+    meta = {"mesh_res": a.mesh_res, "designs": a.designs, "seed0": a.seed0, "config": a.config,
+            "inputs": int(pool.shape[0]), "steps": a.steps, "rounds": a.rounds,
+            "loss": "operator", "final_loss": float(np.mean(losses[-100:])),
+            "train_seconds": round(time.time() - t0, 1)}
+    save_params(net, out, meta)
+    print(json.dumps({"out": out, **meta, "loss_first": float(np.mean(losses[:50]))}), flush=True)
 
 
 if __name__ == "__main__":

@@ -42,3 +42,47 @@ def test_bf16_rounding():
     r = bf16(v)
     assert r[0] == 1.0 and r[1] == 1.0  # exact tie -> even mantissa
     assert r[2] == 1.0 and r[3] == 1.0 + 2 ** -7 and r[4] == -2.5
+
+
+def test_netref_operator_gradient_finite_differences():
+    for linear in (True, False):
+        net = NetRef(levels=3, channels=(8, 16, 16), linear=linear)
+        p = net.init_params(4).astype(np.float64)
+        rng = np.random.default_rng(2)
+        x = rng.standard_normal((2, 8, 8))
+        loss, g = net.loss_grad_operator(p, x)
+        idx = rng.choice(np.flatnonzero(p), 20, replace=False)
+        eps = 1e-6
+        for i in idx:
+            pp, pm = p.copy(), p.copy()
+            pp[i] += eps
+            pm[i] -= eps
+            fd = (net.loss_grad_operator(pp, x)[0] - net.loss_grad_operator(pm, x)[0]) / (2 * eps)
+            assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (linear, i, fd, g[i])
+
+
+def test_netref_operator_matches_independent_stencil_and_exact_inverse():
+    from tests.helpers import apply_operator, dct_solve, eigvec
+    rng = np.random.default_rng(3)
+    v = rng.standard_normal((12, 20))
+    assert np.allclose(NetRef.lap(v[None])[0], apply_operator(v), atol=1e-12)
+    e, lam = eigvec(20, 12, 3, 2)
+    assert np.allclose(NetRef.lap(e[None])[0], lam * e, atol=1e-12)
+    # the loss vanishes exactly at the inverse operator: y = Lc^{-1} x / cs
+    x = rng.standard_normal((12, 20))
+    x -= x.mean()
+    cs = (20 / np.pi) ** 2
+    y = dct_solve(x) / cs
+    u = cs * NetRef.lap(y[None])[0] - x
+    assert np.abs(u).max() < 1e-9
+
+
+def test_linear_netref_is_linear():
+    net = NetRef(levels=3, channels=(8, 16, 16), linear=True)
+    p = net.init_params(5).astype(np.float64)
+    rng = np.random.default_rng(4)
+    x1, x2 = rng.standard_normal((2, 1, 16, 16))
+    y1, _ = net.forward(p, x1)
+    y2, _ = net.forward(p, x2)
+    y3, _ = net.forward(p, 2.0 * x1 - 0.5 * x2)
+    assert np.allclose(y3, 2.0 * y1 - 0.5 * y2, atol=1e-12)
