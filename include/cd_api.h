@@ -148,6 +148,13 @@ int cd_design_transport_step(cd_design* d, double* step_out);
 /* perform_height_map_iteration(itr) (caustic_design.cpp:269-332); *update_out
  * is the sum of squared vertex-height updates (mesh.cpp:724-742) */
 int cd_design_height_step(cd_design* d, double* update_out);
+/* Mesh::laplacian_smoothing(target_points, factor) (mesh.cpp:534-606). */
+int cd_design_laplacian_smoothing(cd_design* d, double factor);
+/* on != 0: a raster miss in cd_design_transport_step smooths the target mesh
+ * with factor 0.1f and returns CD_OK with a NaN step, the recovery of
+ * caustic_design.cpp:215-218 (in the reference an exit(0) at mesh.cpp:279
+ * precedes it).  Default off: the step fails with CD_ERR_RASTER_MISS. */
+int cd_design_set_miss_recovery(cd_design* d, int on);
 
 typedef struct {
     int status;
@@ -159,6 +166,7 @@ typedef struct {
     int learned_applied, learned_rejected;
     double max_phi_residual, max_h_residual;
     double elapsed_ms;
+    int recoveries;               /* raster misses recovered by smoothing            */
 } cd_run_stats;
 
 /* The reference driver loop (main.cpp:243-262): up to max_transport_steps

@@ -88,6 +88,7 @@ class Oracle:
         L.orc_raster.restype = _I
         L.orc_vertex_step.argtypes = [_P, _I, _I, _D, _D, _I, _I, _P]
         L.orc_vertex_step.restype = _D
+        L.orc_laplacian_smoothing.argtypes = [_P, _I, _I, _D]
         L.orc_inverse_map.argtypes = [_I, _I, _D, _D, _P, _P]
         L.orc_inverse_map.restype = _I
         L.orc_normals.argtypes = [_I, _I, _D, _D, _P, _P, _D, _D, _P, _P]
@@ -199,6 +200,11 @@ class Oracle:
         s = self.L.orc_vertex_step(_ptr(phi), phi.shape[1], phi.shape[0], W, H, mx, my, _ptr(xy))
         return xy, s
 
+    def laplacian_smoothing(self, xy, mx, my, factor):
+        xy = _f64(xy).copy()
+        self.L.orc_laplacian_smoothing(_ptr(xy), mx, my, factor)
+        return xy
+
     def inverse_map(self, mx, my, W, H, target_xy):
         target_xy = _f64(target_xy)
         inv = np.empty((mx * my, 2))
@@ -303,6 +309,7 @@ class RefLib:
         L.ref_set_grid.argtypes = [_P, _I, _P]
         L.ref_get_vector.argtypes = [_P, _I, _P]
         L.ref_inverted_map.argtypes = [_P, _P]
+        L.ref_laplacian_smoothing.argtypes = [_P, _D]
         L.ref_save_solid_obj.argtypes = [C.c_char_p, _P, _P, _I, _I, _D, _D, _D]
         L.ref_export_grid_svg.argtypes = [C.c_char_p, _P, _I, _I, _I, _D, _D, _D]
         L.ref_export_triangles_svg.argtypes = [C.c_char_p, _P, _I, _I, _I, _D, _D, _D]
@@ -406,6 +413,9 @@ class RefDesign:
     def set_source_z(self, z):
         z = _f64(z)
         self.L.ref_set_source_z(self.h, _ptr(z))
+
+    def laplacian_smoothing(self, factor):
+        self.L.ref_laplacian_smoothing(self.h, factor)
 
     def grid(self, which):
         a = np.empty((self.ny, self.nx))

@@ -539,6 +539,44 @@ double orc_vertex_step(const double* phi, int nx, int ny, double W, double H, in
     return maxd / W;
 }
 
+void orc_laplacian_smoothing(double* xy, int mx, int my, double f) {
+    const int V = mx * my;
+    double* avg = (double*)malloc(sizeof(double) * 2 * (size_t)V);
+#define PX(xx, yy) xy[2 * ((yy) * mx + (xx))]
+#define PY(xx, yy) xy[2 * ((yy) * mx + (xx)) + 1]
+    for (int i = 0; i < V; ++i) {
+        const int y = i / mx, x = i % mx;
+        double ax = xy[2 * i], ay = xy[2 * i + 1];
+        const int left = x == 0, right = x == mx - 1, top = y == 0, bottom = y == my - 1;
+        if ((left || right) && (top || bottom)) {
+            /* corner: unchanged */
+        } else if (left) {
+            ay = ((0.0 + PY(x + 1, y)) + PY(x, y + 1)) + PY(x, y - 1);
+            ay /= 3.0;
+        } else if (right) {
+            ay = ((0.0 + PY(x - 1, y)) + PY(x, y - 1)) + PY(x, y + 1);
+            ay /= 3.0;
+        } else if (top) {
+            ax = ((0.0 + PX(x - 1, y)) + PX(x + 1, y)) + PX(x, y + 1);
+            ax /= 3.0;
+        } else if (bottom) {
+            ax = ((0.0 + PX(x - 1, y)) + PX(x, y - 1)) + PX(x + 1, y);
+            ax /= 3.0;
+        } else {
+            ax = (((0.0 + PX(x - 1, y)) + PX(x + 1, y)) + PX(x, y - 1)) + PX(x, y + 1);
+            ay = (((0.0 + PY(x - 1, y)) + PY(x + 1, y)) + PY(x, y - 1)) + PY(x, y + 1);
+            ax /= 4.0;
+            ay /= 4.0;
+        }
+        avg[2 * i] = ax;
+        avg[2 * i + 1] = ay;
+    }
+#undef PX
+#undef PY
+    for (int i = 0; i < 2 * V; ++i) xy[i] = avg[i] * f + xy[i] * (1.0 - f);
+    free(avg);
+}
+
 int orc_inverse_map(int mx, int my, double W, double H, const double* target_xy, double* inv_xy) {
     const double eps = 1e-8;                                       /* mesh.cpp:351 */
     tri_bins b;

@@ -18,6 +18,7 @@
 //   bilinearInterpolation       src/caustic_design.cpp:156-188
 //   integrate_cell_intensities  src/polygon_utils.cpp:311-366
 //   Mesh::calculate_inverted_transport_map  src/mesh.cpp:348-409
+//   Mesh::laplacian_smoothing   src/mesh.cpp:534-606
 //   save_solid_obj              src/utils.cpp:198-260
 //   export_grid_to_svg          src/utils.cpp:262-307
 //   export_triangles_to_svg     src/utils.cpp:309-344
@@ -260,6 +261,12 @@ static std::vector<std::vector<double>> to_points(const double* a, int n, int st
     for (int i = 0; i < n; ++i)
         for (int k = 0; k < stride && k < 3; ++k) p[i][k] = a[(size_t)i * stride + k];
     return p;
+}
+
+// Mesh::laplacian_smoothing (mesh.cpp:534-606) on the design's target mesh
+void ref_laplacian_smoothing(void* h, double factor) {
+    auto* cd = static_cast<Caustic_design*>(h);
+    cd->mesh->laplacian_smoothing(cd->mesh->target_points, factor);
 }
 
 // The writers run through the public Mesh methods, which supply the

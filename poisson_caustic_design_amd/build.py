@@ -74,6 +74,13 @@ def build(verbose: bool = False, defines=(), out: str = LIB) -> str:
     _run([hipcc, "-shared", "-fPIC", "--offload-arch=" + ARCH, *objs, "-o", tmp,
           "-Wl,-soname,libcaustic_amd.so", "-lz"])
     os.replace(tmp, out)
+    if out == LIB:
+        # the command-line drop-in, linked against the library next to it
+        cli = os.path.join(os.path.dirname(LIB), "caustic_design_amd")
+        _run(["g++", "-O2", "-std=c++17", "-I" + INCLUDE, os.path.join(CSRC, "cli_main.cpp"),
+              "-o", cli + ".tmp", "-L" + os.path.dirname(LIB), "-l:libcaustic_amd.so",
+              "-Wl,-rpath,$ORIGIN"])
+        os.replace(cli + ".tmp", cli)
     return out
 
 

@@ -15,8 +15,10 @@ and every call fails with ``CDError`` when the HIP device is unavailable.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 from typing import Optional
 
 import numpy as np
@@ -63,7 +65,8 @@ class RunStats(C.Structure):
     _fields_ = [("status", _I), ("transport_steps", _I), ("height_steps", _I),
                 ("last_step", _D), ("phi_sweeps", _LL), ("h_sweeps", _LL), ("checks", _LL),
                 ("learned_applied", _I), ("learned_rejected", _I),
-                ("max_phi_residual", _D), ("max_h_residual", _D), ("elapsed_ms", _D)]
+                ("max_phi_residual", _D), ("max_h_residual", _D), ("elapsed_ms", _D),
+                ("recoveries", _I)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -121,6 +124,8 @@ def load(path: str = LIB_PATH):
         "cd_design_device_ptr": ([_P, _I], _P),
         "cd_design_last_solve_stats": ([_P, C.POINTER(SolveStats), C.POINTER(SolveStats)], _I),
         "cd_design_get_params": ([_P, C.POINTER(DesignParams)], _I),
+        "cd_design_laplacian_smoothing": ([_P, _D], _I),
+        "cd_design_set_miss_recovery": ([_P, _I], _I),
         "cd_png_read_rgba8": ([C.c_char_p, C.POINTER(_I), C.POINTER(_I), _P, _SZ], _I),
         "cd_image_gray_resize": ([_P, _P, _I, _I, _I, _I, _P], _I),
         "cd_image_gray_resize_device": ([_P, _P, _I, _I, _I, _I, _P], _I),
@@ -173,6 +178,7 @@ def exported_symbols():
         "cd_image_gray_resize", "cd_image_gray_resize_device", "cd_obj_write_solid",
         "cd_json_write_heightmap", "cd_svg_write_grid", "cd_svg_write_triangles",
         "cd_design_save_solid_obj", "cd_design_save_heightmap_json", "cd_design_export_svg",
+        "cd_design_laplacian_smoothing", "cd_design_set_miss_recovery",
     ]
 
 
@@ -192,8 +198,20 @@ def default_opts(**kw) -> SolveOpts:
     return o
 
 
+_live_contexts: "weakref.WeakSet[Context]" = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all():
+    # before the HIP runtime's own exit handlers run: nets and designs first,
+    # then their contexts
+    for c in list(_live_contexts):
+        c.close()
+
+
 class Context:
-    """One HIP device + stream (cd_context)."""
+    """One HIP device + stream (cd_context).  Nets and designs created on it
+    are closed before it (close() or interpreter exit)."""
 
     def __init__(self, device: int = 0):
         L = load()
@@ -202,9 +220,13 @@ class Context:
         if rc != OK:
             raise CDError(rc, f"cd_context_create: {status_string(rc)}")
         self.L, self.h = L, h
+        self.children = weakref.WeakSet()
+        _live_contexts.add(self)
 
     def close(self):
         if getattr(self, "h", None):
+            for ch in list(getattr(self, "children", ())):
+                ch.close()
             self.L.cd_context_destroy(self.h)
             self.h = None
 
@@ -345,6 +367,7 @@ class CausticDesign:
         self.ctx.check(self.L.cd_design_create(self.ctx.h, C.byref(p), C.byref(h)),
                        "cd_design_create")
         self.d = h
+        self.ctx.children.add(self)
 
     def close(self):
         if getattr(self, "d", None):
@@ -392,6 +415,13 @@ class CausticDesign:
                                   C.byref(rs))
         self.ctx.check(rc, "cd_design_run", allow)
         return rs
+
+    def laplacian_smoothing(self, factor: float):
+        """mesh->laplacian_smoothing(mesh->target_points, factor) (mesh.cpp:534-606)."""
+        self.ctx.check(self.L.cd_design_laplacian_smoothing(self.d, factor), "laplacian_smoothing")
+
+    def set_miss_recovery(self, on: bool):
+        self.ctx.check(self.L.cd_design_set_miss_recovery(self.d, int(on)), "set_miss_recovery")
 
     # file outputs (caustic_design.cpp:20-26,55-61; main.cpp:269)
     def save_solid_obj_source(self, filename: str):
@@ -531,6 +561,7 @@ class Net:
         h = _P()
         ctx.check(self.L.cd_net_create(ctx.h, C.byref(self.arch), C.byref(h)), "cd_net_create")
         self.h = h
+        ctx.children.add(self)
         self.n = self.L.cd_net_num_params(h)
 
     def close(self):

@@ -172,6 +172,8 @@ void cd_context_destroy(cd_context* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (cd_net* n : std::vector<cd_net*>(ctx->nets)) cd_net_detach(n);
+    for (cd_design* d : std::vector<cd_design*>(ctx->designs)) cd_design_detach(d);
     ctx->e32.release();
     ctx->e32b.release();
     ctx->r32.release();

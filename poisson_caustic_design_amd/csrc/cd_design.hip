@@ -516,6 +516,44 @@ __global__ __launch_bounds__(256) void k_height_apply(const double* __restrict__
     if (threadIdx.x == 0) partials[blockIdx.x] = a;
 }
 
+// Laplacian smoothing of the target mesh (mesh.cpp:534-606): neighbour
+// average from the unmodified points (interior: x and y over 4 neighbours;
+// edges: the free coordinate over 3 in the reference's summation order;
+// corners kept), then p = avg * f + p * (1 - f) for every vertex
+__global__ __launch_bounds__(256) void k_laplacian_smooth(const double* __restrict__ src,
+                                                          double* __restrict__ dst, int mx,
+                                                          int my, double f) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= mx * my) return;
+    const int y = i / mx, x = i - y * mx;
+    auto X = [&](int xx, int yy) { return src[2 * (yy * mx + xx)]; };
+    auto Y = [&](int xx, int yy) { return src[2 * (yy * mx + xx) + 1]; };
+    double ax = src[2 * i], ay = src[2 * i + 1];
+    const bool left = x == 0, right = x == mx - 1, top = y == 0, bottom = y == my - 1;
+    if ((left || right) && (top || bottom)) {
+        // corner: unchanged
+    } else if (left) {
+        ay = ((0.0 + Y(x + 1, y)) + Y(x, y + 1)) + Y(x, y - 1);
+        ay /= 3.0;
+    } else if (right) {
+        ay = ((0.0 + Y(x - 1, y)) + Y(x, y - 1)) + Y(x, y + 1);
+        ay /= 3.0;
+    } else if (top) {
+        ax = ((0.0 + X(x - 1, y)) + X(x + 1, y)) + X(x, y + 1);
+        ax /= 3.0;
+    } else if (bottom) {
+        ax = ((0.0 + X(x - 1, y)) + X(x, y - 1)) + X(x + 1, y);
+        ax /= 3.0;
+    } else {
+        ax = (((0.0 + X(x - 1, y)) + X(x + 1, y)) + X(x, y - 1)) + X(x, y + 1);
+        ay = (((0.0 + Y(x - 1, y)) + Y(x + 1, y)) + Y(x, y - 1)) + Y(x, y + 1);
+        ax /= 4.0;
+        ay /= 4.0;
+    }
+    dst[2 * i] = ax * f + src[2 * i] * (1.0 - f);
+    dst[2 * i + 1] = ay * f + src[2 * i + 1] * (1.0 - f);
+}
+
 __global__ void k_points3(const double* __restrict__ xy, const double* __restrict__ z, int V,
                           double* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -537,6 +575,9 @@ struct cd_design {
     size_t V = 0, G = 0;
     int T = 0;
     bool initialized = false;
+    bool miss_recovery = false;  // smooth and continue on a raster miss
+    int recoveries = 0;          // misses recovered since the last init
+    DevBuf smooth_tmp;           // V x 2 copy of the target points
     DevBuf txy, sxy, sz, tareas, errors, area, vgrad, inv, nrm_x, nrm_y, hv, quadacc;
     DevBuf pixels, image, raster, phi, h, div, normx, normy, owner, owner_src, owner_v;
     DevBuf partials, scalars, tmp3;
@@ -671,6 +712,7 @@ int design_init_from_device_image(cd_design* d, const double* img) {
                        d->sxy.as<double>(), d->mx, d->my, d->nx, d->ny, d->p.width, d->p.height,
                        d->owner_src.as<int>());
     CD_CHECK_LAUNCH(c);
+    d->recoveries = 0;
     d->initialized = true;
     d->folded_cells = 0;
     return CD_OK;
@@ -717,18 +759,31 @@ int cd_design_create(cd_context* ctx, const cd_design_params* p, cd_design** out
     d->T = 2 * (d->mx - 1) * (d->my - 1);
     int st = alloc_design(d);
     if (st != CD_OK) { delete d; return st; }
+    ctx->designs.push_back(d);
     *out = d;
     return CD_OK;
 }
 
 void cd_design_destroy(cd_design* d) {
     if (!d) return;
-    (void)hipStreamSynchronize(d->ctx->stream);
+    if (d->ctx) {
+        (void)hipStreamSynchronize(d->ctx->stream);
+        cd_design_detach(d);
+    }
     delete d;
+}
+
+void cd_design_detach(cd_design* d) {
+    cd_context* c = d->ctx;
+    if (!c) return;
+    for (size_t i = 0; i < c->designs.size(); ++i)
+        if (c->designs[i] == d) { c->designs.erase(c->designs.begin() + i); break; }
+    d->ctx = nullptr;
 }
 
 int cd_design_set_solve_opts(cd_design* d, const cd_solve_opts* phi, const cd_solve_opts* h) {
     if (!d) return CD_ERR_BAD_ARG;
+    if (!d->ctx) return CD_ERR_STATE;
     if (phi) d->p.phi_solve = *phi;
     if (h) d->p.h_solve = *h;
     return CD_OK;
@@ -736,6 +791,7 @@ int cd_design_set_solve_opts(cd_design* d, const cd_solve_opts* phi, const cd_so
 
 int cd_design_init(cd_design* d, const double* image) {
     if (!d || !image) return CD_ERR_BAD_ARG;
+    if (!d->ctx) return CD_ERR_STATE;
     cd_context* c = d->ctx;
     CD_HIP(c, hipSetDevice(c->device));
     CD_HIP(c, hipMemcpyAsync(d->image.ptr, image, 8 * d->G, hipMemcpyHostToDevice, c->stream));
@@ -746,12 +802,46 @@ int cd_design_init(cd_design* d, const double* image) {
 
 int cd_design_init_device(cd_design* d, const double* d_image) {
     if (!d || !d_image) return CD_ERR_BAD_ARG;
+    if (!d->ctx) return CD_ERR_STATE;
     CD_HIP(d->ctx, hipSetDevice(d->ctx->device));
     return design_init_from_device_image(d, d_image);
 }
 
+namespace {
+int smooth_target(cd_design* d, double factor) {
+    cd_context* c = d->ctx;
+    const int V = (int)d->V;
+    CD_TRY(c->ensure(d->smooth_tmp, sizeof(double) * 2 * (size_t)V));
+    CD_HIP(c, hipMemcpyAsync(d->smooth_tmp.ptr, d->txy.ptr, sizeof(double) * 2 * (size_t)V,
+                             hipMemcpyDeviceToDevice, c->stream));
+    hipLaunchKernelGGL(k_laplacian_smooth, dim3(div_up(V, CD_BLOCK)), dim3(CD_BLOCK), 0,
+                       c->stream, d->smooth_tmp.as<double>(), d->txy.as<double>(), d->mx, d->my,
+                       factor);
+    CD_CHECK_LAUNCH(c);
+    return CD_OK;
+}
+}  // namespace
+
+int cd_design_laplacian_smoothing(cd_design* d, double factor) {
+    if (!d) return CD_ERR_BAD_ARG;
+    if (!d->ctx) return CD_ERR_STATE;
+    if (!d->initialized) return CD_ERR_STATE;
+    CD_HIP(d->ctx, hipSetDevice(d->ctx->device));
+    CD_TRY(smooth_target(d, factor));
+    CD_HIP(d->ctx, hipStreamSynchronize(d->ctx->stream));
+    return CD_OK;
+}
+
+int cd_design_set_miss_recovery(cd_design* d, int on) {
+    if (!d) return CD_ERR_BAD_ARG;
+    if (!d->ctx) return CD_ERR_STATE;
+    d->miss_recovery = on != 0;
+    return CD_OK;
+}
+
 int cd_design_transport_step(cd_design* d, double* step_out) {
     if (!d) return CD_ERR_BAD_ARG;
+    if (!d->ctx) return CD_ERR_STATE;
     if (!d->initialized) return CD_ERR_STATE;
     cd_context* c = d->ctx;
     hipStream_t s = c->stream;
@@ -771,8 +861,16 @@ int cd_design_transport_step(cd_design* d, double* step_out) {
     CD_TRY(raster(d, d->txy.as<double>(), d->owner.as<int>(), true, d->errors.as<double>(),
                   nullptr, d->raster.as<double>(), nullptr, &misses));
     if (misses > 0) {
-        c->set_error("raster miss: " + std::to_string((long long)misses) + " samples");
         if (step_out) *step_out = NAN;
+        if (d->miss_recovery) {
+            // caustic_design.cpp:215-218: smooth the target mesh and report a
+            // NaN step (the driver's `step < conv` test keeps iterating)
+            CD_TRY(smooth_target(d, (double)0.1f));
+            d->recoveries += 1;
+            CD_HIP(c, hipStreamSynchronize(s));
+            return CD_OK;
+        }
+        c->set_error("raster miss: " + std::to_string((long long)misses) + " samples");
         return CD_ERR_RASTER_MISS;
     }
     // zero-mean projection (caustic_design.cpp:221) and the phi solve (:222)
@@ -798,6 +896,7 @@ int cd_design_transport_step(cd_design* d, double* step_out) {
 
 int cd_design_height_step(cd_design* d, double* update_out) {
     if (!d) return CD_ERR_BAD_ARG;
+    if (!d->ctx) return CD_ERR_STATE;
     if (!d->initialized) return CD_ERR_STATE;
     cd_context* c = d->ctx;
     hipStream_t s = c->stream;
@@ -865,11 +964,13 @@ int cd_design_height_step(cd_design* d, double* update_out) {
 int cd_design_run(cd_design* d, int max_transport_steps, double conv_tres, int height_steps,
                   cd_run_stats* rs) {
     if (!d) return CD_ERR_BAD_ARG;
+    if (!d->ctx) return CD_ERR_STATE;
     if (!d->initialized) return CD_ERR_STATE;
     auto t0 = std::chrono::steady_clock::now();
     cd_run_stats r;
     std::memset(&r, 0, sizeof(r));
     int status = CD_OK;
+    const int recoveries0 = d->recoveries;
     // main.cpp:243-256: OT steps, stop after the first step below conv_tres
     for (int it = 0; it < max_transport_steps; ++it) {
         double step = 0.0;
@@ -897,6 +998,7 @@ int cd_design_run(cd_design* d, int max_transport_steps, double conv_tres, int h
         if (st != CD_OK) { status = st; break; }
     }
     r.status = status;
+    r.recoveries = d->recoveries - recoveries0;
     r.elapsed_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (rs) *rs = r;
     return status;
@@ -934,12 +1036,14 @@ int cd_image_gray_resize(cd_context* c, const unsigned char* rgba, int width, in
 
 int cd_design_get_params(const cd_design* d, cd_design_params* p) {
     if (!d || !p) return CD_ERR_BAD_ARG;
+    if (!d->ctx) return CD_ERR_STATE;
     *p = d->p;
     return CD_OK;
 }
 
 size_t cd_design_field_size(const cd_design* d, int field) {
     if (!d) return 0;
+    if (!d->ctx) return 0;
     switch (field) {
         case CD_FIELD_TARGET_POINTS:
         case CD_FIELD_SOURCE_POINTS: return 3 * d->V;
@@ -962,6 +1066,7 @@ size_t cd_design_field_size(const cd_design* d, int field) {
 
 const void* cd_design_device_ptr(cd_design* d, int field) {
     if (!d) return nullptr;
+    if (!d->ctx) return nullptr;
     switch (field) {
         case CD_FIELD_PHI: return d->phi.ptr;
         case CD_FIELD_H: return d->h.ptr;
@@ -981,6 +1086,7 @@ const void* cd_design_device_ptr(cd_design* d, int field) {
 
 int cd_design_get(cd_design* d, int field, double* out, size_t count) {
     if (!d || !out) return CD_ERR_BAD_ARG;
+    if (!d->ctx) return CD_ERR_STATE;
     const size_t n = cd_design_field_size(d, field);
     if (n == 0 || count < n) return CD_ERR_BAD_ARG;
     cd_context* c = d->ctx;
@@ -1017,6 +1123,7 @@ int cd_design_get(cd_design* d, int field, double* out, size_t count) {
 
 int cd_design_set(cd_design* d, int field, const double* in, size_t count) {
     if (!d || !in) return CD_ERR_BAD_ARG;
+    if (!d->ctx) return CD_ERR_STATE;
     const size_t n = cd_design_field_size(d, field);
     if (n == 0 || count < n) return CD_ERR_BAD_ARG;
     cd_context* c = d->ctx;
@@ -1047,6 +1154,7 @@ int cd_design_set(cd_design* d, int field, const double* in, size_t count) {
 
 int cd_design_last_solve_stats(const cd_design* d, cd_solve_stats* phi, cd_solve_stats* h) {
     if (!d) return CD_ERR_BAD_ARG;
+    if (!d->ctx) return CD_ERR_STATE;
     if (phi) *phi = d->last_phi;
     if (h) *h = d->last_h;
     return CD_OK;

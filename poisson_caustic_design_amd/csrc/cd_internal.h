@@ -85,8 +85,15 @@ struct PendingTimer {
     double bytes;
 };
 
+void cd_net_detach(cd_net* n);
+void cd_design_detach(cd_design* d);
+
 struct cd_context {
     int device = 0;
+    // nets and designs created on this context; destroying the context
+    // detaches them (their later calls return CD_ERR_STATE)
+    std::vector<cd_net*> nets;
+    std::vector<cd_design*> designs;
     hipStream_t stream = nullptr;
     std::string last_error;
     bool profiling = false;

@@ -760,6 +760,15 @@ int cd_net_forward_internal(cd_net* n, const float* d_x, float* d_y, int h, int 
 
 const cd_net_arch* cd_net_arch_of(const cd_net* n) { return &n->arch; }
 
+void cd_net_detach(cd_net* n) {
+    cd_context* c = n->ctx;
+    if (!c) return;
+    if (c->net == n) c->net = nullptr;
+    for (size_t i = 0; i < c->nets.size(); ++i)
+        if (c->nets[i] == n) { c->nets.erase(c->nets.begin() + i); break; }
+    n->ctx = nullptr;
+}
+
 extern "C" {
 
 void cd_net_arch_default(cd_net_arch* a) {
@@ -801,14 +810,17 @@ int cd_net_create(cd_context* ctx, const cd_net_arch* arch, cd_net** out) {
         delete n;
         return st;
     }
+    ctx->nets.push_back(n);
     *out = n;
     return CD_OK;
 }
 
 void cd_net_destroy(cd_net* n) {
     if (!n) return;
-    (void)hipStreamSynchronize(n->ctx->stream);
-    if (n->ctx->net == n) n->ctx->net = nullptr;
+    if (n->ctx) {
+        (void)hipStreamSynchronize(n->ctx->stream);
+        cd_net_detach(n);
+    }
     delete n;
 }
 
@@ -816,6 +828,7 @@ size_t cd_net_num_params(const cd_net* n) { return n ? n->nparam : 0; }
 
 int cd_net_set_params(cd_net* n, const float* host, size_t count) {
     if (!n || !host || count != n->nparam) return CD_ERR_BAD_ARG;
+    if (!n->ctx) return CD_ERR_STATE;
     cd_context* c = n->ctx;
     CD_HIP(c, hipMemcpyAsync(n->w.ptr, host, sizeof(float) * count, hipMemcpyHostToDevice,
                              c->stream));
@@ -829,6 +842,7 @@ int cd_net_set_params(cd_net* n, const float* host, size_t count) {
 
 int cd_net_get_params(cd_net* n, float* host, size_t count) {
     if (!n || !host || count != n->nparam) return CD_ERR_BAD_ARG;
+    if (!n->ctx) return CD_ERR_STATE;
     cd_context* c = n->ctx;
     CD_HIP(c, hipMemcpyAsync(host, n->w.ptr, sizeof(float) * count, hipMemcpyDeviceToHost,
                              c->stream));
@@ -838,6 +852,7 @@ int cd_net_get_params(cd_net* n, float* host, size_t count) {
 
 int cd_net_init_params(cd_net* n, unsigned long long seed) {
     if (!n) return CD_ERR_BAD_ARG;
+    if (!n->ctx) return CD_ERR_STATE;
     std::vector<float> w(n->nparam, 0.f);
     std::mt19937_64 rng(seed);
     for (const NetConv& cv : n->convs) {
@@ -862,6 +877,7 @@ int cd_net_init_params(cd_net* n, unsigned long long seed) {
 
 int cd_net_forward(cd_net* n, const float* d_x, float* d_y, int batch, int h, int w) {
     if (!n || !d_x || !d_y || batch < 1) return CD_ERR_BAD_ARG;
+    if (!n->ctx) return CD_ERR_STATE;
     CD_HIP(n->ctx, hipSetDevice(n->ctx->device));
     CD_TRY(prep(n, d_x, batch, h, w));
     CD_TRY(forward(n, batch, h, w));
@@ -876,6 +892,7 @@ int cd_net_forward(cd_net* n, const float* d_x, float* d_y, int batch, int h, in
 int cd_net_loss_grad(cd_net* n, const float* d_x, const float* d_t, int batch, int h, int w,
                      double* loss, float* host_grad, size_t count) {
     if (!n || !d_x || !d_t || batch < 1) return CD_ERR_BAD_ARG;
+    if (!n->ctx) return CD_ERR_STATE;
     if (host_grad && count != n->nparam) return CD_ERR_BAD_ARG;
     cd_context* c = n->ctx;
     CD_HIP(c, hipSetDevice(c->device));
@@ -895,6 +912,7 @@ int cd_net_loss_grad(cd_net* n, const float* d_x, const float* d_t, int batch, i
 int cd_net_loss_grad_operator(cd_net* n, const float* d_x, int batch, int h, int w,
                               double* loss, float* host_grad, size_t count) {
     if (!n || !d_x || batch < 1) return CD_ERR_BAD_ARG;
+    if (!n->ctx) return CD_ERR_STATE;
     if (host_grad && count != n->nparam) return CD_ERR_BAD_ARG;
     cd_context* c = n->ctx;
     CD_HIP(c, hipSetDevice(c->device));

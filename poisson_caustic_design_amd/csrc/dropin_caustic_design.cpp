@@ -180,15 +180,33 @@ void Caustic_design::perform_height_map_iteration(int /*itr*/) {
     if (mirror_) refresh_grids();
 }
 
-void Caustic_design::save_solid_obj_target(const std::string&) {
-    throw std::logic_error("save_solid_obj_target: file output is not part of this build");
+// file outputs (caustic_design.cpp:20-26,55-61): the reference's writers'
+// formats, with the thickness member as it is at call time
+void Caustic_design::save_solid_obj_target(const std::string& filename) {
+    if (!design_) throw std::logic_error("save_solid_obj_target before initialize_solvers");
+    const size_t V = (size_t)mesh_res_x * mesh_res_y;
+    std::vector<double> tgt(3 * V), src(3 * V);
+    check(ctx_, cd_design_get(design_, CD_FIELD_TARGET_POINTS, tgt.data(), 3 * V), "get");
+    check(ctx_, cd_design_get(design_, CD_FIELD_SOURCE_POINTS, src.data(), 3 * V), "get");
+    check(ctx_, cd_obj_write_solid(filename.c_str(), tgt.data(), src.data(), mesh_res_x,
+                                   mesh_res_y, thickness, width, height), "save_solid_obj_target");
 }
-void Caustic_design::save_solid_obj_source(const std::string&) {
-    throw std::logic_error("save_solid_obj_source: file output is not part of this build");
+void Caustic_design::save_solid_obj_source(const std::string& filename) {
+    if (!design_) throw std::logic_error("save_solid_obj_source before initialize_solvers");
+    const size_t V = (size_t)mesh_res_x * mesh_res_y;
+    std::vector<double> src(3 * V);
+    check(ctx_, cd_design_get(design_, CD_FIELD_SOURCE_POINTS, src.data(), 3 * V), "get");
+    check(ctx_, cd_obj_write_solid(filename.c_str(), src.data(), src.data(), mesh_res_x,
+                                   mesh_res_y, thickness, width, height), "save_solid_obj_source");
 }
-void Caustic_design::export_paramererization_to_svg(const std::string&, double) {
-    throw std::logic_error("export_paramererization_to_svg: file output is not part of this build");
+void Caustic_design::export_paramererization_to_svg(const std::string& filename,
+                                                      double line_width) {
+    if (!design_) throw std::logic_error("export_paramererization_to_svg before initialize_solvers");
+    check(ctx_, cd_design_export_svg(design_, filename.c_str(), CD_SVG_PARAMETERIZATION,
+                                     line_width), "export_paramererization_to_svg");
 }
-void Caustic_design::export_inverted_transport_map(std::string, double) {
-    throw std::logic_error("export_inverted_transport_map: file output is not part of this build");
+void Caustic_design::export_inverted_transport_map(std::string filename, double stroke_width) {
+    if (!design_) throw std::logic_error("export_inverted_transport_map before initialize_solvers");
+    check(ctx_, cd_design_export_svg(design_, filename.c_str(), CD_SVG_INVERTED_MAP, stroke_width),
+          "export_inverted_transport_map");
 }

@@ -97,3 +97,33 @@ def test_raster_miss_is_an_error_not_exit(ctx, oracle):
     with pytest.raises(capi.CDError) as ei:
         cd.perform_transport_iteration()
     assert ei.value.status == capi.RASTER_MISS
+
+
+def test_laplacian_smoothing_parity(ctx, oracle):
+    M = 12
+    cd, _ = make_pair(ctx, oracle, M)
+    rng = np.random.default_rng(21)
+    tp = cd.get("target_points")
+    tp[:, :2] += 0.003 * rng.standard_normal((M * M, 2))
+    cd.set("target_points", tp)
+    f = float(np.float32(0.1))
+    cd.laplacian_smoothing(f)
+    exp = oracle.laplacian_smoothing(tp[:, :2], M, M, f)
+    assert np.array_equal(cd.get("target_points")[:, :2], exp)
+
+
+def test_raster_miss_recovery_smooths_and_continues(ctx, oracle):
+    """caustic_design.cpp:215-218: a miss smooths the target mesh (factor 0.1f)
+    and the step reports NaN; the driver loop keeps going."""
+    M = 10
+    cd, _ = make_pair(ctx, oracle, M)
+    tp = cd.get("target_points")
+    tp[M * M - 1, :2] = [0.5 * 0.9, 0.5 * 0.9]
+    cd.set("target_points", tp)
+    cd.set_miss_recovery(True)
+    step = cd.perform_transport_iteration()
+    assert np.isnan(step)
+    exp = oracle.laplacian_smoothing(tp[:, :2], M, M, float(np.float32(0.1)))
+    assert np.array_equal(cd.get("target_points")[:, :2], exp)
+    rs = cd.run(3, float(np.float32(0.01)), 0, allow=(capi.OK, capi.RASTER_MISS))
+    assert rs.transport_steps >= 1

@@ -60,3 +60,52 @@ def test_design_outputs_match_reference_writers(ctx, reflib, tmp_path):
     cd.save_heightmap_json(str(j))
     assert j.read_text() == ingest.heightmap_json(cd.get("h"))
     cd.close()
+
+
+def test_cli_matches_library_calls(ctx, tmp_path):
+    """The command-line drop-in (main.cpp:137-272 flow) writes the same
+    output.obj / heightmap.json / progress SVGs as the same calls made through
+    the library directly."""
+    import subprocess
+
+    cli = os.path.join(os.path.dirname(capi.LIB_PATH), "caustic_design_amd")
+    out, prog = tmp_path / "out", tmp_path / "prog"
+    out.mkdir()
+    prog.mkdir()
+    png = os.path.join(GOLDEN_IMG, "siggraph.png")
+    r = subprocess.run([cli, "--input_png", png, "--res_w", "24", "--output", str(out) + "/",
+                        "--progress_out", str(prog) + "/", "--conv_tres=0.02"],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout[-400:], r.stderr[-400:])
+    assert r.returncode == 0
+    assert "Height solver done! Exporting as solidified obj" in r.stdout
+    n_steps = r.stdout.count("starting iteration")
+    # the same flow through the library
+    rgba = capi.png_read_rgba8(png)
+    h, w = rgba.shape[:2]
+    gw, gh, mx, my, mh = ingest.driver_geometry(w, h, 24, 1.0)
+    cd = capi.CausticDesign(ctx)
+    cd.set_mesh_resolution(mx, my)
+    cd.set_domain_resolution(gw, gh)
+    cd.set_mesh_size(1.0, mh)
+    cd.set_lens_focal_length(1.5)
+    cd.set_lens_thickness(0.2)
+    cd.initialize_solvers(capi.image_gray_resize(ctx, rgba, gw, gh))
+    conv = float(np.float32(0.02))
+    steps = 0
+    for _ in range(50):
+        steps += 1
+        if cd.perform_transport_iteration() < conv:
+            break
+    for i in range(3):
+        cd.perform_height_map_iteration(i)
+    assert steps == n_steps
+    cd.save_solid_obj_source(str(tmp_path / "api.obj"))
+    cd.save_heightmap_json(str(tmp_path / "api.json"))
+    cd.export_paramererization_to_svg(str(tmp_path / "api.svg"), 1.0)
+    assert (out / "output.obj").read_bytes() == (tmp_path / "api.obj").read_bytes()
+    assert (out / "heightmap.json").read_bytes() == (tmp_path / "api.json").read_bytes()
+    assert (prog / f"parameterization_{n_steps}.svg").read_bytes() == \
+        (tmp_path / "api.svg").read_bytes()
+    assert (prog / "parameterization_0.svg").exists() and (prog / "inverted.svg").exists()
+    cd.close()

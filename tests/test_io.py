@@ -212,3 +212,19 @@ def test_heightmap_json_matches_restatement(tmp_path):
     p = tmp_path / "h.json"
     capi.json_write_heightmap(str(p), h)
     assert p.read_text() == ingest.heightmap_json(h)
+
+
+def test_cli_arguments():
+    import subprocess
+
+    cli = os.path.join(os.path.dirname(capi.LIB_PATH), "caustic_design_amd")
+    if not os.path.exists(cli):
+        pytest.skip("command-line binary not built")
+    r = subprocess.run([cli, "--help"], capture_output=True, text=True)
+    assert r.returncode == 0 and "--input_png" in r.stdout and "--conv_tres" in r.stdout
+    r = subprocess.run([cli, "--bogus", "1"], capture_output=True, text=True)
+    assert r.returncode == 1 and "bogus" in r.stderr
+    r = subprocess.run([cli, "--res_w", "abc"], capture_output=True, text=True)
+    assert r.returncode == 1
+    r = subprocess.run([cli, "--input_png", "/nonexistent.png"], capture_output=True, text=True)
+    assert r.returncode == 1 and "PNG" in r.stderr

@@ -167,3 +167,36 @@ def test_design_trajectory_vs_reference(oracle, reflib, M, steps):
         assert rc == 0
         assert np.abs(rd.source_points()[:, 2] - od.vec(1)).max() < 1e-12
         assert np.abs(rd.grid(1) - od.grid(1)).max() < 1e-12
+
+
+def test_laplacian_smoothing_matches_reference(oracle, reflib):
+    """orc_laplacian_smoothing == Mesh::laplacian_smoothing (mesh.cpp:534-606)
+    bit for bit on a perturbed mesh, and its closed forms: a regular grid's
+    interior is a fixed point of the averaging; corners never move."""
+    from oracle.bind import RefDesign
+    mx, my = 9, 7
+    W, H = 1.0, 0.75
+    rd = RefDesign(reflib, mx, my, 4 * mx, 4 * my, W, H, 1.5)
+    rd.init(np.ones((4 * my, 4 * mx)))  # builds the reference's Mesh
+    rng = np.random.default_rng(11)
+    pts = rd.target_points()
+    pts[:, :2] += 0.01 * rng.standard_normal((mx * my, 2))
+    rd.set_target_points(pts)
+    f = float(np.float32(0.1))
+    rd.laplacian_smoothing(f)
+    ref = rd.target_points()[:, :2]
+    ours = oracle.laplacian_smoothing(pts[:, :2], mx, my, f)
+    assert np.array_equal(ours, ref)
+    # closed forms on the regular grid
+    xs, ys = np.meshgrid(np.linspace(0, W, mx), np.linspace(0, H, my))
+    reg = np.stack([xs.ravel(), ys.ravel()], 1)
+    sm = oracle.laplacian_smoothing(reg, mx, my, 0.5)
+    assert np.allclose(sm, reg, atol=1e-15)
+    pert = pts[:, :2].copy()
+    one = oracle.laplacian_smoothing(pert, mx, my, 1.0)
+    for c in (0, mx - 1, (my - 1) * mx, mx * my - 1):
+        assert np.array_equal(one[c], pert[c])
+    # interior vertex: exact 4-neighbour mean at f = 1
+    i = 3 * mx + 4
+    nb = pert[[i - 1, i + 1, i - mx, i + mx]]
+    assert np.allclose(one[i], nb.mean(0), rtol=0, atol=1e-15)
