@@ -85,8 +85,10 @@ struct PendingTimer {
     double bytes;
 };
 
-void cd_net_detach(cd_net* n);
-void cd_design_detach(cd_design* d);
+// detach a net / design from its context (internal; C linkage because the
+// design's definition sits in that file's extern "C" block)
+extern "C" void cd_net_detach(cd_net* n);
+extern "C" void cd_design_detach(cd_design* d);
 
 struct cd_context {
     int device = 0;

@@ -760,7 +760,7 @@ int cd_net_forward_internal(cd_net* n, const float* d_x, float* d_y, int h, int 
 
 const cd_net_arch* cd_net_arch_of(const cd_net* n) { return &n->arch; }
 
-void cd_net_detach(cd_net* n) {
+extern "C" void cd_net_detach(cd_net* n) {
     cd_context* c = n->ctx;
     if (!c) return;
     if (c->net == n) c->net = nullptr;
