@@ -81,6 +81,14 @@ def build(verbose: bool = False, defines=(), out: str = LIB) -> str:
               "-o", cli + ".tmp", "-L" + os.path.dirname(LIB), "-l:libcaustic_amd.so",
               "-Wl,-rpath,$ORIGIN"])
         os.replace(cli + ".tmp", cli)
+        # a caller compiled unchanged against the drop-in C++ headers (tests)
+        src = os.path.join(ROOT, "tests", "dropin", "dropin_main.cpp")
+        if os.path.exists(src):
+            exe = os.path.join(ROOT, "tests", "dropin", "dropin_main")
+            _run(["g++", "-O2", "-std=c++17", "-I" + INCLUDE, src, "-o", exe + ".tmp",
+                  "-L" + os.path.dirname(LIB), "-l:libcaustic_amd.so",
+                  "-Wl,-rpath,$ORIGIN/../../poisson_caustic_design_amd"])
+            os.replace(exe + ".tmp", exe)
     return out
 
 

@@ -15,7 +15,10 @@ mean((cs * Lc net(x) - x)^2), which needs inputs only:
 
 Training runs on the GPU through the library's own BF16-MFMA forward/backward
 kernels and Adam (cd_net_train_step_operator).  Device buffers for the inputs
-are torch tensors (memory plumbing only).
+are torch tensors (memory plumbing only).  Import torch before the first
+capi.Context of the process: torch ships its own HIP runtime with the same
+soname, and it must be the one loaded first or torch finds no device.
+Loading and using trained weights (load_net) needs no torch.
 
     python -m poisson_caustic_design_amd.learned --mesh-res 512 --designs 16 \
         --out poisson_caustic_design_amd/weights/learned_grid2048.npz
@@ -212,6 +215,8 @@ def load_net(ctx: capi.Context, path: str) -> capi.Net:
 
 
 def main(argv=None):
+    import torch  # noqa: F401  (before the first Context: see the module docstring)
+
     ap = argparse.ArgumentParser()
     ap.add_argument("--mesh-res", type=int, default=512)
     ap.add_argument("--designs", type=int, default=16)

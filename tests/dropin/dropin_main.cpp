@@ -1,0 +1,77 @@
+// A caller written against the reference's own headers and call sequence
+// (main.cpp:218-269, src/solver.h), compiled unchanged against the drop-in
+// headers in include/ and linked with libcaustic_amd.so.  Used by
+// tests/test_gpu_dropin.py.
+//
+//   dropin_main <mesh_res> <out_dir/>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "caustic_design.h"
+#include "solver.h"
+
+int main(int argc, char** argv) {
+    if (argc < 3) return 1;
+    const int mesh_resolution = std::atoi(argv[1]);
+    const std::string output_path = argv[2];
+    const int N = 4 * mesh_resolution;
+    // a deterministic target: two Gaussian blobs on a dim background
+    std::vector<std::vector<double>> pixels(N, std::vector<double>(N));
+    for (int y = 0; y < N; ++y)
+        for (int x = 0; x < N; ++x) {
+            const double u = (x + 0.5) / N, v = (y + 0.5) / N;
+            pixels[y][x] = 0.1 + std::exp(-((u - 0.3) * (u - 0.3) + (v - 0.4) * (v - 0.4)) / 0.01) +
+                           0.5 * std::exp(-((u - 0.7) * (u - 0.7) + (v - 0.6) * (v - 0.6)) / 0.02);
+        }
+
+    Caustic_design caustic_design;
+    caustic_design.set_mesh_resolution(mesh_resolution, mesh_resolution);
+    caustic_design.set_domain_resolution(N, N);
+    caustic_design.set_mesh_size(1.0, 1.0);
+    caustic_design.set_lens_focal_length(1.5);
+    caustic_design.set_lens_thickness(0.2);
+    caustic_design.set_solver_max_threads(1);
+    caustic_design.initialize_solvers(pixels);
+
+    int steps = 0;
+    for (int itr = 0; itr < 50; itr++) {
+        double step_size = caustic_design.perform_transport_iteration();
+        ++steps;
+        if (step_size < (double)0.01f) break;
+    }
+    for (int itr = 0; itr < 3; itr++) caustic_design.perform_height_map_iteration(itr);
+    caustic_design.save_solid_obj_source(output_path + "output.obj");
+
+    double hs = 0.0;
+    for (auto& row : caustic_design.h)
+        for (double v : row) hs += std::fabs(v);
+    std::printf("steps %d\nh_abs_sum %.17g\n", steps, hs);
+    std::printf("mesh_points %zu\n", caustic_design.mesh->source_points.size());
+
+    // the free solver entry point (src/solver.h:8) on a small problem
+    const int w = 24, h = 16;
+    std::vector<std::vector<double>> D(h, std::vector<double>(w)), phi(h, std::vector<double>(w, 0.0));
+    double mean = 0.0;
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) mean += (D[y][x] = std::cos(3.14159265358979 * (x + 0.5) / w) +
+                                                       0.5 * std::cos(2 * 3.14159265358979 * (y + 0.5) / h));
+    mean /= w * h;
+    for (auto& row : D)
+        for (double& v : row) v -= mean;
+    poisson_solver(D, phi, w, h, 100000, 1e-10, 1);
+    double rmax = 0.0;
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            double s = 0.0, cnt = 0.0;
+            if (x > 0) { s += phi[y][x - 1]; cnt += 1; }
+            if (x < w - 1) { s += phi[y][x + 1]; cnt += 1; }
+            if (y > 0) { s += phi[y - 1][x]; cnt += 1; }
+            if (y < h - 1) { s += phi[y + 1][x]; cnt += 1; }
+            rmax = std::fmax(rmax, std::fabs(s - cnt * phi[y][x] - D[y][x]));
+        }
+    std::printf("poisson_max_residual %.3e\n", rmax);
+    return 0;
+}
