@@ -272,6 +272,9 @@ typedef struct {
     int smooth_sweeps; /* red-black sweeps between learned corrections (4)   */
     double smooth_omega; /* over-relaxation of those sweeps (1.2)            */
     int linear;        /* 1: no activation (a linear operator, default)      */
+    int out_smooth;    /* smoothing steps y += Lc y / 8 on the output (0)    */
+    int loss_smooth;   /* the same steps on the operator-loss residual (0):  */
+                       /* weights it towards the low frequencies             */
 } cd_net_arch;
 
 void cd_net_arch_default(cd_net_arch* a);

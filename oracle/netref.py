@@ -34,10 +34,13 @@ class Conv:
 
 
 class NetRef:
-    def __init__(self, levels=4, channels=(16, 32, 64, 64), linear=False):
+    def __init__(self, levels=4, channels=(16, 32, 64, 64), linear=False, out_smooth=0,
+                 loss_smooth=0):
         self.L = levels
         self.C = list(channels)[:levels]
         self.act = 0 if linear else 1
+        self.out_smooth = out_smooth    # y = S^k head, S = I + Lc/8
+        self.loss_smooth = loss_smooth  # operator loss on S^k (cs Lc y - x)
         self.convs: list[Conv] = []
         self.tensors: list[tuple[int, int]] = []  # (level, channels)
         self.nparam = 0
@@ -95,12 +98,25 @@ class NetRef:
             if self.act:
                 std = np.sqrt(2.0 / fan) * (0.1 if cv.out < 0 else 1.0)
             else:
-                std = np.sqrt(1.0 / (3.0 * fan))
+                std = np.sqrt(1.0 / (3.0 * fan)) * (0.0 if cv.out < 0 else 1.0)
             w = np.zeros((cv.coutpad, cv.kpad), dtype=np.float32)
             vals = rng.standard_normal((cv.cout, cv.K)).astype(np.float32) * std
             if i == 0:
                 vals[:, [k for k in range(cv.K) if k % cin != 0]] = 0.0
             w[:cv.cout, :cv.K] = vals
+            p[cv.woff:cv.woff + cv.nparam] = w.ravel()
+        return p
+
+    def random_params(self, seed=0, head_scale=0.3):
+        """Random weights in every layer (head included), for gradient and
+        parity checks that must not start from a zero head."""
+        rng = np.random.default_rng(seed)
+        p = np.zeros(self.nparam, dtype=np.float32)
+        for cv in self.convs:
+            cin = cv.c1 + cv.c2
+            std = np.sqrt(1.0 / (9 * cin)) * (head_scale if cv.out < 0 else 1.0)
+            w = np.zeros((cv.coutpad, cv.kpad), dtype=np.float32)
+            w[:cv.cout, :cv.K] = rng.standard_normal((cv.cout, cv.K)).astype(np.float32) * std
             p[cv.woff:cv.woff + cv.nparam] = w.ravel()
         return p
 
@@ -169,7 +185,7 @@ class NetRef:
             out = np.maximum(pre, 0.0) if cv.relu else pre
             cache.append((xin, wt, out))
             if cv.out < 0:
-                y = out[..., 0]
+                y = self.smooth(out[..., 0], self.out_smooth)
             else:
                 acts[cv.out] = rd(out)
                 if idx < 2 * L and idx % 2 == 1 and cv.level < L - 1:
@@ -177,6 +193,13 @@ class NetRef:
                 if idx + 1 < len(self.convs) and self.convs[idx + 1].in2 >= 0:
                     acts[self.convs[idx + 1].in1] = self.up(acts[cv.out])
         return y, (acts, cache)
+
+    @classmethod
+    def smooth(cls, v, k):
+        """k steps of v += Lc v / 8 (symmetric, gain 1 - lambda/8)."""
+        for _ in range(k):
+            v = v + cls.lap(v) / 8.0
+        return v
 
     def loss_grad(self, params, x, t, emulate_bf16=False):
         """Mean squared error against targets t and its parameter gradient."""
@@ -197,13 +220,15 @@ class NetRef:
         return out - cnt * v
 
     def loss_grad_operator(self, params, x, emulate_bf16=False):
-        """Operator loss mean((cs * Lc y - x)^2), cs = (max(H, W)/pi)^2, and its
-        parameter gradient ((2 cs / n) Lc u into the head; Lc is symmetric)."""
+        """Operator loss mean((S^k (cs * Lc y - x))^2), cs = (max(H, W)/pi)^2,
+        k = loss_smooth, and its parameter gradient (cs Lc S^k (2 v / n) into
+        y; Lc and S are symmetric)."""
         y, cache = self.forward(params, x, emulate_bf16)
         cs = (max(x.shape[1], x.shape[2]) / np.pi) ** 2
-        u = cs * self.lap(y) - x
-        loss = float(np.mean(u ** 2))
-        return loss, self._backward(cache, 2.0 * cs / y.size * self.lap(u))
+        v = self.smooth(cs * self.lap(y) - x, self.loss_smooth)
+        loss = float(np.mean(v ** 2))
+        dv = 2.0 * v / y.size
+        return loss, self._backward(cache, cs * self.lap(self.smooth(dv, self.loss_smooth)))
 
     def _backward(self, fwd_cache, g_y):
         acts, cache = fwd_cache
@@ -215,7 +240,7 @@ class NetRef:
             xin, wt, out = cache[idx]
             if cv.out < 0:
                 g = np.zeros(out.shape)
-                g[..., 0] = g_y
+                g[..., 0] = self.smooth(g_y, self.out_smooth)  # S is symmetric
             else:
                 g = g_t[cv.out] * (out > 0 if cv.relu else 1.0)
             dx, dw = self.conv_bwd(xin, wt, g)

@@ -275,30 +275,26 @@ __global__ void k_upsample2_bwd(const float* __restrict__ dy, float* __restrict_
     dx[i] += dy[base] + dy[base + C] + dy[base + (long long)W * C] + dy[base + (long long)W * C + C];
 }
 
-// head output channel 0 -> y; loss partials and head gradient
-__global__ __launch_bounds__(256) void k_loss(const float* __restrict__ head, const float* __restrict__ t,
-                                              float* __restrict__ ghead, long long npix,
+// supervised loss on the output y (FP32 [npix]): partial sums of (y - t)^2
+// and dL/dy = 2 (y - t) / npix
+__global__ __launch_bounds__(256) void k_loss(const float* __restrict__ y, const float* __restrict__ t,
+                                              float* __restrict__ gy, long long npix,
                                               double* __restrict__ partials) {
     __shared__ double smem[16];
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double e2 = 0.0;
     if (p < npix) {
-        const float d = head[p * 8] - t[p];
+        const float d = y[p] - t[p];
         e2 = (double)d * d;
-        if (ghead) {
-            ghead[p * 8] = 2.f * d / (float)npix;
-#pragma unroll
-            for (int c = 1; c < 8; ++c) ghead[p * 8 + c] = 0.f;
-        }
+        if (gy) gy[p] = 2.f * d / (float)npix;
     }
     const double s = cd_block_sum(e2, smem);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
-// operator (residual) loss of the coarse solve: with y the head output and
-// Lc the coarse Neumann 5-point operator, u = cs * Lc y - x and
-// loss = mean(u^2); the head gradient is (2 cs / npix) Lc u (Lc symmetric)
-__device__ __forceinline__ float lap_nhwc8(const float* __restrict__ v, int stride, long long p,
+// Neumann 5-point operator (sum of in-grid neighbours - count * centre) of a
+// [B][H][W] field with element stride `stride`
+__device__ __forceinline__ float lap_field(const float* __restrict__ v, int stride, long long p,
                                            int x, int y, int H, int W) {
     const float c = v[p * stride];
     float s = 0.f, cnt = 0.f;
@@ -309,37 +305,65 @@ __device__ __forceinline__ float lap_nhwc8(const float* __restrict__ v, int stri
     return s - cnt * c;
 }
 
-__global__ __launch_bounds__(256) void k_op_residual(const float* __restrict__ head,
+// out = in + Lc(in) / 8: one symmetric smoothing step (gain 1 - lambda/8 on
+// the operator's eigenvalue lambda in [0, 8]); used on the network output and
+// to weight the operator loss towards the frequencies the fine smoother
+// cannot reach
+__global__ __launch_bounds__(256) void k_smooth_step(const float* __restrict__ in,
+                                                     float* __restrict__ out, int B, int H, int W) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (long long)B * H * W) return;
+    const int xx = (int)(p % W), yy = (int)((p / W) % H);
+    out[p] = in[p] + 0.125f * lap_field(in, 1, p, xx, yy, H, W);
+}
+
+// operator residual u = cs * Lc y - x
+__global__ __launch_bounds__(256) void k_op_residual(const float* __restrict__ y,
                                                      const float* __restrict__ x,
                                                      float* __restrict__ u, int B, int H, int W,
-                                                     float cs, double* __restrict__ partials) {
+                                                     float cs) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (long long)B * H * W) return;
+    const int xx = (int)(p % W), yy = (int)((p / W) % H);
+    u[p] = cs * lap_field(y, 1, p, xx, yy, H, W) - x[p];
+}
+
+// partial sums of v^2; dv = (2 / npix) v in place when want_grad
+__global__ __launch_bounds__(256) void k_sumsq(float* __restrict__ v, long long npix, int want_grad,
+                                               double* __restrict__ partials) {
     __shared__ double smem[16];
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double e2 = 0.0;
-    if (p < (long long)B * H * W) {
-        const int xx = (int)(p % W), yy = (int)((p / W) % H);
-        const float d = cs * lap_nhwc8(head, 8, p, xx, yy, H, W) - x[p];
-        u[p] = d;
-        e2 = (double)d * d;
+    if (p < npix) {
+        e2 = (double)v[p] * v[p];
+        if (want_grad) v[p] = 2.f * v[p] / (float)npix;
     }
     const double s = cd_block_sum(e2, smem);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
-__global__ __launch_bounds__(256) void k_op_grad(const float* __restrict__ u,
-                                                 float* __restrict__ ghead, int B, int H, int W,
-                                                 float scale) {
+// gy = cs * Lc du (Lc symmetric)
+__global__ __launch_bounds__(256) void k_op_grad(const float* __restrict__ du,
+                                                 float* __restrict__ gy, int B, int H, int W,
+                                                 float cs) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= (long long)B * H * W) return;
     const int xx = (int)(p % W), yy = (int)((p / W) % H);
-    ghead[p * 8] = scale * lap_nhwc8(u, 1, p, xx, yy, H, W);
-#pragma unroll
-    for (int c = 1; c < 8; ++c) ghead[p * 8 + c] = 0.f;
+    gy[p] = cs * lap_field(du, 1, p, xx, yy, H, W);
 }
 
+// head channel 0 -> y (stride 8 -> 1), and dL/dy -> head gradient
 __global__ void k_head_out(const float* __restrict__ head, float* __restrict__ y, long long npix) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p < npix) y[p] = head[p * 8];
+}
+__global__ void k_head_grad(const float* __restrict__ gy, float* __restrict__ ghead,
+                            long long npix) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    ghead[p * 8] = gy[p];
+#pragma unroll
+    for (int c = 1; c < 8; ++c) ghead[p * 8 + c] = 0.f;
 }
 
 // FP32 master -> BF16 forward weights [coutpad][kpad] (same layout)
@@ -412,7 +436,7 @@ struct cd_net {
     std::vector<NetConv> convs;
     size_t nparam = 0;
     DevBuf w, m, v, gw, wbf, wtbf, partials, head, ghead, x8;
-    DevBuf ures;  // operator-loss residual [B][H][W]
+    DevBuf yout, ytmp, gy, ures, utmp;  // FP32 [B][H][W]: output, dL/dy, loss work
     size_t wtoff_total = 0;
     std::vector<size_t> wtoff;
     long long step = 0;
@@ -584,6 +608,8 @@ int ensure_batch(cd_net* n, int B, int H, int W) {
     CD_TRY(c->ensure(n->x8, sizeof(u16) * px * 8));
     CD_TRY(c->ensure(n->head, sizeof(float) * px * 8));
     CD_TRY(c->ensure(n->ghead, sizeof(float) * px * 8));
+    for (DevBuf* b : {&n->yout, &n->ytmp, &n->gy, &n->ures, &n->utmp})
+        CD_TRY(c->ensure(*b, sizeof(float) * px));
     CD_TRY(c->ensure(n->partials, sizeof(double) * (size_t)div_up(px, 256)));
     n->batch = B;
     n->bh = H;
@@ -678,14 +704,41 @@ int backward(cd_net* n, int B, int H, int W) {
     return CD_OK;
 }
 
-int loss_and_head_grad(cd_net* n, const float* d_t, int B, int H, int W, double* loss,
-                       bool want_grad) {
-    cd_context* c = n->ctx;
+// k symmetric smoothing steps on a [B][H][W] field held in *a (b is scratch);
+// the result ends in *a
+int smooth_steps(cd_net* n, DevBuf& a, DevBuf& b, int k, int B, int H, int W) {
     const long long px = (long long)B * H * W;
-    const int nb = div_up(px, 256);
-    hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, c->stream, n->head.as<float>(), d_t,
-                       want_grad ? n->ghead.as<float>() : nullptr, px, n->partials.as<double>());
-    CD_CHECK_LAUNCH(c);
+    for (int i = 0; i < k; ++i) {
+        hipLaunchKernelGGL(k_smooth_step, dim3(div_up(px, 256)), dim3(256), 0, n->ctx->stream,
+                           a.as<float>(), b.as<float>(), B, H, W);
+        std::swap(a.ptr, b.ptr);
+        std::swap(a.bytes, b.bytes);
+    }
+    CD_CHECK_LAUNCH(n->ctx);
+    return CD_OK;
+}
+
+// network output y = S^out_smooth (head channel 0) into n->yout
+int finish_output(cd_net* n, int B, int H, int W) {
+    const long long px = (long long)B * H * W;
+    hipLaunchKernelGGL(k_head_out, dim3(div_up(px, 256)), dim3(256), 0, n->ctx->stream,
+                       n->head.as<float>(), n->yout.as<float>(), px);
+    CD_CHECK_LAUNCH(n->ctx);
+    return smooth_steps(n, n->yout, n->ytmp, n->arch.out_smooth, B, H, W);
+}
+
+// dL/dy in n->gy -> dL/dhead (S is symmetric, so its transpose is itself)
+int head_grad(cd_net* n, int B, int H, int W) {
+    const long long px = (long long)B * H * W;
+    CD_TRY(smooth_steps(n, n->gy, n->ytmp, n->arch.out_smooth, B, H, W));
+    hipLaunchKernelGGL(k_head_grad, dim3(div_up(px, 256)), dim3(256), 0, n->ctx->stream,
+                       n->gy.as<float>(), n->ghead.as<float>(), px);
+    CD_CHECK_LAUNCH(n->ctx);
+    return CD_OK;
+}
+
+int read_loss(cd_net* n, int nb, long long px, double* loss) {
+    cd_context* c = n->ctx;
     CD_TRY(c->ensure(c->fill, sizeof(double) * 4));
     CD_TRY(cd_reduce_partials(c, n->partials.as<double>(), nb, 1, c->fill.as<double>()));
     double s = 0.0;
@@ -693,6 +746,18 @@ int loss_and_head_grad(cd_net* n, const float* d_t, int B, int H, int W, double*
     CD_HIP(c, hipStreamSynchronize(c->stream));
     if (loss) *loss = s / (double)px;
     return CD_OK;
+}
+
+int loss_and_head_grad(cd_net* n, const float* d_t, int B, int H, int W, double* loss,
+                       bool want_grad) {
+    const long long px = (long long)B * H * W;
+    const int nb = div_up(px, 256);
+    CD_TRY(finish_output(n, B, H, W));
+    hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, n->ctx->stream, n->yout.as<float>(), d_t,
+                       want_grad ? n->gy.as<float>() : nullptr, px, n->partials.as<double>());
+    CD_CHECK_LAUNCH(n->ctx);
+    if (want_grad) CD_TRY(head_grad(n, B, H, W));
+    return read_loss(n, nb, px, loss);
 }
 
 // coarse operator scale (n/pi)^2, n the larger side: makes outputs O(1)
@@ -701,26 +766,30 @@ inline float op_scale(int H, int W) {
     return (float)(m * m);
 }
 
+// operator loss: u = cs Lc y - x, v = S^loss_smooth u, loss = mean(v^2);
+// dL/dy = cs Lc S^loss_smooth (2 v / npix)
 int op_loss_and_head_grad(cd_net* n, const float* d_x, int B, int H, int W, double* loss,
                           bool want_grad) {
     cd_context* c = n->ctx;
     const long long px = (long long)B * H * W;
     const int nb = div_up(px, 256);
-    CD_TRY(c->ensure(n->ures, sizeof(float) * px));
     const float cs = op_scale(H, W);
-    hipLaunchKernelGGL(k_op_residual, dim3(nb), dim3(256), 0, c->stream, n->head.as<float>(), d_x,
-                       n->ures.as<float>(), B, H, W, cs, n->partials.as<double>());
-    if (want_grad)
-        hipLaunchKernelGGL(k_op_grad, dim3(nb), dim3(256), 0, c->stream, n->ures.as<float>(),
-                           n->ghead.as<float>(), B, H, W, 2.f * cs / (float)px);
+    CD_TRY(finish_output(n, B, H, W));
+    hipLaunchKernelGGL(k_op_residual, dim3(nb), dim3(256), 0, c->stream, n->yout.as<float>(), d_x,
+                       n->ures.as<float>(), B, H, W, cs);
     CD_CHECK_LAUNCH(c);
-    CD_TRY(c->ensure(c->fill, sizeof(double) * 4));
-    CD_TRY(cd_reduce_partials(c, n->partials.as<double>(), nb, 1, c->fill.as<double>()));
-    double s = 0.0;
-    CD_HIP(c, hipMemcpyAsync(&s, c->fill.ptr, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    CD_HIP(c, hipStreamSynchronize(c->stream));
-    if (loss) *loss = s / (double)px;
-    return CD_OK;
+    CD_TRY(smooth_steps(n, n->ures, n->utmp, n->arch.loss_smooth, B, H, W));
+    hipLaunchKernelGGL(k_sumsq, dim3(nb), dim3(256), 0, c->stream, n->ures.as<float>(), px,
+                       want_grad ? 1 : 0, n->partials.as<double>());
+    CD_CHECK_LAUNCH(c);
+    if (want_grad) {
+        CD_TRY(smooth_steps(n, n->ures, n->utmp, n->arch.loss_smooth, B, H, W));
+        hipLaunchKernelGGL(k_op_grad, dim3(nb), dim3(256), 0, c->stream, n->ures.as<float>(),
+                           n->gy.as<float>(), B, H, W, cs);
+        CD_CHECK_LAUNCH(c);
+        CD_TRY(head_grad(n, B, H, W));
+    }
+    return read_loss(n, nb, px, loss);
 }
 
 int prep(cd_net* n, const float* d_x, int B, int H, int W) {
@@ -752,9 +821,9 @@ int adam_step(cd_net* n, float lr) {
 int cd_net_forward_internal(cd_net* n, const float* d_x, float* d_y, int h, int w) {
     CD_TRY(prep(n, d_x, 1, h, w));
     CD_TRY(forward(n, 1, h, w));
-    hipLaunchKernelGGL(k_head_out, dim3(div_up((long long)h * w, 256)), dim3(256), 0,
-                       n->ctx->stream, n->head.as<float>(), d_y, (long long)h * w);
-    CD_CHECK_LAUNCH(n->ctx);
+    CD_TRY(finish_output(n, 1, h, w));
+    CD_HIP(n->ctx, hipMemcpyAsync(d_y, n->yout.ptr, sizeof(float) * (size_t)h * w,
+                                  hipMemcpyDeviceToDevice, n->ctx->stream));
     return CD_OK;
 }
 
@@ -783,12 +852,17 @@ void cd_net_arch_default(cd_net_arch* a) {
     a->smooth_sweeps = 4;
     a->smooth_omega = 1.2;
     a->linear = 1;
+    a->out_smooth = 0;
+    a->loss_smooth = 0;
 }
 
 int cd_net_create(cd_context* ctx, const cd_net_arch* arch, cd_net** out) {
     if (!ctx || !arch || !out) return CD_ERR_BAD_ARG;
     *out = nullptr;
     if (arch->levels < 2 || arch->levels > 6 || arch->coarsen < 1) return CD_ERR_BAD_ARG;
+    if (arch->out_smooth < 0 || arch->out_smooth > 64 || arch->loss_smooth < 0 ||
+        arch->loss_smooth > 64)
+        return CD_ERR_BAD_ARG;
     for (int l = 0; l < arch->levels; ++l)
         if (arch->channels[l] <= 0 || arch->channels[l] % 8 || arch->channels[l] > 128)
             return CD_ERR_BAD_ARG;
@@ -864,7 +938,9 @@ int cd_net_init_params(cd_net* n, unsigned long long seed) {
         const float sd = n->arch.linear ? std::sqrt(1.f / (3.f * fan_in))
                                         : std::sqrt(2.f / (float)fan_in);
         std::normal_distribution<float> nd(0.f, sd);
-        const float scale = (!n->arch.linear && cv.out < 0) ? 0.1f : 1.f;
+        // head: small for ReLU nets; zero for linear ones (the zero operator,
+        // operator loss exactly 1 at the start: no amplified initial noise)
+        const float scale = cv.out < 0 ? (n->arch.linear ? 0.f : 0.1f) : 1.f;
         for (int co = 0; co < cv.cout; ++co)
             for (int k = 0; k < cv.K; ++k) {
                 const int ci = k % (cv.c1 + cv.c2);
@@ -882,9 +958,9 @@ int cd_net_forward(cd_net* n, const float* d_x, float* d_y, int batch, int h, in
     CD_TRY(prep(n, d_x, batch, h, w));
     CD_TRY(forward(n, batch, h, w));
     const long long px = (long long)batch * h * w;
-    hipLaunchKernelGGL(k_head_out, dim3(div_up(px, 256)), dim3(256), 0, n->ctx->stream,
-                       n->head.as<float>(), d_y, px);
-    CD_CHECK_LAUNCH(n->ctx);
+    CD_TRY(finish_output(n, batch, h, w));
+    CD_HIP(n->ctx, hipMemcpyAsync(d_y, n->yout.ptr, sizeof(float) * (size_t)px,
+                                  hipMemcpyDeviceToDevice, n->ctx->stream));
     CD_HIP(n->ctx, hipStreamSynchronize(n->ctx->stream));
     return CD_OK;
 }

@@ -20,7 +20,7 @@ capi.Context of the process: torch ships its own HIP runtime with the same
 soname, and it must be the one loaded first or torch finds no device.
 Loading and using trained weights (load_net) needs no torch.
 
-    python -m poisson_caustic_design_amd.learned --mesh-res 512 --designs 16 \
+    python -m poisson_caustic_design_amd.learned --mesh-res 512 \
         --out poisson_caustic_design_amd/weights/learned_grid2048.npz
 """
 from __future__ import annotations
@@ -134,6 +134,8 @@ def train_operator(net: capi.Net, x, steps: int = 3000, batch: int = 16, lr: flo
         if step % 4 >= 2:
             xb = xb.flip(1)
         xb = xb.contiguous()
+        # the batch is written on torch's stream, read on the library's: wait
+        torch.cuda.synchronize()
         rate = lr * 0.5 * (1 + np.cos(np.pi * step / steps))  # cosine schedule
         losses.append(net.train_step_operator(xb.data_ptr(), batch, hc, wc, rate))
         if log and (step % log_every == 0 or step == steps - 1):
@@ -160,6 +162,7 @@ def train(net: capi.Net, x, t, steps: int = 2000, batch: int = 8, lr: float = 1e
             xb, tb = xb.flip(2).contiguous(), tb.flip(2).contiguous()
         if step % 4 >= 2:
             xb, tb = xb.flip(1).contiguous(), tb.flip(1).contiguous()
+        torch.cuda.synchronize()  # written on torch's stream, read on the library's
         rate = lr * 0.5 * (1 + np.cos(np.pi * step / steps))  # cosine schedule
         losses.append(net.train_step(xb.data_ptr(), tb.data_ptr(), batch, hc, wc, rate))
         if log and (step % log_every == 0 or step == steps - 1):
@@ -201,14 +204,16 @@ def save_params(net: capi.Net, path: str, meta: dict):
     np.savez_compressed(path, params=net.get_params(), levels=a.levels,
                         channels=np.array(list(a.channels)), coarsen=a.coarsen,
                         smooth_sweeps=a.smooth_sweeps, smooth_omega=a.smooth_omega,
-                        linear=a.linear, meta=json.dumps(meta))
+                        linear=a.linear, out_smooth=a.out_smooth, loss_smooth=a.loss_smooth,
+                        meta=json.dumps(meta))
 
 
 def load_net(ctx: capi.Context, path: str) -> capi.Net:
     d = np.load(path, allow_pickle=False)
     arch = capi.default_arch(levels=int(d["levels"]), channels=[int(c) for c in d["channels"]],
                              coarsen=int(d["coarsen"]), smooth_sweeps=int(d["smooth_sweeps"]),
-                             smooth_omega=float(d["smooth_omega"]), linear=int(d["linear"]))
+                             smooth_omega=float(d["smooth_omega"]), linear=int(d["linear"]),
+                             out_smooth=int(d["out_smooth"]), loss_smooth=int(d["loss_smooth"]))
     net = capi.Net(ctx, arch)
     net.set_params(d["params"])
     return net
@@ -222,14 +227,19 @@ def main(argv=None):
     ap.add_argument("--designs", type=int, default=16)
     ap.add_argument("--seed0", type=int, default=1000)
     ap.add_argument("--config", default="mixture")
-    ap.add_argument("--levels", type=int, default=5)
-    ap.add_argument("--channels", default="16,16,32,32,32")
+    ap.add_argument("--levels", type=int, default=4)
+    ap.add_argument("--channels", default="16,32,32,32")
     ap.add_argument("--relu", action="store_true")
-    ap.add_argument("--random", type=int, default=1000)
+    ap.add_argument("--out-smooth", type=int, default=4)
+    ap.add_argument("--loss-smooth", type=int, default=0)
+    ap.add_argument("--coarsen", type=int, default=16)
+    ap.add_argument("--sweeps", type=int, default=8, help="smoothing sweeps per FCG iteration")
+    ap.add_argument("--omega", type=float, default=1.5)
+    ap.add_argument("--random", type=int, default=600)
     ap.add_argument("--capacity", type=int, default=4000)
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--batch", type=int, default=8)
-    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--lr", type=float, default=5e-4)
     ap.add_argument("--rounds", type=int, default=1)
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
@@ -239,7 +249,9 @@ def main(argv=None):
 
     ctx = capi.Context(0)
     arch = capi.default_arch(levels=a.levels, channels=[int(c) for c in a.channels.split(",")],
-                             linear=0 if a.relu else 1)
+                             linear=0 if a.relu else 1, out_smooth=a.out_smooth,
+                             loss_smooth=a.loss_smooth, coarsen=a.coarsen,
+                             smooth_sweeps=a.sweeps, smooth_omega=a.omega)
     t0 = time.time()
     net, losses, pool = fit(ctx, arch, a.mesh_res, range(a.seed0, a.seed0 + a.designs), a.config,
                             a.random, a.capacity, a.steps, a.batch, a.lr, a.rounds, a.seed0,

@@ -27,6 +27,13 @@ int main(int argc, char** argv) {
                            0.5 * std::exp(-((u - 0.7) * (u - 0.7) + (v - 0.6) * (v - 0.6)) / 0.02);
         }
 
+    {   // the exact input, for the comparison made by the test
+        FILE* f = std::fopen((output_path + "image.f64").c_str(), "wb");
+        if (!f) return 1;
+        for (auto& row : pixels) std::fwrite(row.data(), sizeof(double), row.size(), f);
+        std::fclose(f);
+    }
+
     Caustic_design caustic_design;
     caustic_design.set_mesh_resolution(mesh_resolution, mesh_resolution);
     caustic_design.set_domain_resolution(N, N);

@@ -25,9 +25,9 @@ def dev(a):
 @pytest.mark.parametrize("linear", [1, 0])
 def test_net_forward_matches_netref(ctx, linear):
     net = capi.Net(ctx, small_arch(linear))
-    net.init_params(5)
-    p = net.get_params()
     ref = NetRef(3, (8, 16, 16), linear=bool(linear))
+    net.set_params(ref.random_params(5))
+    p = net.get_params()
     assert ref.nparam == net.n
     x = np.random.default_rng(0).standard_normal((2, 16, 24))
     dx, dy = dev(x), torch.zeros((2, 16, 24), device="cuda:0")
@@ -41,9 +41,9 @@ def test_net_forward_matches_netref(ctx, linear):
 @pytest.mark.parametrize("linear", [1, 0])
 def test_net_gradient_matches_netref(ctx, linear):
     net = capi.Net(ctx, small_arch(linear))
-    net.init_params(7)
-    p = net.get_params()
     ref = NetRef(3, (8, 16, 16), linear=bool(linear))
+    net.set_params(ref.random_params(7))
+    p = net.get_params()
     rng = np.random.default_rng(1)
     x = rng.standard_normal((2, 16, 16))
     t = 0.3 * rng.standard_normal((2, 16, 16))
@@ -57,12 +57,13 @@ def test_net_gradient_matches_netref(ctx, linear):
     assert cos > 0.998
 
 
-@pytest.mark.parametrize("linear", [1, 0])
-def test_net_operator_gradient_matches_netref(ctx, linear):
-    net = capi.Net(ctx, small_arch(linear))
-    net.init_params(9)
+@pytest.mark.parametrize("linear,out_smooth,loss_smooth", [(1, 0, 0), (0, 0, 0), (1, 2, 3)])
+def test_net_operator_gradient_matches_netref(ctx, linear, out_smooth, loss_smooth):
+    net = capi.Net(ctx, small_arch(linear, out_smooth=out_smooth, loss_smooth=loss_smooth))
+    ref = NetRef(3, (8, 16, 16), linear=bool(linear), out_smooth=out_smooth,
+                 loss_smooth=loss_smooth)
+    net.set_params(ref.random_params(9))
     p = net.get_params()
-    ref = NetRef(3, (8, 16, 16), linear=bool(linear))
     rng = np.random.default_rng(5)
     x = rng.standard_normal((2, 16, 24))
     dx = dev(x)
@@ -86,7 +87,9 @@ def test_operator_training_learns_coarse_inverse(ctx):
     net.init_params(2)
     torch.cuda.synchronize()
     losses = learned.train_operator(net, dx, steps=3000, batch=16, lr=2e-3, log=None)
-    assert np.mean(losses[-50:]) < 0.02 * np.mean(losses[:10])
+    # the zero operator (the initial network) has loss mean(x^2) = 1
+    assert losses[0] == pytest.approx(1.0, rel=1e-3)
+    assert np.mean(losses[-50:]) < 0.1
     rng = np.random.default_rng(7)
     xt = smooth_rhs(rng, 32, 32)
     xt /= np.sqrt(np.mean(xt ** 2))
@@ -105,7 +108,7 @@ def test_operator_training_learns_coarse_inverse(ctx):
 
 def test_default_arch_forward_and_step(ctx):
     net = capi.Net(ctx)  # 4 levels, 16-64 channels
-    net.init_params(0)
+    net.set_params(NetRef(linear=True).random_params(0))
     x = dev(np.random.default_rng(2).standard_normal((2, 64, 64)))
     t = dev(np.zeros((2, 64, 64)))
     torch.cuda.synchronize()
@@ -135,7 +138,7 @@ def test_training_reduces_loss(ctx):
 def test_learned_solve_with_untrained_net_still_converges(ctx):
     """A prediction that raises the residual is rejected and plain SOR finishes."""
     net = capi.Net(ctx)
-    net.init_params(11)
+    net.set_params(NetRef(linear=True).random_params(11, head_scale=3.0))  # garbage
     ctx.set_net(net)
     try:
         b = smooth_rhs(np.random.default_rng(4), 256, 256)

@@ -1,6 +1,7 @@
 """Pins the FP64 network restatement (oracle/netref.py) by central finite
 differences; it is then the checker of the GPU network kernels."""
 import numpy as np
+import pytest
 
 from oracle.netref import NetRef, bf16
 
@@ -21,6 +22,26 @@ def test_netref_gradient_finite_differences():
         pm[i] -= eps
         fd = (net.loss_grad(pp, x, t)[0] - net.loss_grad(pm, x, t)[0]) / (2 * eps)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (i, fd, g[i])
+
+
+def test_smoothing_step_closed_form():
+    from tests.helpers import eigvec
+    e, lam = eigvec(12, 10, 2, 3)
+    # S = I + Lc/8 scales an eigenvector by 1 + lambda/8 (lambda <= 0)
+    assert np.allclose(NetRef.smooth(e[None], 3)[0], (1 + lam / 8.0) ** 3 * e, atol=1e-13)
+    c = np.full((1, 5, 7), 2.5)
+    assert np.array_equal(NetRef.smooth(c, 4), c)  # constants pass unchanged
+
+
+def test_linear_init_is_the_zero_operator():
+    net = NetRef(levels=3, channels=(8, 16, 16), linear=True)
+    p = net.init_params(1).astype(np.float64)
+    x = np.random.default_rng(2).standard_normal((1, 16, 16))
+    y, _ = net.forward(p, x)
+    assert np.all(y == 0)
+    loss, g = net.loss_grad_operator(p, x)
+    assert loss == pytest.approx(np.mean(x ** 2))  # u = -x
+    assert np.abs(g).max() > 0  # the head still receives a gradient
 
 
 def test_netref_shapes_and_layout():
@@ -44,10 +65,12 @@ def test_bf16_rounding():
     assert r[2] == 1.0 and r[3] == 1.0 + 2 ** -7 and r[4] == -2.5
 
 
-def test_netref_operator_gradient_finite_differences():
+@pytest.mark.parametrize("out_smooth,loss_smooth", [(0, 0), (2, 3)])
+def test_netref_operator_gradient_finite_differences(out_smooth, loss_smooth):
     for linear in (True, False):
-        net = NetRef(levels=3, channels=(8, 16, 16), linear=linear)
-        p = net.init_params(4).astype(np.float64)
+        net = NetRef(levels=3, channels=(8, 16, 16), linear=linear, out_smooth=out_smooth,
+                     loss_smooth=loss_smooth)
+        p = net.random_params(4).astype(np.float64)
         rng = np.random.default_rng(2)
         x = rng.standard_normal((2, 8, 8))
         loss, g = net.loss_grad_operator(p, x)
@@ -79,7 +102,7 @@ def test_netref_operator_matches_independent_stencil_and_exact_inverse():
 
 def test_linear_netref_is_linear():
     net = NetRef(levels=3, channels=(8, 16, 16), linear=True)
-    p = net.init_params(5).astype(np.float64)
+    p = net.random_params(5).astype(np.float64)
     rng = np.random.default_rng(4)
     x1, x2 = rng.standard_normal((2, 1, 16, 16))
     y1, _ = net.forward(p, x1)
