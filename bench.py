@@ -325,7 +325,17 @@ def main(argv=None):
     wpath = a.weights or os.path.join(ROOT, "poisson_caustic_design_amd", "weights",
                                       f"learned_grid{N}.npz")
     if a.learned == "auto" and os.path.exists(wpath):
+        plain_final = {k: cd.get(k) for k in ("target_points", "source_points")}
         out["learned"] = learned_leg(ctx, cd, one_design, wpath, a, value)
+        # equal final error: the two forms' final designs against each other
+        tp, sp = cd.get("target_points"), cd.get("source_points")
+        zr = np.abs(plain_final["source_points"][:, 2]).max()
+        out["learned"]["vs_plain_final_design"] = {
+            "max_abs_target_xy_over_width": float(
+                np.abs(tp[:, :2] - plain_final["target_points"][:, :2]).max() / W),
+            "max_abs_height_over_max_height": float(
+                np.abs(sp[:, 2] - plain_final["source_points"][:, 2]).max() / max(zr, 1e-300)),
+        }
     cb = None
     if not a.no_cpu_baseline:
         cb = cpu_baseline(a.config, a.seed, per_design, a.cpu_timeout)

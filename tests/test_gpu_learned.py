@@ -189,3 +189,30 @@ def test_trained_network_gains_over_plain_sor(ctx):
     print("plain vs learned (sweeps, learned iterations, worst residual):", res)
     assert res[0][2] <= 1e-6 and res[1][2] <= 1e-6
     assert res[1][0] < 0.25 * res[0][0]
+
+
+def test_shipped_weights_speed_up_grid_2048(ctx):
+    """The committed network (weights/learned_grid2048.npz, trained on other
+    seeds) on a held-out 2048^2 problem: same 1e-6 residual, no fallback,
+    a fraction of the sweeps and of the time of plain SOR."""
+    import os
+    w = os.path.join(os.path.dirname(capi.LIB_PATH), "weights", "learned_grid2048.npz")
+    if not os.path.exists(w):
+        pytest.skip("no shipped weights")
+    net = learned.load_net(ctx, w)
+    img, _ = synth.target_for("mixture", 4242, 512)
+    b = img / img.mean() - 1.0
+    _, st0 = ctx.poisson_solve(b, np.zeros_like(b), capi.default_opts(tol=1e-6))
+    ctx.set_net(net)
+    try:
+        for _ in range(2):  # the second run is timed warm
+            phi, st1 = ctx.poisson_solve(b, np.zeros_like(b),
+                                         capi.default_opts(tol=1e-6, learned_init=1))
+    finally:
+        ctx.set_net(None)
+    print(f"plain {st0.sweeps} sweeps {st0.elapsed_ms:.1f} ms; learned {st1.sweeps} sweeps "
+          f"{st1.learned_applied} iterations {st1.elapsed_ms:.1f} ms")
+    assert st0.rel_residual <= 1e-6 and st1.rel_residual <= 1e-6
+    assert st1.learned_rejected == 0
+    assert st1.sweeps < 0.1 * st0.sweeps
+    assert st1.elapsed_ms < 0.5 * st0.elapsed_ms
