@@ -305,6 +305,10 @@ int cd_net_loss_grad_operator(cd_net* net, const float* d_x, int batch, int h, i
 /* One Adam step on the operator loss; *loss is the loss before the step. */
 int cd_net_train_step_operator(cd_net* net, const float* d_x, int batch, int h, int w, float lr,
                                double* loss);
+/* Learned FCG iterations replay their kernel sequence (restriction, network,
+ * prolongation, smoothing sweeps, FCG update) as one captured HIP graph
+ * (default on; off: the same kernels launched one by one, identical results). */
+int cd_context_set_graphs(cd_context* ctx, int on);
 /* Attach (or detach with NULL) the network used by solves with learned_init. */
 int cd_context_set_net(cd_context* ctx, cd_net* net);
 

@@ -124,6 +124,7 @@ def load(path: str = LIB_PATH):
         "cd_design_device_ptr": ([_P, _I], _P),
         "cd_design_last_solve_stats": ([_P, C.POINTER(SolveStats), C.POINTER(SolveStats)], _I),
         "cd_design_get_params": ([_P, C.POINTER(DesignParams)], _I),
+        "cd_context_set_graphs": ([_P, _I], _I),
         "cd_design_laplacian_smoothing": ([_P, _D], _I),
         "cd_design_set_miss_recovery": ([_P, _I], _I),
         "cd_png_read_rgba8": ([C.c_char_p, C.POINTER(_I), C.POINTER(_I), _P, _SZ], _I),
@@ -175,6 +176,7 @@ def exported_symbols():
         "cd_net_train_step", "cd_net_loss_grad_operator", "cd_net_train_step_operator",
         "cd_context_set_net", "cd_context_set_capture",
         "cd_context_capture_count", "cd_design_get_params", "cd_png_read_rgba8",
+        "cd_context_set_graphs",
         "cd_image_gray_resize", "cd_image_gray_resize_device", "cd_obj_write_solid",
         "cd_json_write_heightmap", "cd_svg_write_grid", "cd_svg_write_triangles",
         "cd_design_save_solid_obj", "cd_design_save_heightmap_json", "cd_design_export_svg",
@@ -269,6 +271,10 @@ class Context:
                                      C.byref(opts or default_opts()), C.byref(st))
         self.check(rc, "cd_poisson_solve", allow)
         return phi, st
+
+    def set_graphs(self, on: bool):
+        """Replay learned FCG iterations as one captured HIP graph (default on)."""
+        self.check(self.L.cd_context_set_graphs(self.h, int(on)), "set_graphs")
 
     def set_net(self, net: Optional["Net"]):
         self.check(self.L.cd_context_set_net(self.h, net.h if net else None), "set_net")

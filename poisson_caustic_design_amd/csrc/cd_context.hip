@@ -66,6 +66,7 @@ int cd_context::ensure(DevBuf& b, size_t bytes) {
     }
     b.ptr = p;
     b.bytes = bytes;
+    ++gen;  // device addresses changed: captured graphs are stale
     return CD_OK;
 }
 
@@ -168,11 +169,18 @@ int cd_context_create(int device, cd_context** out) {
     return CD_OK;
 }
 
+int cd_context_set_graphs(cd_context* ctx, int on) {
+    if (!ctx) return CD_ERR_BAD_ARG;
+    ctx->graphs = on != 0;
+    return CD_OK;
+}
+
 void cd_context_destroy(cd_context* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (cd_net* n : std::vector<cd_net*>(ctx->nets)) cd_net_detach(n);
+    if (ctx->fcg_exec) (void)hipGraphExecDestroy(ctx->fcg_exec);
     for (cd_design* d : std::vector<cd_design*>(ctx->designs)) cd_design_detach(d);
     ctx->e32.release();
     ctx->e32b.release();

@@ -96,6 +96,21 @@ struct cd_context {
     // detaches them (their later calls return CD_ERR_STATE)
     std::vector<cd_net*> nets;
     std::vector<cd_design*> designs;
+    // bumped whenever ensure() moves a buffer (captured graphs hold addresses)
+    unsigned long long gen = 0;
+    // learned FCG iteration body captured as a HIP graph, and what it was
+    // captured for
+    bool graphs = true;
+    hipGraphExec_t fcg_exec = nullptr;
+    struct FcgKey {
+        int w = 0, h = 0, nsmooth = 0;
+        const void* net = nullptr;
+        unsigned long long cgen = 0, ngen = 0;
+        bool operator==(const FcgKey& o) const {
+            return w == o.w && h == o.h && nsmooth == o.nsmooth && net == o.net &&
+                   cgen == o.cgen && ngen == o.ngen;
+        }
+    } fcg_key;
     hipStream_t stream = nullptr;
     std::string last_error;
     bool profiling = false;
@@ -191,3 +206,5 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
 // Learned network forward on one coarse field (cd_learned.hip).
 int cd_net_forward_internal(cd_net* n, const float* d_x, float* d_y, int h, int w);
 const cd_net_arch* cd_net_arch_of(const cd_net* n);
+// bumped whenever the net's activation buffers move (batch shape changes)
+unsigned long long cd_net_gen(const cd_net* n);

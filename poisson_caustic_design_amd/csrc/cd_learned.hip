@@ -12,6 +12,8 @@
 //   D (16x16): lane l holds D[4*(l/16) + r][l % 16], r = 0..3
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -437,6 +439,8 @@ struct cd_net {
     size_t nparam = 0;
     DevBuf w, m, v, gw, wbf, wtbf, partials, head, ghead, x8;
     DevBuf yout, ytmp, gy, ures, utmp;  // FP32 [B][H][W]: output, dL/dy, loss work
+    unsigned long long gen = 0;          // bumped when the activation buffers move
+    unsigned id = 0;                     // unique per net created in the process
     size_t wtoff_total = 0;
     std::vector<size_t> wtoff;
     long long step = 0;
@@ -599,6 +603,7 @@ int ensure_batch(cd_net* n, int B, int H, int W) {
         return CD_ERR_BAD_ARG;
     }
     if (n->batch == B && n->bh == H && n->bw == W) return CD_OK;
+    ++n->gen;
     for (auto& t : n->t) {
         const long long px = (long long)B * (H >> t.level) * (W >> t.level);
         CD_TRY(c->ensure(t.act, sizeof(u16) * px * t.C));
@@ -828,6 +833,9 @@ int cd_net_forward_internal(cd_net* n, const float* d_x, float* d_y, int h, int 
 }
 
 const cd_net_arch* cd_net_arch_of(const cd_net* n) { return &n->arch; }
+unsigned long long cd_net_gen(const cd_net* n) {
+    return ((unsigned long long)n->id << 32) | (n->gen & 0xffffffffULL);
+}
 
 extern "C" void cd_net_detach(cd_net* n) {
     cd_context* c = n->ctx;
@@ -884,6 +892,8 @@ int cd_net_create(cd_context* ctx, const cd_net_arch* arch, cd_net** out) {
         delete n;
         return st;
     }
+    static std::atomic<unsigned> next_id{0};
+    n->id = ++next_id;
     ctx->nets.push_back(n);
     *out = n;
     return CD_OK;

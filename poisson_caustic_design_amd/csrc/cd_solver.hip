@@ -625,24 +625,38 @@ __device__ __forceinline__ size_t brick_at(const Brick& bk, int x, int y) {
 }
 
 // coarse residual rc = block mean of the projected BF16 residual the blocked
-// sweeps see (r16 - rmean32); partials (sum, sum^2)
+// sweeps see (r16 - rmean32); partials (sum, sum^2) per workgroup.  A
+// workgroup of 256 threads covers `per` = max(1, 256 / f^2) coarse cells, one
+// thread per fine cell (f^2 > 256: each thread sums f^2 / 256 of them).
 __global__ __launch_bounds__(256) void k_restrict_r16(const unsigned short* __restrict__ r16,
                                                       Brick bk, const SolveDev* __restrict__ st,
                                                       int f, int wc, int hc,
                                                       float* __restrict__ rc,
                                                       double* __restrict__ partials) {
+    __shared__ float part[256];
     __shared__ double smem[16];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ff = f * f;
+    const int per = ff >= 256 ? 1 : 256 / ff;
+    const int tpc = ff >= 256 ? 256 : ff;  // threads per coarse cell
+    const int local = threadIdx.x / tpc, k0 = threadIdx.x - local * tpc;
+    const int cell = blockIdx.x * per + local;
+    const float rm = st->rmean32;
+    float acc = 0.f;
+    if (local < per && cell < wc * hc) {
+        const int J = cell / wc, I = cell - J * wc;
+        for (int k = k0; k < ff; k += tpc) {
+            const int yy = J * f + k / f, xx = I * f + (k - (k / f) * f);
+            acc += bf16_to_f32(r16[brick_at(bk, xx, yy)]) - rm;
+        }
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
     double v = 0.0;
-    if (i < wc * hc) {
-        const int J = i / wc, I = i - J * wc;
-        const float rm = st->rmean32;
-        float acc = 0.f;
-        for (int yy = J * f; yy < J * f + f; ++yy)
-            for (int xx = I * f; xx < I * f + f; ++xx)
-                acc += bf16_to_f32(r16[brick_at(bk, xx, yy)]) - rm;
-        v = (double)acc / (double)(f * f);
-        rc[i] = (float)v;
+    if (threadIdx.x < per && blockIdx.x * per + threadIdx.x < wc * hc) {
+        float t = 0.f;
+        for (int i = 0; i < tpc; ++i) t += part[threadIdx.x * tpc + i];  // fixed order
+        v = (double)t / (double)ff;
+        rc[blockIdx.x * per + threadIdx.x] = (float)v;
     }
     const double a = cd_block_sum(v, smem);
     const double q = cd_block_sum(v * v, smem);
@@ -663,16 +677,30 @@ __global__ void k_normalize_coarse(const float* __restrict__ rc, const double* _
     x[i] = (float)(((double)rc[i] - mean) / sig);
 }
 
+// packed-layout walk: element g of the two colour planes -> (plane c, row Y,
+// packed column j, cell x); false for padding
+__device__ __forceinline__ bool packed_cell(long long g, const Brick& bk, int w, int h, int& c,
+                                            int& Y, int& j, int& x) {
+    const long long per_plane = (long long)h * bk.wpb;
+    c = (int)(g / per_plane);
+    const long long r = g - (long long)c * per_plane;
+    Y = (int)(r / bk.wpb);
+    j = (int)(r - (long long)Y * bk.wpb);
+    x = 2 * j + ((c + Y) & 1);
+    return c < 2 && x < w;
+}
+
 // e = rms * (nmax/pi)^2 * bilinear(y) at cell centres, written in the
-// bricked layout of the blocked sweeps (the start of the smoothing)
+// bricked layout of the blocked sweeps (the start of the smoothing); one
+// thread per packed element, coalesced writes
 __global__ __launch_bounds__(256) void k_prolong_bricked(const float* __restrict__ y,
                                                          const double* __restrict__ stats, int nc,
                                                          int w, int h, int f, int wc, int hc,
                                                          double nscale, Brick bk,
                                                          float* __restrict__ e) {
-    const int xx = blockIdx.x * blockDim.x + threadIdx.x;
-    const int yy = blockIdx.y;
-    if (xx >= w || yy >= h) return;
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int c, yy, j, xx;
+    if (!packed_cell(g, bk, w, h, c, yy, j, xx)) return;
     const double mean = stats[0] / nc;
     const double var = stats[1] / nc - mean * mean;
     const double sig = sqrt(var > 1e-300 ? var : 1e-300);
@@ -684,7 +712,7 @@ __global__ __launch_bounds__(256) void k_prolong_bricked(const float* __restrict
     j0 = min(max(j0, 0), hc - 1);
     const float ev = (y[j0 * wc + i0] * (1.f - tu) + y[j0 * wc + i1] * tu) * (1.f - tv) +
                      (y[j1 * wc + i0] * (1.f - tu) + y[j1 * wc + i1] * tu) * tv;
-    e[brick_at(bk, xx, yy)] = (float)(sig * nscale * (double)ev);
+    e[brick_row(bk, c, yy) + brick_col(j)] = (float)(sig * nscale * (double)ev);
 }
 
 // ---------------------------------------------------------------------------
@@ -704,17 +732,21 @@ struct FcgDev {
     int bad;       // the last step had no descent direction
 };
 
-__device__ __forceinline__ float lap_bricked(const float* __restrict__ e, const Brick& bk, int x,
-                                             int y, int w, int h, float c) {
+// Lz at packed element (c, Y, j) (cell x): the four neighbours lie in the
+// other colour plane, up/down at column j, left/right at (x -+ 1) >> 1
+__device__ __forceinline__ float lap_packed(const float* __restrict__ z, const Brick& bk, int c,
+                                            int Y, int j, int x, int w, int h, float zc) {
+    const int o = 1 - c;
     float s = 0.f, cnt = 0.f;
-    if (x > 0) { s += e[brick_at(bk, x - 1, y)]; cnt += 1.f; }
-    if (y > 0) { s += e[brick_at(bk, x, y - 1)]; cnt += 1.f; }
-    if (x < w - 1) { s += e[brick_at(bk, x + 1, y)]; cnt += 1.f; }
-    if (y < h - 1) { s += e[brick_at(bk, x, y + 1)]; cnt += 1.f; }
-    return s - cnt * c;
+    if (x > 0) { s += z[brick_row(bk, o, Y) + brick_col((x - 1) >> 1)]; cnt += 1.f; }
+    if (x < w - 1) { s += z[brick_row(bk, o, Y) + brick_col((x + 1) >> 1)]; cnt += 1.f; }
+    if (Y > 0) { s += z[brick_row(bk, o, Y - 1) + brick_col(j)]; cnt += 1.f; }
+    if (Y < h - 1) { s += z[brick_row(bk, o, Y + 1) + brick_col(j)]; cnt += 1.f; }
+    return s - cnt * zc;
 }
 
-// partials[4 * block + k]: z.q_old, z.r, p_old.r, z.Lz
+// partials[4 * block + k]: z.q_old, z.r, p_old.r, z.Lz over the packed
+// elements the workgroup strides through (fixed order: deterministic)
 __global__ __launch_bounds__(256) void k_fcg_dots(const float* __restrict__ z,
                                                   const float* __restrict__ p,
                                                   const float* __restrict__ q,
@@ -722,29 +754,31 @@ __global__ __launch_bounds__(256) void k_fcg_dots(const float* __restrict__ z,
                                                   const SolveDev* __restrict__ st, Brick bk,
                                                   int w, int h, double* __restrict__ partials) {
     __shared__ double smem[16];
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const float rm = st->rmean32;
+    const long long total = 2LL * h * bk.wpb;
     double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
-    if (x < w && y < h) {
-        const size_t i = brick_at(bk, x, y);
+    for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+         g += (long long)gridDim.x * blockDim.x) {
+        int c, Y, j, x;
+        if (!packed_cell(g, bk, w, h, c, Y, j, x)) continue;
+        const size_t i = brick_row(bk, c, Y) + brick_col(j);
         const float zc = z[i];
-        const float lz = lap_bricked(z, bk, x, y, w, h, zc);
-        const float rr = bf16_to_f32(r16[i]) - st->rmean32;
-        d0 = (double)zc * q[i];
-        d1 = (double)zc * rr;
-        d2 = (double)p[i] * rr;
-        d3 = (double)zc * lz;
+        const float lz = lap_packed(z, bk, c, Y, j, x, w, h, zc);
+        const float rr = bf16_to_f32(r16[i]) - rm;
+        d0 += (double)(zc * q[i]);
+        d1 += (double)(zc * rr);
+        d2 += (double)(p[i] * rr);
+        d3 += (double)(zc * lz);
     }
     d0 = cd_block_sum(d0, smem);
     d1 = cd_block_sum(d1, smem);
     d2 = cd_block_sum(d2, smem);
     d3 = cd_block_sum(d3, smem);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-        const size_t bid = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
-        partials[4 * bid] = d0;
-        partials[4 * bid + 1] = d1;
-        partials[4 * bid + 2] = d2;
-        partials[4 * bid + 3] = d3;
+    if (threadIdx.x == 0) {
+        partials[4 * blockIdx.x] = d0;
+        partials[4 * blockIdx.x + 1] = d1;
+        partials[4 * blockIdx.x + 2] = d2;
+        partials[4 * blockIdx.x + 3] = d3;
     }
 }
 
@@ -779,20 +813,20 @@ __global__ void k_fcg_coef(const double* __restrict__ partials, int nblocks, Fcg
 }
 
 // p = z + beta p; q = Lz + beta q; eo = alpha p (the correction the next check
-// folds into phi64); st->cur names eo's buffer
+// folds into phi64); st->cur names eo's buffer.  One thread per packed element.
 __global__ __launch_bounds__(256) void k_fcg_update(const float* __restrict__ z,
                                                     float* __restrict__ p, float* __restrict__ q,
                                                     float* __restrict__ eo,
                                                     const FcgDev* __restrict__ fd, Brick bk,
                                                     int w, int h, SolveDev* st, int eo_index) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (x == 0 && y == 0) st->cur = eo_index;
-    if (x >= w || y >= h) return;
-    const size_t i = brick_at(bk, x, y);
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g == 0) st->cur = eo_index;
+    int c, Y, j, x;
+    if (!packed_cell(g, bk, w, h, c, Y, j, x)) return;
+    const size_t i = brick_row(bk, c, Y) + brick_col(j);
     const float beta = (float)fd->beta, alpha = (float)fd->alpha;
     const float zc = z[i];
-    const float lz = lap_bricked(z, bk, x, y, w, h, zc);
+    const float lz = lap_packed(z, bk, c, Y, j, x, w, h, zc);
     const float pn = zc + beta * p[i];
     p[i] = pn;
     q[i] = lz + beta * q[i];
@@ -1003,7 +1037,8 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     // xout; stats slot receives (sum, sum^2) of the coarse residual
     auto restrict_normalize = [&](int f, int wc, int hc, float* xout, double* stats) -> int {
         const int nc = wc * hc;
-        const int nb = div_up(nc, 256);  // <= nred: the partials buffer is large enough
+        const int per = f * f >= 256 ? 1 : 256 / (f * f);  // coarse cells per workgroup
+        const int nb = div_up(nc, per);  // <= nred: the partials buffer is large enough
         hipLaunchKernelGGL(k_restrict_r16, dim3(nb), dim3(256), 0, s, r16, bk, sd, f, wc, hc,
                            ctx->lc_r.as<float>(), partials);
         CD_CHECK_LAUNCH(ctx);
@@ -1032,6 +1067,10 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     // the smoothing launches start from e32; after `nl` launches the result
     // sits in e32 (nl even) or e32b (nl odd)
     const int smooth_launches = div_up(nsmooth, BH / 2);
+    // packed-element kernels: one thread per element of the two colour planes
+    // (rows x padded packed width); dot products stride over a capped grid
+    const int packed_blocks = (int)div_up(2LL * h * bk.wpb, 256);
+    const int dot_blocks = std::min(packed_blocks, std::min(nred, 2048));
     float* zbuf = (smooth_launches & 1) ? e32b : e32;
     float* obuf = (smooth_launches & 1) ? e32 : e32b;
     const int obuf_index = (smooth_launches & 1) ? 0 : 1;
@@ -1056,33 +1095,71 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     // falling (the iterate is kept: every step lowered the error's energy norm)
     double best_rel = ctx->host_status->rel;
     int stall = 0;
-    while (!done && learned && cum < o->max_sweeps) {
+    // one FCG iteration up to the next check: restriction, network, prolongation,
+    // smoothing sweeps, FCG update.  Launched eagerly the first time in a solve
+    // (which also sizes every buffer), then replayed as one captured graph.
+    auto fcg_body = [&](int first_sweep) -> int {
         FcgDev* fd = ctx->fcg_dev.as<FcgDev>();
         CD_TRY(ctx->timer_begin());
         double* stats = ctx->lc_stats.as<double>();
         CD_TRY(restrict_normalize(lf, lwc, lhc, ctx->lc_x.as<float>(), stats));
         CD_TRY(cd_net_forward_internal(ctx->net, ctx->lc_x.as<float>(), ctx->lc_y.as<float>(), lhc,
                                        lwc));
-        hipLaunchKernelGGL(k_prolong_bricked, dim3(div_up(w, 256), h), dim3(256), 0, s,
+        hipLaunchKernelGGL(k_prolong_bricked, dim3(packed_blocks), dim3(256), 0, s,
                            ctx->lc_y.as<float>(), stats, lwc * lhc, w, h, lf, lwc, lhc, nscale, bk,
                            e32);
         CD_CHECK_LAUNCH(ctx);
         CD_TRY(ctx->timer_end(CD_KERNEL_LEARNED, 4, 0.0));
-        const int nsw = std::min(nsmooth, o->max_sweeps - cum);
-        CD_TRY(enqueue_blocked_om(cum, nsw, oms / 2.f, oms / 3.f, oms / 4.f));
-        cum += nsw;
-        if (div_up(nsw, BH / 2) != smooth_launches) { learned = false; break; }  // sweep cap hit
+        CD_TRY(enqueue_blocked_om(first_sweep, nsmooth, oms / 2.f, oms / 3.f, oms / 4.f));
         CD_TRY(ctx->timer_begin());
-        hipLaunchKernelGGL(k_fcg_dots, rgrid, rblk, 0, s, zbuf, ctx->fcg_p.as<float>(),
-                           ctx->fcg_q.as<float>(), r16, sd, bk, w, h, partials);
-        hipLaunchKernelGGL(k_fcg_coef, dim3(1), dim3(1024), 0, s, partials, nred, fd);
-        hipLaunchKernelGGL(k_fcg_update, rgrid, rblk, 0, s, zbuf, ctx->fcg_p.as<float>(),
-                           ctx->fcg_q.as<float>(), obuf, fd, bk, w, h, sd, obuf_index);
+        hipLaunchKernelGGL(k_fcg_dots, dim3(dot_blocks), dim3(256), 0, s, zbuf,
+                           ctx->fcg_p.as<float>(), ctx->fcg_q.as<float>(), r16, sd, bk, w, h,
+                           partials);
+        hipLaunchKernelGGL(k_fcg_coef, dim3(1), dim3(1024), 0, s, partials, dot_blocks, fd);
+        hipLaunchKernelGGL(k_fcg_update, dim3(packed_blocks), dim3(256), 0, s, zbuf,
+                           ctx->fcg_p.as<float>(), ctx->fcg_q.as<float>(), obuf, fd, bk, w, h,
+                           sd, obuf_index);
         CD_CHECK_LAUNCH(ctx);
         CD_TRY(ctx->timer_end(CD_KERNEL_LEARNED, 3, 26.0 * (double)n));
+        return CD_OK;
+    };
+    int fcg_iter = 0;
+    while (!done && learned && cum < o->max_sweeps) {
+        if (o->max_sweeps - cum < nsmooth) { learned = false; break; }  // a cycle no longer fits
+        // profiling keeps the launches separate so that every timer records
+        const bool replay = ctx->graphs && !ctx->profiling && fcg_iter > 0;
+        if (replay) {
+            const cd_context::FcgKey key{w, h, nsmooth, ctx->net, ctx->gen, cd_net_gen(ctx->net)};
+            if (!(ctx->fcg_exec && ctx->fcg_key == key)) {
+                if (ctx->fcg_exec) {
+                    CD_HIP(ctx, hipGraphExecDestroy(ctx->fcg_exec));
+                    ctx->fcg_exec = nullptr;
+                }
+                hipGraph_t graph = nullptr;
+                CD_HIP(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+                const int body = fcg_body(cum);
+                const hipError_t ce = hipStreamEndCapture(s, &graph);
+                if (body != CD_OK) {
+                    if (graph) (void)hipGraphDestroy(graph);
+                    return body;
+                }
+                CD_HIP(ctx, ce);
+                const hipError_t ie = hipGraphInstantiate(&ctx->fcg_exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                CD_HIP(ctx, ie);
+                ctx->fcg_key = key;
+            }
+            CD_HIP(ctx, hipGraphLaunch(ctx->fcg_exec, s));
+        } else {
+            CD_TRY(fcg_body(cum));
+        }
+        ++fcg_iter;
+        cum += nsmooth;
         CD_TRY(enqueue_check(++check_id));
         cum_at_check.push_back(cum);
         ++learned_applied;
+        FcgDev* fd = ctx->fcg_dev.as<FcgDev>();
+        (void)fd;
         const SolveDev& hs = *ctx->host_status;
         done = hs.done != 0;
         if (!std::isfinite(hs.rel)) break;

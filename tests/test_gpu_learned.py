@@ -216,3 +216,33 @@ def test_shipped_weights_speed_up_grid_2048(ctx):
     assert st1.learned_rejected == 0
     assert st1.sweeps < 0.1 * st0.sweeps
     assert st1.elapsed_ms < 0.5 * st0.elapsed_ms
+
+
+def test_graph_replay_matches_eager_launches(ctx):
+    """The captured FCG iteration, replayed across iterations and solves with
+    eager work and synchronisations in between, gives bit-identical iterates
+    and statistics to launching the same kernels one by one."""
+    import os
+    w = os.path.join(os.path.dirname(capi.LIB_PATH), "weights", "learned_grid2048.npz")
+    if not os.path.exists(w):
+        pytest.skip("no shipped weights")
+    net = learned.load_net(ctx, w)
+    rhs = []
+    for seed in (11, 12):
+        img, _ = synth.target_for("mixture", seed, 512)
+        rhs.append(img / img.mean() - 1.0)
+    o = capi.default_opts(tol=1e-6, learned_init=1)
+    ctx.set_net(net)
+    try:
+        ctx.set_graphs(False)
+        eager = [ctx.poisson_solve(b, np.zeros_like(b), o) for b in rhs]
+        ctx.set_graphs(True)
+        graph = [ctx.poisson_solve(b, np.zeros_like(b), o) for b in rhs + rhs[:1]]
+    finally:
+        ctx.set_graphs(True)
+        ctx.set_net(None)
+    for (pe, se), (pg, sg) in zip(eager + eager[:1], graph):
+        assert se.learned_applied > 2 and se.rel_residual <= 1e-6
+        assert np.array_equal(pe, pg)
+        assert (se.sweeps, se.learned_applied, se.rel_residual) == \
+            (sg.sweeps, sg.learned_applied, sg.rel_residual)
