@@ -199,6 +199,66 @@ def fit(ctx: capi.Context, arch: capi.NetArch, mesh_res: int, seeds, config: str
     return net, np.concatenate(losses), pool
 
 
+def evaluate(ctx: capi.Context, net: capi.Net, config: str, seeds, mesh_res: int,
+             tol: float = 1e-6, log=print, control: bool = False) -> dict:
+    """Full designs on held-out seeds, plain SOR and learned form on the same
+    GPU at the same residual tolerance: per-design times, sweeps, iterations,
+    residuals and the final-design difference between the two forms."""
+    rows = []
+    conv = float(np.float32(0.01))
+    for i, seed in enumerate(seeds):
+        img, prm = synth.target_for(config, int(seed), mesh_res)
+        rec = {"seed": int(seed)}
+        final = {}
+        for use in (0, 1, 2):
+            if use == 2 and not control:
+                continue
+            ctx.set_net(net if use == 1 else None)
+            # use 2: plain SOR at a 2x tighter tolerance, the sensitivity of the
+            # design loop itself to solves that differ within the tolerance
+            cd = design_for(ctx, img, prm, capi.default_opts(tol=tol / 2 if use == 2 else tol,
+                                                             learned_init=1 if use == 1 else 0))
+            rs = cd.run(50, conv, 3, allow=(capi.OK, capi.RASTER_MISS))
+            key = ("plain", "learned", "plain_tight")[use]
+            rec[key] = {"ms": rs.elapsed_ms, "status": rs.status, "ot_steps": rs.transport_steps,
+                        "sweeps": rs.phi_sweeps + rs.h_sweeps, "iterations": rs.learned_applied,
+                        "fallbacks": rs.learned_rejected,
+                        "max_residual": max(rs.max_phi_residual, rs.max_h_residual)}
+            final[use] = (cd.get("target_points"), cd.get("source_points"))
+            cd.close()
+        ctx.set_net(None)
+        W = prm["width"]
+        zr = max(np.abs(final[0][1][:, 2]).max(), 1e-300)
+        for k, lbl in ((1, ""), (2, "control_")):
+            if k in final:
+                rec[lbl + "max_xy_diff_over_width"] = float(
+                    np.abs(final[k][0][:, :2] - final[0][0][:, :2]).max() / W)
+                rec[lbl + "max_height_diff_rel"] = float(
+                    np.abs(final[k][1][:, 2] - final[0][1][:, 2]).max() / zr)
+        rows.append(rec)
+        if log and i % 25 == 0:
+            log(f"eval {i}: seed {seed} plain {rec['plain']['ms']:.1f} ms learned "
+                f"{rec['learned']['ms']:.1f} ms ({rec['learned']['iterations']} it)")
+    ok = [r for r in rows if r["plain"]["status"] == capi.OK and r["learned"]["status"] == capi.OK]
+    tp = sum(r["plain"]["ms"] for r in ok)
+    tl = sum(r["learned"]["ms"] for r in ok)
+    return {
+        "designs": len(rows), "both_ok": len(ok),
+        "plain_ms_mean": tp / max(len(ok), 1), "learned_ms_mean": tl / max(len(ok), 1),
+        "speedup_total_time": tp / max(tl, 1e-30),
+        "speedup_median": float(np.median([r["plain"]["ms"] / r["learned"]["ms"] for r in ok])) if ok else None,
+        "learned_iterations_mean": float(np.mean([r["learned"]["iterations"] for r in ok])) if ok else None,
+        "fallbacks": int(sum(r["learned"]["fallbacks"] for r in ok)),
+        "worst_residual_plain": max((r["plain"]["max_residual"] for r in ok), default=None),
+        "worst_residual_learned": max((r["learned"]["max_residual"] for r in ok), default=None),
+        "max_xy_diff_over_width": max((r["max_xy_diff_over_width"] for r in ok), default=None),
+        "max_height_diff_rel": max((r["max_height_diff_rel"] for r in ok), default=None),
+        "median_xy_diff_over_width": float(np.median([r["max_xy_diff_over_width"] for r in ok]))
+        if ok else None,
+        "rows": rows,
+    }
+
+
 def save_params(net: capi.Net, path: str, meta: dict):
     a = net.arch
     np.savez_compressed(path, params=net.get_params(), levels=a.levels,
@@ -242,12 +302,30 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=5e-4)
     ap.add_argument("--rounds", type=int, default=1)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--evaluate", default=None, metavar="FIRST:COUNT",
+                    help="evaluate --weights (or the trained net) on COUNT held-out seeds")
+    ap.add_argument("--weights", default=None)
+    ap.add_argument("--eval-out", default=None)
+    ap.add_argument("--control", action="store_true",
+                    help="also run plain SOR at half the tolerance (design-loop sensitivity)")
     a = ap.parse_args(argv)
 
     def log(*m):
         print(*m, file=sys.stderr, flush=True)
 
     ctx = capi.Context(0)
+    if a.weights and a.evaluate:
+        first, count = (int(v) for v in a.evaluate.split(":"))
+        net = load_net(ctx, a.weights)
+        res = evaluate(ctx, net, a.config, range(first, first + count), a.mesh_res, log=log,
+                       control=a.control)
+        res.update(config=a.config, mesh_res=a.mesh_res, weights=a.weights,
+                   seeds=f"{first}..{first + count - 1}")
+        if a.eval_out:
+            with open(a.eval_out, "w") as f:
+                json.dump(res, f, indent=1)
+        print(json.dumps({k: v for k, v in res.items() if k != "rows"}), flush=True)
+        return
     arch = capi.default_arch(levels=a.levels, channels=[int(c) for c in a.channels.split(",")],
                              linear=0 if a.relu else 1, out_smooth=a.out_smooth,
                              loss_smooth=a.loss_smooth, coarsen=a.coarsen,
