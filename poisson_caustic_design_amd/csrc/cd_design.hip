@@ -223,37 +223,135 @@ __global__ __launch_bounds__(256) void k_cell_errors(const double* __restrict__ 
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
-// Rasterization, pass 1: every triangle claims the samples it contains; a
-// sample keeps the lowest claiming triangle index.  Samples sit at the
-// node-centred points x_j = j*(W-eps)/(nx-1) + eps/2 (mesh.cpp:241-254).
-__global__ __launch_bounds__(256) void k_raster_scatter(const double* __restrict__ xy, int mx,
-                                                        int my, int nx, int ny, double W,
-                                                        double H, int* __restrict__ owner) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 2 * (mx - 1) * (my - 1)) return;
+// Pass 1 of the rasterization (mesh.cpp:241-254) and of the inverse map
+// (mesh.cpp:354-364): every triangle claims the samples it contains; a sample
+// keeps the lowest claiming triangle index.
+//
+// Sample grids of the two scatters.  px/py are the exact sample coordinates
+// the reference uses; jf/if_ map a coordinate to a fractional sample index
+// (used only to narrow the candidates, every candidate is still tested).
+struct RasterGrid {  // mesh.cpp:245,252: node-centred samples of the N x N grid
+    int nx, ny;
+    double W, H, eps;
+    __device__ double px(int j) const { return (double)j * (W - eps) / (nx - 1) + 0.5 * eps; }
+    __device__ double py(int i) const { return (double)i * (H - eps) / (ny - 1) + 0.5 * eps; }
+    __device__ double jf(double x) const { return (x - 0.5 * eps) / ((W - eps) / (nx - 1)); }
+    __device__ double if_(double y) const { return (y - 0.5 * eps) / ((H - eps) / (ny - 1)); }
+    static constexpr bool kSkipRows = true;  // rows outside [ymin, ymax] +- 1e-9 are skipped
+};
+struct InverseGrid {  // mesh.cpp:355-362: source vertices pulled eps inside the domain
+    int nx, ny;
+    double W, H, eps;
+    __device__ double px(int j) const {
+        return eps + ((double)j * W / (nx - 1)) * ((W - 2 * eps) / W);
+    }
+    __device__ double py(int i) const {
+        return eps + ((double)i * H / (ny - 1)) * ((H - 2 * eps) / H);
+    }
+    __device__ double jf(double x) const { return (x - eps) / ((W - 2 * eps) / W) / (W / (nx - 1)); }
+    __device__ double if_(double y) const { return (y - eps) / ((H - 2 * eps) / H) / (H / (ny - 1)); }
+    static constexpr bool kSkipRows = false;
+};
+
+struct Tri2 {
+    double ax, ay, bx, by, cx, cy, xmin, xmax, ymin, ymax;
+};
+
+__device__ __forceinline__ bool load_tri(const double* __restrict__ xy, int mx, int t, Tri2& T) {
     int a, b, c;
     tri_vertices(mx, t, a, b, c);
-    const double ax = xy[2 * a], ay = xy[2 * a + 1];
-    const double bx = xy[2 * b], by = xy[2 * b + 1];
-    const double cx = xy[2 * c], cy = xy[2 * c + 1];
-    const double xmin = fmin(ax, fmin(bx, cx)), xmax = fmax(ax, fmax(bx, cx));
-    const double ymin = fmin(ay, fmin(by, cy)), ymax = fmax(ay, fmax(by, cy));
-    if (!(isfinite(xmin) && isfinite(xmax) && isfinite(ymin) && isfinite(ymax))) return;
-    const double eps = 1e-8;
-    const double sxs = (W - eps) / (nx - 1), sys = (H - eps) / (ny - 1);
-    const int j0 = max(0, (int)floor((xmin - 0.5 * eps) / sxs) - 1);
-    const int j1 = min(nx - 1, (int)ceil((xmax - 0.5 * eps) / sxs) + 1);
-    const int i0 = max(0, (int)floor((ymin - 0.5 * eps) / sys) - 1);
-    const int i1 = min(ny - 1, (int)ceil((ymax - 0.5 * eps) / sys) + 1);
-    for (int i = i0; i <= i1; ++i) {
-        const double py = (double)i * (H - eps) / (ny - 1) + 0.5 * eps;
-        if (py < ymin - 1e-9 || py > ymax + 1e-9) continue;
-        for (int j = j0; j <= j1; ++j) {
-            const double px = (double)j * (W - eps) / (nx - 1) + 0.5 * eps;
-            double u, v, w;
-            barycentric(ax, ay, bx, by, cx, cy, px, py, u, v, w);
-            if (inside(u, v)) atomicMin(&owner[(size_t)i * nx + j], t);
+    T.ax = xy[2 * a]; T.ay = xy[2 * a + 1];
+    T.bx = xy[2 * b]; T.by = xy[2 * b + 1];
+    T.cx = xy[2 * c]; T.cy = xy[2 * c + 1];
+    T.xmin = fmin(T.ax, fmin(T.bx, T.cx)); T.xmax = fmax(T.ax, fmax(T.bx, T.cx));
+    T.ymin = fmin(T.ay, fmin(T.by, T.cy)); T.ymax = fmax(T.ay, fmax(T.by, T.cy));
+    return isfinite(T.xmin) && isfinite(T.xmax) && isfinite(T.ymin) && isfinite(T.ymax);
+}
+
+// x-extent of the triangle on the horizontal line y clamped into [ymin, ymax]
+__device__ __forceinline__ void row_extent(const Tri2& T, double y, double& xl, double& xr) {
+    const double yc = fmin(fmax(y, T.ymin), T.ymax);
+    xl = INFINITY;
+    xr = -INFINITY;
+    const double P[3][2] = {{T.ax, T.ay}, {T.bx, T.by}, {T.cx, T.cy}};
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        const double px = P[e][0], py = P[e][1], qx = P[(e + 1) % 3][0], qy = P[(e + 1) % 3][1];
+        if (yc < fmin(py, qy) || yc > fmax(py, qy)) continue;
+        if (py == qy) {
+            xl = fmin(xl, fmin(px, qx));
+            xr = fmax(xr, fmax(px, qx));
+        } else {
+            const double s = fmin(fmax((yc - py) / (qy - py), 0.0), 1.0);
+            const double x = px + s * (qx - px);
+            xl = fmin(xl, x);
+            xr = fmax(xr, x);
         }
+    }
+}
+
+// candidate rows [i0, i1] and, per row, columns [j0, j1] (one sample of
+// margin each side); every candidate is tested with the reference's
+// barycentric predicate, so the owners are those of a full bounding-box scan
+template <typename G>
+__device__ __forceinline__ void tri_rows(const G& g, const Tri2& T, int& i0, int& i1) {
+    // clamped in floating point first: a run-away vertex cannot overflow the
+    // integer conversion
+    i0 = (int)fmax(0.0, floor(g.if_(T.ymin)) - 1.0);
+    i1 = (int)fmin((double)(g.ny - 1), ceil(g.if_(T.ymax)) + 1.0);
+}
+
+template <typename G>
+__device__ __forceinline__ void scan_row(const G& g, const Tri2& T, int t, int i,
+                                         int* __restrict__ owner) {
+    const double py = g.py(i);
+    if (G::kSkipRows && (py < T.ymin - 1e-9 || py > T.ymax + 1e-9)) return;
+    double xl, xr;
+    row_extent(T, py, xl, xr);
+    const int j0 = (int)fmax(0.0, floor(g.jf(xl)) - 1.0);
+    const int j1 = (int)fmin((double)(g.nx - 1), ceil(g.jf(xr)) + 1.0);
+    for (int j = j0; j <= j1; ++j) {
+        double u, v, w;
+        barycentric(T.ax, T.ay, T.bx, T.by, T.cx, T.cy, g.px(j), py, u, v, w);
+        if (inside(u, v)) atomicMin(&owner[(size_t)i * g.nx + j], t);
+    }
+}
+
+// one thread per triangle; triangles spanning more than kBigRows candidate
+// rows go to a list that one workgroup per triangle scans row-parallel
+constexpr int kBigRows = 16;
+constexpr int kBigBlocks = 1024;  // workgroups striding over the big-triangle list
+
+template <typename G>
+__global__ __launch_bounds__(256) void k_scatter_small(const double* __restrict__ xy, int mx,
+                                                       int my, G g, int* __restrict__ owner,
+                                                       int* __restrict__ big, int* __restrict__ nbig) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * (mx - 1) * (my - 1)) return;
+    Tri2 T;
+    if (!load_tri(xy, mx, t, T)) return;
+    int i0, i1;
+    tri_rows(g, T, i0, i1);
+    if (i1 - i0 + 1 > kBigRows) {
+        big[atomicAdd(nbig, 1)] = t;
+        return;
+    }
+    for (int i = i0; i <= i1; ++i) scan_row(g, T, t, i, owner);
+}
+
+template <typename G>
+__global__ __launch_bounds__(256) void k_scatter_big(const double* __restrict__ xy, int mx, G g,
+                                                     int* __restrict__ owner,
+                                                     const int* __restrict__ big,
+                                                     const int* __restrict__ nbig) {
+    const int n = *nbig;
+    for (int k = blockIdx.x; k < n; k += gridDim.x) {
+        const int t = big[k];
+        Tri2 T;
+        if (!load_tri(xy, mx, t, T)) continue;
+        int i0, i1;
+        tri_rows(g, T, i0, i1);
+        for (int i = i0 + (int)threadIdx.x; i <= i1; i += blockDim.x) scan_row(g, T, t, i, owner);
     }
 }
 
@@ -363,41 +461,6 @@ __global__ __launch_bounds__(256) void k_vertex_step(const double* __restrict__ 
 // ---------------------------------------------------------------------------
 // height step (caustic_design.cpp:269-332)
 // ---------------------------------------------------------------------------
-// Inverse map, pass 1: each deformed target triangle claims the inset
-// regular source vertices it contains (mesh.cpp:354-364).
-__global__ __launch_bounds__(256) void k_inverse_scatter(const double* __restrict__ xy, int mx,
-                                                         int my, double W, double H,
-                                                         int* __restrict__ owner) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 2 * (mx - 1) * (my - 1)) return;
-    int a, b, c;
-    tri_vertices(mx, t, a, b, c);
-    const double ax = xy[2 * a], ay = xy[2 * a + 1];
-    const double bx = xy[2 * b], by = xy[2 * b + 1];
-    const double cx = xy[2 * c], cy = xy[2 * c + 1];
-    const double xmin = fmin(ax, fmin(bx, cx)), xmax = fmax(ax, fmax(bx, cx));
-    const double ymin = fmin(ay, fmin(by, cy)), ymax = fmax(ay, fmax(by, cy));
-    if (!(isfinite(xmin) && isfinite(xmax) && isfinite(ymin) && isfinite(ymax))) return;
-    const double eps = 1e-8;
-    const double kx = (W - 2 * eps) / W, ky = (H - 2 * eps) / H;
-    const double gx = W / (mx - 1), gy = H / (my - 1);
-    const int j0 = max(0, (int)floor((xmin - eps) / kx / gx) - 1);
-    const int j1 = min(mx - 1, (int)ceil((xmax - eps) / kx / gx) + 1);
-    const int i0 = max(0, (int)floor((ymin - eps) / ky / gy) - 1);
-    const int i1 = min(my - 1, (int)ceil((ymax - eps) / ky / gy) + 1);
-    for (int i = i0; i <= i1; ++i) {
-        const double sy = (double)i * H / (my - 1);
-        const double py = eps + sy * ((H - 2 * eps) / H);
-        for (int j = j0; j <= j1; ++j) {
-            const double sx = (double)j * W / (mx - 1);
-            const double px = eps + sx * ((W - 2 * eps) / W);
-            double u, v, w;
-            barycentric(ax, ay, bx, by, cx, cy, px, py, u, v, w);
-            if (inside(u, v)) atomicMin(&owner[(size_t)i * mx + j], t);
-        }
-    }
-}
-
 // Inverse map, pass 2 + refractive normals (mesh.cpp:365-402, 677-722).
 // partials: misses.
 __global__ __launch_bounds__(256) void k_inverse_normals(const double* __restrict__ xy, int mx,
@@ -579,6 +642,7 @@ struct cd_design {
     int recoveries = 0;          // misses recovered since the last init
     DevBuf smooth_tmp;           // V x 2 copy of the target points
     DevBuf txy, sxy, sz, tareas, errors, area, vgrad, inv, nrm_x, nrm_y, hv, quadacc;
+    DevBuf big, nbig;  // scatter: triangles spanning many sample rows, and their count
     DevBuf pixels, image, raster, phi, h, div, normx, normy, owner, owner_src, owner_v;
     DevBuf partials, scalars, tmp3;
     cd_solve_stats last_phi{}, last_h{};
@@ -603,6 +667,8 @@ int alloc_design(cd_design* d) {
     CD_TRY(c->ensure(d->hv, 8 * V));
     CD_TRY(c->ensure(d->quadacc, 6 * 8 * V));
     CD_TRY(c->ensure(d->owner_v, 4 * V));
+    CD_TRY(c->ensure(d->big, 4 * (size_t)d->T));
+    CD_TRY(c->ensure(d->nbig, sizeof(int)));
     CD_TRY(c->ensure(d->pixels, 8 * G));
     CD_TRY(c->ensure(d->image, 8 * G));
     CD_TRY(c->ensure(d->raster, 8 * G));
@@ -647,8 +713,12 @@ int raster(cd_design* d, const double* xy, int* owner, bool rebuild_owner, const
     CD_TRY(c->timer_begin());
     if (rebuild_owner) {
         CD_TRY(cd_fill_i32(c, owner, d->G, INT_MAX));
-        hipLaunchKernelGGL(k_raster_scatter, dim3(tb), dim3(CD_BLOCK), 0, s, xy, d->mx, d->my,
-                           d->nx, d->ny, d->p.width, d->p.height, owner);
+        CD_TRY(cd_fill_i32(c, d->nbig.as<int>(), 1, 0));
+        const RasterGrid g{d->nx, d->ny, d->p.width, d->p.height, 1e-8};
+        hipLaunchKernelGGL(k_scatter_small<RasterGrid>, dim3(tb), dim3(CD_BLOCK), 0, s, xy, d->mx,
+                           d->my, g, owner, d->big.as<int>(), d->nbig.as<int>());
+        hipLaunchKernelGGL(k_scatter_big<RasterGrid>, dim3(kBigBlocks), dim3(CD_BLOCK), 0, s, xy,
+                           d->mx, g, owner, d->big.as<int>(), d->nbig.as<int>());
     }
     hipLaunchKernelGGL(k_raster_gather, dim3(gb), dim3(CD_BLOCK), 0, s, xy, d->mx, v0, v1, d->nx,
                        d->ny, d->p.width, d->p.height, owner, out0, out1,
@@ -708,9 +778,14 @@ int design_init_from_device_image(cd_design* d, const double* img) {
     // owners of the static source triangulation (mesh.cpp:291-345 locates
     // samples in the undeformed source mesh, whose xy never change)
     CD_TRY(cd_fill_i32(c, d->owner_src.as<int>(), d->G, INT_MAX));
-    hipLaunchKernelGGL(k_raster_scatter, dim3(div_up(d->T, CD_BLOCK)), dim3(CD_BLOCK), 0, s,
-                       d->sxy.as<double>(), d->mx, d->my, d->nx, d->ny, d->p.width, d->p.height,
-                       d->owner_src.as<int>());
+    CD_TRY(cd_fill_i32(c, d->nbig.as<int>(), 1, 0));
+    const RasterGrid g{d->nx, d->ny, d->p.width, d->p.height, 1e-8};
+    hipLaunchKernelGGL(k_scatter_small<RasterGrid>, dim3(div_up(d->T, CD_BLOCK)), dim3(CD_BLOCK),
+                       0, s, d->sxy.as<double>(), d->mx, d->my, g, d->owner_src.as<int>(),
+                       d->big.as<int>(), d->nbig.as<int>());
+    hipLaunchKernelGGL(k_scatter_big<RasterGrid>, dim3(kBigBlocks), dim3(CD_BLOCK), 0, s,
+                       d->sxy.as<double>(), d->mx, g, d->owner_src.as<int>(), d->big.as<int>(),
+                       d->nbig.as<int>());
     CD_CHECK_LAUNCH(c);
     d->recoveries = 0;
     d->initialized = true;
@@ -907,9 +982,14 @@ int cd_design_height_step(cd_design* d, double* update_out) {
     const double focal = d->nx / d->p.width * d->p.focal_l;
     CD_TRY(c->timer_begin());
     CD_TRY(cd_fill_i32(c, d->owner_v.as<int>(), d->V, INT_MAX));
-    hipLaunchKernelGGL(k_inverse_scatter, dim3(div_up(d->T, CD_BLOCK)), dim3(CD_BLOCK), 0, s,
-                       d->txy.as<double>(), d->mx, d->my, d->p.width, d->p.height,
-                       d->owner_v.as<int>());
+    CD_TRY(cd_fill_i32(c, d->nbig.as<int>(), 1, 0));
+    const InverseGrid ig{d->mx, d->my, d->p.width, d->p.height, 1e-8};
+    hipLaunchKernelGGL(k_scatter_small<InverseGrid>, dim3(div_up(d->T, CD_BLOCK)), dim3(CD_BLOCK),
+                       0, s, d->txy.as<double>(), d->mx, d->my, ig, d->owner_v.as<int>(),
+                       d->big.as<int>(), d->nbig.as<int>());
+    hipLaunchKernelGGL(k_scatter_big<InverseGrid>, dim3(kBigBlocks), dim3(CD_BLOCK), 0, s,
+                       d->txy.as<double>(), d->mx, ig, d->owner_v.as<int>(), d->big.as<int>(),
+                       d->nbig.as<int>());
     hipLaunchKernelGGL(k_inverse_normals, dim3(vb), dim3(CD_BLOCK), 0, s, d->txy.as<double>(),
                        d->mx, d->my, d->p.width, d->p.height, d->owner_v.as<int>(),
                        d->sz.as<double>(), focal, d->p.ior, d->inv.as<double>(),
