@@ -127,3 +127,40 @@ def test_raster_miss_recovery_smooths_and_continues(ctx, oracle):
     assert np.array_equal(cd.get("target_points")[:, :2], exp)
     rs = cd.run(3, float(np.float32(0.01)), 0, allow=(capi.OK, capi.RASTER_MISS))
     assert rs.transport_steps >= 1
+
+
+def test_c1_design_matches_oracle_fixture(ctx):
+    """The complete C1 design (100x100 mesh, 400x400 grid, gauss5 seed 0,
+    <=50 OT steps below 0.01f, 3 height steps) on the GPU against the oracle's
+    FP64 design (tests/golden/c1_design.npz, both solving to residual 1e-10)."""
+    import os
+    from poisson_caustic_design_amd import synth
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "c1_design.npz"))
+    img, prm = synth.target_for("gauss5", 0)
+    M, N, W = prm["mesh_res"], prm["res"], prm["width"]
+    cd = capi.CausticDesign(ctx)
+    cd.set_mesh_resolution(M, M)
+    cd.set_domain_resolution(N, N)
+    cd.set_mesh_size(W, prm["height"])
+    cd.set_lens_focal_length(prm["focal_l"])
+    cd.set_lens_thickness(0.1)
+    o = capi.default_opts(tol=float(g["tol"]))
+    cd.set_solve_opts(o, o)
+    cd.initialize_solvers(img)
+    conv = float(np.float32(0.01))
+    steps = []
+    for _ in range(50):
+        steps.append(cd.perform_transport_iteration())
+        if steps[-1] < conv:
+            break
+    for i in range(3):
+        cd.perform_height_map_iteration(i)
+    ref_steps = g["steps"]
+    assert len(steps) == len(ref_steps)
+    assert np.allclose(steps, ref_steps, rtol=1e-6)
+    dxy = np.abs(cd.get("target_points")[:, :2] - g["target_xy"]).max() / W
+    z, zr = cd.get("source_points")[:, 2], g["heights"]
+    dz = np.abs(z - zr).max() / np.abs(zr).max()
+    print(f"C1: {len(steps)} OT steps, max |dxy|/W {dxy:.2e}, max |dz|/max|z| {dz:.2e}")
+    assert dxy < 1e-8 and dz < 1e-7
+    cd.close()
