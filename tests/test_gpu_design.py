@@ -162,5 +162,5 @@ def test_c1_design_matches_oracle_fixture(ctx):
     z, zr = cd.get("source_points")[:, 2], g["heights"]
     dz = np.abs(z - zr).max() / np.abs(zr).max()
     print(f"C1: {len(steps)} OT steps, max |dxy|/W {dxy:.2e}, max |dz|/max|z| {dz:.2e}")
-    assert dxy < 1e-8 and dz < 1e-7
+    assert dxy < 1e-11 and dz < 1e-10  # measured 1.0e-13 and 7.4e-13
     cd.close()
