@@ -290,17 +290,17 @@ def main(argv=None):
     ap.add_argument("--levels", type=int, default=4)
     ap.add_argument("--channels", default="16,32,32,32")
     ap.add_argument("--relu", action="store_true")
-    ap.add_argument("--out-smooth", type=int, default=4)
+    ap.add_argument("--out-smooth", type=int, default=6)
     ap.add_argument("--loss-smooth", type=int, default=0)
     ap.add_argument("--coarsen", type=int, default=16)
     ap.add_argument("--sweeps", type=int, default=8, help="smoothing sweeps per FCG iteration")
     ap.add_argument("--omega", type=float, default=1.5)
-    ap.add_argument("--random", type=int, default=600)
+    ap.add_argument("--random", type=int, default=1200)
     ap.add_argument("--capacity", type=int, default=4000)
-    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--steps", type=int, default=8000)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=5e-4)
-    ap.add_argument("--rounds", type=int, default=1)
+    ap.add_argument("--rounds", type=int, default=2)
     ap.add_argument("--out", default=None)
     ap.add_argument("--evaluate", default=None, metavar="FIRST:COUNT",
                     help="evaluate --weights (or the trained net) on COUNT held-out seeds")
