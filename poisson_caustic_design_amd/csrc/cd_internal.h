@@ -206,5 +206,8 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
 // Learned network forward on one coarse field (cd_learned.hip).
 int cd_net_forward_internal(cd_net* n, const float* d_x, float* d_y, int h, int w);
 const cd_net_arch* cd_net_arch_of(const cd_net* n);
+// forward on a coarse residual normalised on the fly (stats = sum, sum^2)
+int cd_net_forward_coarse(cd_net* n, const float* rc, const double* stats, float* d_y, int h,
+                          int w);
 // bumped whenever the net's activation buffers move (batch shape changes)
 unsigned long long cd_net_gen(const cd_net* n);

@@ -212,6 +212,21 @@ __global__ void k_prep_input(const float* __restrict__ x, u16* __restrict__ x8, 
     *reinterpret_cast<uint4*>(x8 + p * 8) = v.u;
 }
 
+// normalised coarse residual -> 8-channel BF16 network input: x = (rc - mean)
+// / rms from stats = (sum, sum^2) over n cells (the solver's coarse residual)
+__global__ void k_prep_normalized(const float* __restrict__ rc, const double* __restrict__ stats,
+                                  int n, u16* __restrict__ x8) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const double mean = stats[0] / n;
+    const double var = stats[1] / n - mean * mean;
+    const double sig = sqrt(var > 1e-300 ? var : 1e-300);
+    V8 v;
+    v.u = make_uint4(0, 0, 0, 0);
+    v.h[0] = to_bf16((float)(((double)rc[p] - mean) / sig));
+    *reinterpret_cast<uint4*>(x8 + (long long)p * 8) = v.u;
+}
+
 __global__ void k_avgpool2(const u16* __restrict__ x, u16* __restrict__ y, int B, int H, int W,
                            int C) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -317,6 +332,58 @@ __global__ __launch_bounds__(256) void k_smooth_step(const float* __restrict__ i
     if (p >= (long long)B * H * W) return;
     const int xx = (int)(p % W), yy = (int)((p / W) % H);
     out[p] = in[p] + 0.125f * lap_field(in, 1, p, xx, yy, H, W);
+}
+
+// k smoothing steps in one launch: a 16x16 output tile with a k-cell halo
+// lives in LDS (two buffers); step s updates the cells at distance >= s from
+// the halo border, so the tile interior equals k global steps.  Same
+// arithmetic and summation order as k_smooth_step (bit-identical).  The input
+// has element stride `istride` (8: head channel 0).
+constexpr int kSmoothTile = 16;  // small tiles: enough workgroups at 128x128
+constexpr int kSmoothMaxK = 12;  // halo <= 12: 2 (16 + 2k)^2 floats of LDS
+__global__ __launch_bounds__(256) void k_smooth_tiled(const float* __restrict__ in, int istride,
+                                                      float* __restrict__ out, int B, int H,
+                                                      int W, int k) {
+    extern __shared__ float lds[];
+    const int R = kSmoothTile + 2 * k;
+    float* a = lds;
+    float* b = lds + R * R;
+    const int bz = blockIdx.z;
+    const int X0 = blockIdx.x * kSmoothTile - k, Y0 = blockIdx.y * kSmoothTile - k;
+    const long long base = (long long)bz * H * W;
+    for (int idx = threadIdx.x; idx < R * R; idx += blockDim.x) {
+        const int ry = idx / R, rx = idx - ry * R;
+        const int gx = X0 + rx, gy = Y0 + ry;
+        a[idx] = (gx >= 0 && gx < W && gy >= 0 && gy < H) ? in[(base + (long long)gy * W + gx) * istride]
+                                                          : 0.f;
+    }
+    __syncthreads();
+    for (int st = 1; st <= k; ++st) {
+        for (int idx = threadIdx.x; idx < R * R; idx += blockDim.x) {
+            const int ry = idx / R, rx = idx - ry * R;
+            const int gx = X0 + rx, gy = Y0 + ry;
+            float v = a[idx];
+            if (rx >= st && rx < R - st && ry >= st && ry < R - st && gx >= 0 && gx < W &&
+                gy >= 0 && gy < H) {
+                float sum = 0.f, cnt = 0.f;
+                if (gx > 0) { sum += a[idx - 1]; cnt += 1.f; }
+                if (gx < W - 1) { sum += a[idx + 1]; cnt += 1.f; }
+                if (gy > 0) { sum += a[idx - R]; cnt += 1.f; }
+                if (gy < H - 1) { sum += a[idx + R]; cnt += 1.f; }
+                v = v + 0.125f * (sum - cnt * v);
+            }
+            b[idx] = v;
+        }
+        __syncthreads();
+        float* t = a;
+        a = b;
+        b = t;
+    }
+    for (int idx = threadIdx.x; idx < kSmoothTile * kSmoothTile; idx += blockDim.x) {
+        const int ty = idx / kSmoothTile, tx = idx - ty * kSmoothTile;
+        const int gx = X0 + k + tx, gy = Y0 + k + ty;
+        if (gx < W && gy < H) out[base + (long long)gy * W + gx] = a[(ty + k) * R + tx + k];
+    }
 }
 
 // operator residual u = cs * Lc y - x
@@ -712,24 +779,49 @@ int backward(cd_net* n, int B, int H, int W) {
 // k symmetric smoothing steps on a [B][H][W] field held in *a (b is scratch);
 // the result ends in *a
 int smooth_steps(cd_net* n, DevBuf& a, DevBuf& b, int k, int B, int H, int W) {
+    if (k <= 0) return CD_OK;
     const long long px = (long long)B * H * W;
-    for (int i = 0; i < k; ++i) {
-        hipLaunchKernelGGL(k_smooth_step, dim3(div_up(px, 256)), dim3(256), 0, n->ctx->stream,
-                           a.as<float>(), b.as<float>(), B, H, W);
+    if (k <= kSmoothMaxK) {
+        const int R = kSmoothTile + 2 * k;
+        const dim3 grid(div_up(W, kSmoothTile), div_up(H, kSmoothTile), B);
+        hipLaunchKernelGGL(k_smooth_tiled, grid, dim3(256), sizeof(float) * 2 * R * R,
+                           n->ctx->stream, a.as<float>(), 1, b.as<float>(), B, H, W, k);
         std::swap(a.ptr, b.ptr);
         std::swap(a.bytes, b.bytes);
+    } else {
+        for (int i = 0; i < k; ++i) {
+            hipLaunchKernelGGL(k_smooth_step, dim3(div_up(px, 256)), dim3(256), 0,
+                               n->ctx->stream, a.as<float>(), b.as<float>(), B, H, W);
+            std::swap(a.ptr, b.ptr);
+            std::swap(a.bytes, b.bytes);
+        }
     }
     CD_CHECK_LAUNCH(n->ctx);
     return CD_OK;
 }
 
-// network output y = S^out_smooth (head channel 0) into n->yout
-int finish_output(cd_net* n, int B, int H, int W) {
+// network output y = S^out_smooth (head channel 0) into dst (n->yout when null)
+int finish_output(cd_net* n, int B, int H, int W, float* dst = nullptr) {
     const long long px = (long long)B * H * W;
+    float* y = dst ? dst : n->yout.as<float>();
+    const int k = n->arch.out_smooth;
+    if (k > 0 && k <= kSmoothMaxK) {
+        // head channel 0 -> k steps -> y in one launch
+        const int R = kSmoothTile + 2 * k;
+        const dim3 grid(div_up(W, kSmoothTile), div_up(H, kSmoothTile), B);
+        hipLaunchKernelGGL(k_smooth_tiled, grid, dim3(256), sizeof(float) * 2 * R * R,
+                           n->ctx->stream, n->head.as<float>(), 8, y, B, H, W, k);
+        CD_CHECK_LAUNCH(n->ctx);
+        return CD_OK;
+    }
     hipLaunchKernelGGL(k_head_out, dim3(div_up(px, 256)), dim3(256), 0, n->ctx->stream,
                        n->head.as<float>(), n->yout.as<float>(), px);
     CD_CHECK_LAUNCH(n->ctx);
-    return smooth_steps(n, n->yout, n->ytmp, n->arch.out_smooth, B, H, W);
+    CD_TRY(smooth_steps(n, n->yout, n->ytmp, k, B, H, W));
+    if (dst)
+        CD_HIP(n->ctx, hipMemcpyAsync(dst, n->yout.ptr, sizeof(float) * px,
+                                      hipMemcpyDeviceToDevice, n->ctx->stream));
+    return CD_OK;
 }
 
 // dL/dy in n->gy -> dL/dhead (S is symmetric, so its transpose is itself)
@@ -822,14 +914,23 @@ int adam_step(cd_net* n, float lr) {
 
 }  // namespace
 
+// solver-side entry: forward on the coarse residual rc (batch 1), normalised
+// on the fly with stats = (sum, sum^2); output FP32 [h][w]
+int cd_net_forward_coarse(cd_net* n, const float* rc, const double* stats, float* d_y, int h,
+                          int w) {
+    CD_TRY(ensure_batch(n, 1, h, w));
+    hipLaunchKernelGGL(k_prep_normalized, dim3(div_up(h * w, 256)), dim3(256), 0, n->ctx->stream,
+                       rc, stats, h * w, n->x8.as<u16>());
+    CD_CHECK_LAUNCH(n->ctx);
+    CD_TRY(forward(n, 1, h, w));
+    return finish_output(n, 1, h, w, d_y);
+}
+
 // solver-side entry: forward on one coarse field (batch 1), output FP32 [h][w]
 int cd_net_forward_internal(cd_net* n, const float* d_x, float* d_y, int h, int w) {
     CD_TRY(prep(n, d_x, 1, h, w));
     CD_TRY(forward(n, 1, h, w));
-    CD_TRY(finish_output(n, 1, h, w));
-    CD_HIP(n->ctx, hipMemcpyAsync(d_y, n->yout.ptr, sizeof(float) * (size_t)h * w,
-                                  hipMemcpyDeviceToDevice, n->ctx->stream));
-    return CD_OK;
+    return finish_output(n, 1, h, w, d_y);
 }
 
 const cd_net_arch* cd_net_arch_of(const cd_net* n) { return &n->arch; }
@@ -968,9 +1069,7 @@ int cd_net_forward(cd_net* n, const float* d_x, float* d_y, int batch, int h, in
     CD_TRY(prep(n, d_x, batch, h, w));
     CD_TRY(forward(n, batch, h, w));
     const long long px = (long long)batch * h * w;
-    CD_TRY(finish_output(n, batch, h, w));
-    CD_HIP(n->ctx, hipMemcpyAsync(d_y, n->yout.ptr, sizeof(float) * (size_t)px,
-                                  hipMemcpyDeviceToDevice, n->ctx->stream));
+    CD_TRY(finish_output(n, batch, h, w, d_y));
     CD_HIP(n->ctx, hipStreamSynchronize(n->ctx->stream));
     return CD_OK;
 }

@@ -625,39 +625,65 @@ __device__ __forceinline__ size_t brick_at(const Brick& bk, int x, int y) {
 }
 
 // coarse residual rc = block mean of the projected BF16 residual the blocked
-// sweeps see (r16 - rmean32); partials (sum, sum^2) per workgroup.  A
-// workgroup of 256 threads covers `per` = max(1, 256 / f^2) coarse cells, one
-// thread per fine cell (f^2 > 256: each thread sums f^2 / 256 of them).
+// sweeps see (r16 - rmean32).  f <= 8: one thread per coarse cell; larger f:
+// one wave per coarse cell (lanes stride over its f^2 fine cells, fixed-order
+// shuffle tree).  Statistics of rc follow in k_coarse_stats.
 __global__ __launch_bounds__(256) void k_restrict_r16(const unsigned short* __restrict__ r16,
                                                       Brick bk, const SolveDev* __restrict__ st,
                                                       int f, int wc, int hc,
-                                                      float* __restrict__ rc,
-                                                      double* __restrict__ partials) {
-    __shared__ float part[256];
-    __shared__ double smem[16];
-    const int ff = f * f;
-    const int per = ff >= 256 ? 1 : 256 / ff;
-    const int tpc = ff >= 256 ? 256 : ff;  // threads per coarse cell
-    const int local = threadIdx.x / tpc, k0 = threadIdx.x - local * tpc;
-    const int cell = blockIdx.x * per + local;
+                                                      float* __restrict__ rc) {
     const float rm = st->rmean32;
+    const int ff = f * f;
+    if (f <= 8) {
+        const int i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= wc * hc) return;
+        const int J = i / wc, I = i - J * wc;
+        float acc = 0.f;
+        for (int yy = J * f; yy < J * f + f; ++yy)
+            for (int xx = I * f; xx < I * f + f; ++xx)
+                acc += bf16_to_f32(r16[brick_at(bk, xx, yy)]) - rm;
+        rc[i] = acc / (float)ff;
+        return;
+    }
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (i >= wc * hc) return;  // whole wave
+    const int J = i / wc, I = i - J * wc;
     float acc = 0.f;
-    if (local < per && cell < wc * hc) {
-        const int J = cell / wc, I = cell - J * wc;
-        for (int k = k0; k < ff; k += tpc) {
-            const int yy = J * f + k / f, xx = I * f + (k - (k / f) * f);
-            acc += bf16_to_f32(r16[brick_at(bk, xx, yy)]) - rm;
-        }
+    for (int k = lane; k < ff; k += 64) {
+        const int dy = k / f, dx = k - dy * f;
+        acc += bf16_to_f32(r16[brick_at(bk, I * f + dx, J * f + dy)]) - rm;
     }
-    part[threadIdx.x] = acc;
-    __syncthreads();
-    double v = 0.0;
-    if (threadIdx.x < per && blockIdx.x * per + threadIdx.x < wc * hc) {
-        float t = 0.f;
-        for (int i = 0; i < tpc; ++i) t += part[threadIdx.x * tpc + i];  // fixed order
-        v = (double)t / (double)ff;
-        rc[blockIdx.x * per + threadIdx.x] = (float)v;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (lane == 0) rc[i] = acc / (float)ff;
+}
+
+// (sum, sum^2) of the coarse residual in one workgroup (fixed order), for
+// coarse grids of up to 64Ki cells
+__global__ __launch_bounds__(1024) void k_coarse_stats1(const float* __restrict__ rc, int nc,
+                                                        double* __restrict__ stats) {
+    __shared__ double smem[16];
+    double a = 0.0, q = 0.0;
+    for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+        const double v = rc[i];
+        a += v;
+        q += v * v;
     }
+    a = cd_block_sum(a, smem);
+    q = cd_block_sum(q, smem);
+    if (threadIdx.x == 0) {
+        stats[0] = a;
+        stats[1] = q;
+    }
+}
+
+// partials (sum, sum^2) of the coarse residual per workgroup of 256 cells
+__global__ __launch_bounds__(256) void k_coarse_stats(const float* __restrict__ rc, int nc,
+                                                      double* __restrict__ partials) {
+    __shared__ double smem[16];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double v = i < nc ? (double)rc[i] : 0.0;
     const double a = cd_block_sum(v, smem);
     const double q = cd_block_sum(v * v, smem);
     if (threadIdx.x == 0) {
@@ -677,17 +703,14 @@ __global__ void k_normalize_coarse(const float* __restrict__ rc, const double* _
     x[i] = (float)(((double)rc[i] - mean) / sig);
 }
 
-// packed-layout walk: element g of the two colour planes -> (plane c, row Y,
-// packed column j, cell x); false for padding
-__device__ __forceinline__ bool packed_cell(long long g, const Brick& bk, int w, int h, int& c,
-                                            int& Y, int& j, int& x) {
-    const long long per_plane = (long long)h * bk.wpb;
-    c = (int)(g / per_plane);
-    const long long r = g - (long long)c * per_plane;
-    Y = (int)(r / bk.wpb);
-    j = (int)(r - (long long)Y * bk.wpb);
+// packed-layout launch: grid (ceil(wpb / 256), rows, 2 colour planes); a
+// thread owns packed column j of row Y in plane c, cell x = 2j + ((c+Y)&1)
+__device__ __forceinline__ bool packed_thread(const Brick& bk, int w, int Y, int& c, int& j,
+                                              int& x) {
+    c = blockIdx.z;
+    j = blockIdx.x * blockDim.x + threadIdx.x;
     x = 2 * j + ((c + Y) & 1);
-    return c < 2 && x < w;
+    return j < bk.wpb && x < w;
 }
 
 // e = rms * (nmax/pi)^2 * bilinear(y) at cell centres, written in the
@@ -698,9 +721,9 @@ __global__ __launch_bounds__(256) void k_prolong_bricked(const float* __restrict
                                                          int w, int h, int f, int wc, int hc,
                                                          double nscale, Brick bk,
                                                          float* __restrict__ e) {
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    int c, yy, j, xx;
-    if (!packed_cell(g, bk, w, h, c, yy, j, xx)) return;
+    const int yy = blockIdx.y;
+    int c, j, xx;
+    if (!packed_thread(bk, w, yy, c, j, xx)) return;
     const double mean = stats[0] / nc;
     const double var = stats[1] / nc - mean * mean;
     const double sig = sqrt(var > 1e-300 ? var : 1e-300);
@@ -746,7 +769,7 @@ __device__ __forceinline__ float lap_packed(const float* __restrict__ z, const B
 }
 
 // partials[4 * block + k]: z.q_old, z.r, p_old.r, z.Lz over the packed
-// elements the workgroup strides through (fixed order: deterministic)
+// elements of the rows the workgroup strides through (fixed order)
 __global__ __launch_bounds__(256) void k_fcg_dots(const float* __restrict__ z,
                                                   const float* __restrict__ p,
                                                   const float* __restrict__ q,
@@ -755,12 +778,10 @@ __global__ __launch_bounds__(256) void k_fcg_dots(const float* __restrict__ z,
                                                   int w, int h, double* __restrict__ partials) {
     __shared__ double smem[16];
     const float rm = st->rmean32;
-    const long long total = 2LL * h * bk.wpb;
     double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
-    for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < total;
-         g += (long long)gridDim.x * blockDim.x) {
-        int c, Y, j, x;
-        if (!packed_cell(g, bk, w, h, c, Y, j, x)) continue;
+    for (int Y = blockIdx.y; Y < h; Y += gridDim.y) {
+        int c, j, x;
+        if (!packed_thread(bk, w, Y, c, j, x)) continue;
         const size_t i = brick_row(bk, c, Y) + brick_col(j);
         const float zc = z[i];
         const float lz = lap_packed(z, bk, c, Y, j, x, w, h, zc);
@@ -775,10 +796,11 @@ __global__ __launch_bounds__(256) void k_fcg_dots(const float* __restrict__ z,
     d2 = cd_block_sum(d2, smem);
     d3 = cd_block_sum(d3, smem);
     if (threadIdx.x == 0) {
-        partials[4 * blockIdx.x] = d0;
-        partials[4 * blockIdx.x + 1] = d1;
-        partials[4 * blockIdx.x + 2] = d2;
-        partials[4 * blockIdx.x + 3] = d3;
+        const int bid = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        partials[4 * bid] = d0;
+        partials[4 * bid + 1] = d1;
+        partials[4 * bid + 2] = d2;
+        partials[4 * bid + 3] = d3;
     }
 }
 
@@ -819,10 +841,10 @@ __global__ __launch_bounds__(256) void k_fcg_update(const float* __restrict__ z,
                                                     float* __restrict__ eo,
                                                     const FcgDev* __restrict__ fd, Brick bk,
                                                     int w, int h, SolveDev* st, int eo_index) {
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g == 0) st->cur = eo_index;
-    int c, Y, j, x;
-    if (!packed_cell(g, bk, w, h, c, Y, j, x)) return;
+    const int Y = blockIdx.y;
+    if (blockIdx.x == 0 && Y == 0 && blockIdx.z == 0 && threadIdx.x == 0) st->cur = eo_index;
+    int c, j, x;
+    if (!packed_thread(bk, w, Y, c, j, x)) return;
     const size_t i = brick_row(bk, c, Y) + brick_col(j);
     const float beta = (float)fd->beta, alpha = (float)fd->alpha;
     const float zc = z[i];
@@ -1033,18 +1055,32 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         CD_TRY(ctx->ensure(ctx->cap_phi, sizeof(double) * (size_t)cwc * chc * (kMaxPairsPerSolve + 1)));
         CD_TRY(ctx->ensure(ctx->lc_r, sizeof(float) * (size_t)std::max(cwc * chc, lwc * lhc)));
     }
+    // restrict the residual of the last check to (wc x hc) into lc_r; stats
+    // slot receives (sum, sum^2) of the coarse residual
+    auto restrict_stats = [&](int f, int wc, int hc, double* stats) -> int {
+        const int nc = wc * hc;
+        const int rb = f <= 8 ? div_up(nc, 256) : div_up(nc, 4);  // thread / wave per cell
+        hipLaunchKernelGGL(k_restrict_r16, dim3(rb), dim3(256), 0, s, r16, bk, sd, f, wc, hc,
+                           ctx->lc_r.as<float>());
+        if (nc <= 65536) {
+            hipLaunchKernelGGL(k_coarse_stats1, dim3(1), dim3(1024), 0, s, ctx->lc_r.as<float>(),
+                               nc, stats);
+        } else {
+            const int nb = div_up(nc, 256);  // <= nred: the partials buffer is large enough
+            hipLaunchKernelGGL(k_coarse_stats, dim3(nb), dim3(256), 0, s, ctx->lc_r.as<float>(),
+                               nc, partials);
+            CD_TRY(cd_reduce_partials(ctx, partials, nb, 2, stats));
+        }
+        CD_CHECK_LAUNCH(ctx);
+        return CD_OK;
+    };
     // restrict the residual of the last check to (wc x hc), normalise into
     // xout; stats slot receives (sum, sum^2) of the coarse residual
     auto restrict_normalize = [&](int f, int wc, int hc, float* xout, double* stats) -> int {
+        CD_TRY(restrict_stats(f, wc, hc, stats));
         const int nc = wc * hc;
-        const int per = f * f >= 256 ? 1 : 256 / (f * f);  // coarse cells per workgroup
-        const int nb = div_up(nc, per);  // <= nred: the partials buffer is large enough
-        hipLaunchKernelGGL(k_restrict_r16, dim3(nb), dim3(256), 0, s, r16, bk, sd, f, wc, hc,
-                           ctx->lc_r.as<float>(), partials);
-        CD_CHECK_LAUNCH(ctx);
-        CD_TRY(cd_reduce_partials(ctx, partials, nb, 2, stats));
-        hipLaunchKernelGGL(k_normalize_coarse, dim3(nb), dim3(256), 0, s, ctx->lc_r.as<float>(),
-                           stats, nc, xout);
+        hipLaunchKernelGGL(k_normalize_coarse, dim3(div_up(nc, 256)), dim3(256), 0, s,
+                           ctx->lc_r.as<float>(), stats, nc, xout);
         CD_CHECK_LAUNCH(ctx);
         return CD_OK;
     };
@@ -1067,10 +1103,16 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     // the smoothing launches start from e32; after `nl` launches the result
     // sits in e32 (nl even) or e32b (nl odd)
     const int smooth_launches = div_up(nsmooth, BH / 2);
-    // packed-element kernels: one thread per element of the two colour planes
-    // (rows x padded packed width); dot products stride over a capped grid
-    const int packed_blocks = (int)div_up(2LL * h * bk.wpb, 256);
-    const int dot_blocks = std::min(packed_blocks, std::min(nred, 2048));
+    // packed-element kernels: grid (packed width / 256, rows, 2 planes); the
+    // dot products stride over rows so that their partials fit nred blocks
+    const int pbx = div_up(bk.wpb, 256);
+    const dim3 packed_grid(pbx, h, 2);
+    // (at most 256 row groups: each thread accumulates several rows before
+    // the block reduction)
+    const int dot_rows = std::max(1, std::min(std::min(h, 256), nred / (2 * pbx)));
+    const dim3 dot_grid(pbx, dot_rows, 2);
+    const int dot_blocks = 2 * pbx * dot_rows;
+    if (dot_blocks > nred) learned = false;  // tiny grids: partials hold nred blocks
     float* zbuf = (smooth_launches & 1) ? e32b : e32;
     float* obuf = (smooth_launches & 1) ? e32 : e32b;
     const int obuf_index = (smooth_launches & 1) ? 0 : 1;
@@ -1102,21 +1144,21 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         FcgDev* fd = ctx->fcg_dev.as<FcgDev>();
         CD_TRY(ctx->timer_begin());
         double* stats = ctx->lc_stats.as<double>();
-        CD_TRY(restrict_normalize(lf, lwc, lhc, ctx->lc_x.as<float>(), stats));
-        CD_TRY(cd_net_forward_internal(ctx->net, ctx->lc_x.as<float>(), ctx->lc_y.as<float>(), lhc,
-                                       lwc));
-        hipLaunchKernelGGL(k_prolong_bricked, dim3(packed_blocks), dim3(256), 0, s,
+        CD_TRY(restrict_stats(lf, lwc, lhc, stats));
+        CD_TRY(cd_net_forward_coarse(ctx->net, ctx->lc_r.as<float>(), stats,
+                                     ctx->lc_y.as<float>(), lhc, lwc));
+        hipLaunchKernelGGL(k_prolong_bricked, packed_grid, dim3(256), 0, s,
                            ctx->lc_y.as<float>(), stats, lwc * lhc, w, h, lf, lwc, lhc, nscale, bk,
                            e32);
         CD_CHECK_LAUNCH(ctx);
         CD_TRY(ctx->timer_end(CD_KERNEL_LEARNED, 4, 0.0));
         CD_TRY(enqueue_blocked_om(first_sweep, nsmooth, oms / 2.f, oms / 3.f, oms / 4.f));
         CD_TRY(ctx->timer_begin());
-        hipLaunchKernelGGL(k_fcg_dots, dim3(dot_blocks), dim3(256), 0, s, zbuf,
+        hipLaunchKernelGGL(k_fcg_dots, dot_grid, dim3(256), 0, s, zbuf,
                            ctx->fcg_p.as<float>(), ctx->fcg_q.as<float>(), r16, sd, bk, w, h,
                            partials);
         hipLaunchKernelGGL(k_fcg_coef, dim3(1), dim3(1024), 0, s, partials, dot_blocks, fd);
-        hipLaunchKernelGGL(k_fcg_update, dim3(packed_blocks), dim3(256), 0, s, zbuf,
+        hipLaunchKernelGGL(k_fcg_update, packed_grid, dim3(256), 0, s, zbuf,
                            ctx->fcg_p.as<float>(), ctx->fcg_q.as<float>(), obuf, fd, bk, w, h,
                            sd, obuf_index);
         CD_CHECK_LAUNCH(ctx);
