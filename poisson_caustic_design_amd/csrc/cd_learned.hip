@@ -80,10 +80,11 @@ __global__ __launch_bounds__(256) void k_conv3x3(const u16* __restrict__ x1,
     for (int nb = 0; nb < NB; ++nb) acc[nb] = floatx4{0.f, 0.f, 0.f, 0.f};
     const int kq = 8 * (lane >> 4);
     const u16* wrow = wt + (size_t)(lane & 15) * kpad + kq;
-    for (int kc = 0; kc < kpad; kc += 32) {
-        const int k = kc + kq;
+    // A fragment of K chunk kc (8 consecutive k of this lane's pixel)
+    auto load_a = [&](int kc) -> V8 {
         V8 a;
         a.u = make_uint4(0, 0, 0, 0);
+        const int k = kc + kq;
         if (pvalid && k < K) {
             const int tap = k / cin, ci = k - tap * cin;
             const int yy = yb + tap / 3 - 1, xx = xb + tap % 3 - 1;
@@ -108,11 +109,31 @@ __global__ __launch_bounds__(256) void k_conv3x3(const u16* __restrict__ x1,
                 }
             }
         }
+        return a;
+    };
+    // K is walked in groups of 4 chunks: all loads of a group are issued
+    // before its MFMAs (latency overlap); the accumulation order is unchanged
+    constexpr int U = 4;
+    for (int kc0 = 0; kc0 < kpad; kc0 += 32 * U) {
+        V8 a[U], bw[U][NB];
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-            V8 bw;
-            bw.u = *reinterpret_cast<const uint4*>(wrow + (size_t)nb * 16 * kpad + kc);
-            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, bw.v, acc[nb], 0, 0, 0);
+        for (int u = 0; u < U; ++u) {
+            const int kc = kc0 + 32 * u;
+            if (kc < kpad) {
+                a[u] = load_a(kc);
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb)
+                    bw[u][nb].u = *reinterpret_cast<const uint4*>(wrow + (size_t)nb * 16 * kpad + kc);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (kc0 + 32 * u < kpad) {
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb)
+                    acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u].v, bw[u][nb].v, acc[nb],
+                                                                      0, 0, 0);
+            }
         }
     }
 #pragma unroll
