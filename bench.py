@@ -185,6 +185,69 @@ def learned_leg(ctx, cd, one_design, wpath, a, plain_value) -> dict:
     }
 
 
+def batched_leg(img, prm, a, wpath) -> dict:
+    """a.batch independent designs of the same target, one context (HIP
+    stream) each, run together through cd_designs_run_batch; timed per batch,
+    plain and (with weights) learned."""
+    from poisson_caustic_design_amd import capi, learned
+
+    M, N = prm["mesh_res"], prm["res"]
+    S = a.batch
+    conv = float(_np().float32(0.01))
+    res = {"designs_per_batch": S,
+           "note": (f"{S} designs of the same target in flight at once, one context and HIP "
+                    "stream each (cd_designs_run_batch); value = designs/s")}
+    for form in ("plain", "learned"):
+        if form == "learned" and not (a.learned == "auto" and os.path.exists(wpath)):
+            continue
+        ctxs = [capi.Context(0) for _ in range(S)]
+        nets, designs = [], []
+        try:
+            for c in ctxs:
+                if form == "learned":
+                    nets.append(learned.load_net(c, wpath))
+                    c.set_net(nets[-1])
+                cd = capi.CausticDesign(c)
+                cd.set_mesh_resolution(M, M)
+                cd.set_domain_resolution(N, N)
+                cd.set_mesh_size(prm["width"], prm["height"])
+                cd.set_lens_focal_length(prm["focal_l"])
+                cd.set_lens_thickness(0.1)
+                o = capi.default_opts(tol=a.tol, learned_init=int(form == "learned"))
+                cd.set_solve_opts(o, o)
+                cd.initialize_solvers(img)
+                designs.append(cd)
+            imgs = [d.device_ptr("image") for d in designs]
+            for _ in range(a.warmup):
+                capi.run_designs_batch(designs, imgs, 50, conv, 3)
+            for c in ctxs:
+                c.synchronize()
+            worst = 0.0
+            t0 = time.perf_counter()
+            for i in range(a.steps):
+                stats = capi.run_designs_batch(designs, imgs, 50, conv, 3)
+                worst = max([worst] + [max(r.max_phi_residual, r.max_h_residual) for r in stats])
+                log(f"batched {form} step {i}: {S} designs, max residual {worst:.2e}")
+            for c in ctxs:
+                c.synchronize()
+            elapsed = time.perf_counter() - t0
+        finally:
+            for d in designs:
+                d.close()
+            for n in nets:
+                n.close()
+            for c in ctxs:
+                c.close()
+        res[form] = {"value": S * a.steps / elapsed, "ms_per_batch": 1e3 * elapsed / a.steps,
+                     "max_residual": worst}
+    return res
+
+
+def _np():
+    import numpy as np
+    return np
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,6 +268,8 @@ def main(argv=None):
                     help="auto: when weights for the grid exist, also time the same designs "
                          "with the learned coarse solve and report them beside the plain ones")
     ap.add_argument("--weights", default=None)
+    ap.add_argument("--batch", type=int, default=4,
+                    help="also time batches of this many concurrent designs (0: skip)")
     ap.add_argument("--sweeps", type=int, default=64)
     ap.add_argument("--mib", type=int, default=256)
     a = ap.parse_args(argv)
@@ -336,6 +401,8 @@ def main(argv=None):
             "max_abs_height_over_max_height": float(
                 np.abs(sp[:, 2] - plain_final["source_points"][:, 2]).max() / max(zr, 1e-300)),
         }
+    if a.batch > 1:
+        out["batched"] = batched_leg(img, prm, a, wpath)
     cb = None
     if not a.no_cpu_baseline:
         cb = cpu_baseline(a.config, a.seed, per_design, a.cpu_timeout)

@@ -175,6 +175,22 @@ typedef struct {
 int cd_design_run(cd_design* d, int max_transport_steps, double conv_tres, int height_steps,
                   cd_run_stats* stats);
 
+/* Batched designs on concurrent streams.  For each i < n: re-initialise
+ * designs[i] from the device image d_images[i] (cd_design_init_device;
+ * skipped when d_images or d_images[i] is NULL), then cd_design_run(designs[i],
+ * max_transport_steps, conv_tres, height_steps, &stats[i]).  The n designs run
+ * concurrently, one host thread each, every one on its own context's stream;
+ * each design must belong to a different context (a context's stream and
+ * workspace serve one design at a time), else CD_ERR_BAD_ARG before any work.
+ * stats[i].status is each design's status; the return value is CD_OK or the
+ * status of the lowest-index design that failed.  The reference runs one
+ * design per process (main.cpp:243-262); this is that loop over a batch of
+ * independent targets.  n <= CD_BATCH_MAX. */
+#define CD_BATCH_MAX 64
+int cd_designs_run_batch(cd_design* const* designs, const double* const* d_images, int n,
+                         int max_transport_steps, double conv_tres, int height_steps,
+                         cd_run_stats* stats);
+
 typedef enum {
     CD_FIELD_TARGET_POINTS = 0,   /* V x 3  mesh->target_points            */
     CD_FIELD_SOURCE_POINTS = 1,   /* V x 3  mesh->source_points            */

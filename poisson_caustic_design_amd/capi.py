@@ -118,6 +118,7 @@ def load(path: str = LIB_PATH):
         "cd_design_transport_step": ([_P, C.POINTER(_D)], _I),
         "cd_design_height_step": ([_P, C.POINTER(_D)], _I),
         "cd_design_run": ([_P, _I, _D, _I, C.POINTER(RunStats)], _I),
+        "cd_designs_run_batch": ([_P, _P, _I, _I, _D, _I, C.POINTER(RunStats)], _I),
         "cd_design_field_size": ([_P, _I], _SZ),
         "cd_design_get": ([_P, _I, _P, _SZ], _I),
         "cd_design_set": ([_P, _I, _P, _SZ], _I),
@@ -181,6 +182,7 @@ def exported_symbols():
         "cd_json_write_heightmap", "cd_svg_write_grid", "cd_svg_write_triangles",
         "cd_design_save_solid_obj", "cd_design_save_heightmap_json", "cd_design_export_svg",
         "cd_design_laplacian_smoothing", "cd_design_set_miss_recovery",
+        "cd_designs_run_batch",
     ]
 
 
@@ -495,6 +497,38 @@ class CausticDesign:
     pixels = property(lambda self: self.get("pixels"))
     normals = property(lambda self: self.get("normals"))
     vertex_gradient = property(lambda self: self.get("vertex_gradient").T.copy())
+
+
+BATCH_MAX = 64
+
+
+def run_designs_batch(designs, d_images=None, max_transport_steps=50,
+                      conv_tres=float(np.float32(0.01)), height_steps=3, allow=(OK,)):
+    """Run a batch of designs concurrently, one per context (each on its own
+    HIP stream): optional re-initialisation from device images, then the
+    driver loop of main.cpp:243-262 for each.  Returns one RunStats per
+    design; raises on the first design whose status is not allowed."""
+    designs = list(designs)
+    n = len(designs)
+    if n == 0:
+        return []
+    if any(d.d is None for d in designs):
+        raise CDError(STATE, "cd_designs_run_batch: initialise every design first")
+    L = designs[0].L
+    D = (C.c_void_p * n)(*[d.d for d in designs])
+    imgs = None
+    if d_images is not None:
+        if len(d_images) != n:
+            raise ValueError("one device image (or None) per design")
+        imgs = (C.c_void_p * n)(*[p if p else None for p in d_images])
+    stats = (RunStats * n)()
+    rc = L.cd_designs_run_batch(D, imgs, n, max_transport_steps, conv_tres, height_steps, stats)
+    if rc == BAD_ARG:
+        raise CDError(rc, "cd_designs_run_batch: every design needs its own context "
+                          f"and n <= {BATCH_MAX}")
+    for d, r in zip(designs, stats):
+        d.ctx.check(r.status, "cd_designs_run_batch", allow)
+    return list(stats)
 
 
 def png_read_rgba8(path: str) -> np.ndarray:

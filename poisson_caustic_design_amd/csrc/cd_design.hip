@@ -8,6 +8,9 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <system_error>
+#include <thread>
+#include <vector>
 
 #include "cd_geometry.h"
 #include "cd_internal.h"
@@ -1082,6 +1085,46 @@ int cd_design_run(cd_design* d, int max_transport_steps, double conv_tres, int h
     r.elapsed_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (rs) *rs = r;
     return status;
+}
+
+int cd_designs_run_batch(cd_design* const* designs, const double* const* d_images, int n,
+                         int max_transport_steps, double conv_tres, int height_steps,
+                         cd_run_stats* stats) {
+    if (n < 0 || (n > 0 && (!designs || !stats)) || n > CD_BATCH_MAX) return CD_ERR_BAD_ARG;
+    // one design per context: a context's stream and workspace serve one
+    // design at a time
+    for (int i = 0; i < n; ++i) {
+        if (!designs[i]) return CD_ERR_BAD_ARG;
+        if (!designs[i]->ctx) return CD_ERR_STATE;
+        for (int j = 0; j < i; ++j)
+            if (designs[j]->ctx == designs[i]->ctx || designs[j] == designs[i]) return CD_ERR_BAD_ARG;
+    }
+    auto one = [&](int i) {
+        cd_run_stats r;
+        std::memset(&r, 0, sizeof(r));
+        int st = CD_OK;
+        if (d_images && d_images[i]) st = cd_design_init_device(designs[i], d_images[i]);
+        if (st == CD_OK) st = cd_design_run(designs[i], max_transport_steps, conv_tres, height_steps, &r);
+        r.status = st;
+        stats[i] = r;
+    };
+    std::vector<std::thread> workers;
+    int spawned = 0;
+    try {
+        workers.reserve(n > 0 ? n - 1 : 0);
+        for (int i = 1; i < n; ++i) {
+            workers.emplace_back(one, i);
+            ++spawned;
+        }
+    } catch (const std::system_error&) {
+        // threads that could not start run inline below
+    }
+    if (n > 0) one(0);
+    for (int i = 1 + spawned; i < n; ++i) one(i);
+    for (auto& t : workers) t.join();
+    for (int i = 0; i < n; ++i)
+        if (stats[i].status != CD_OK) return stats[i].status;
+    return CD_OK;
 }
 
 int cd_image_gray_resize_device(cd_context* c, const unsigned char* d_rgba, int width,
