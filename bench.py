@@ -173,7 +173,7 @@ def learned_leg(ctx, cd, one_design, wpath, a, plain_value) -> dict:
     return {
         "value": value,
         "ms_per_step": 1e3 * elapsed / a.steps,
-        "speedup_vs_plain": value / plain_value,
+        "speedup_vs_plain": value / plain_value if plain_value else None,
         "weights": os.path.relpath(wpath, ROOT),
         "per_design": {
             "ot_steps": sum(r["transport_steps"] for r in runs) / n,
@@ -183,6 +183,36 @@ def learned_leg(ctx, cd, one_design, wpath, a, plain_value) -> dict:
             "max_residual": max(max(r["max_phi_residual"], r["max_h_residual"]) for r in runs),
         },
     }
+
+
+def learned_only(a) -> int:
+    """The learned leg alone on the benchmark workload (for kernel traces)."""
+    import numpy as np
+
+    from poisson_caustic_design_amd import capi, synth
+
+    img, prm = synth.target_for(a.config, a.seed, a.mesh_res)
+    M, N = prm["mesh_res"], prm["res"]
+    wpath = a.weights or os.path.join(ROOT, "poisson_caustic_design_amd", "weights",
+                                      f"learned_grid{N}.npz")
+    ctx = capi.Context(0)
+    cd = capi.CausticDesign(ctx)
+    cd.set_mesh_resolution(M, M)
+    cd.set_domain_resolution(N, N)
+    cd.set_mesh_size(prm["width"], prm["height"])
+    cd.set_lens_focal_length(prm["focal_l"])
+    cd.set_lens_thickness(0.1)
+    cd.initialize_solvers(img)
+    d_img = cd.device_ptr("image")
+    conv = float(np.float32(0.01))
+
+    def one_design():
+        cd.reinitialize_device(d_img)
+        return cd.run(50, conv, 3)
+
+    res = learned_leg(ctx, cd, one_design, wpath, a, None)
+    print(json.dumps({"mode": "learned", "mesh": M, "grid": N, **res}), flush=True)
+    return 0
 
 
 def batched_leg(img, prm, a, wpath) -> dict:
@@ -260,10 +290,11 @@ def main(argv=None):
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-timeout", type=int, default=240)
-    ap.add_argument("--mode", choices=["design", "sweeps", "copy"], default="design",
+    ap.add_argument("--mode", choices=["design", "sweeps", "copy", "learned"], default="design",
                     help="design: the benchmark; sweeps: a fixed number of SOR sweeps at the "
                          "workload grid (profiling); copy: a device copy of known size "
-                         "(calibrates rocprofv3 byte counters)")
+                         "(calibrates rocprofv3 byte counters); learned: only the learned-"
+                         "init designs (profiling)")
     ap.add_argument("--learned", choices=["auto", "off"], default="auto",
                     help="auto: when weights for the grid exist, also time the same designs "
                          "with the learned coarse solve and report them beside the plain ones")
@@ -277,6 +308,9 @@ def main(argv=None):
         return fixed_sweeps(a)
     if a.mode == "copy":
         return device_copy(a)
+
+    if a.mode == "learned":
+        return learned_only(a)
 
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "

@@ -178,7 +178,9 @@ __global__ __launch_bounds__(256) void k_flush_packed(double* __restrict__ phi,
 // partials[block] = (sum r, sum r^2)
 // ---------------------------------------------------------------------------
 // OUT: 0 norms only, 1 FP32 copy of r (natural layout), 2 BF16 copy of r
-// (bricked layout, blocked sweeps)
+// (bricked layout, blocked sweeps).  A thread takes `rows` rows (stride
+// blockDim.y) of its column before the two block reductions, so a launch
+// reduces over few workgroups.
 template <int OUT>
 __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ phi,
                                                   const double* __restrict__ b,
@@ -186,12 +188,16 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ phi
                                                   const SolveDev* __restrict__ st,
                                                   float* __restrict__ r32,
                                                   unsigned short* __restrict__ r16, Brick bk,
-                                                  int w, int h, double* __restrict__ partials) {
+                                                  int w, int h, int rows,
+                                                  double* __restrict__ partials) {
     __shared__ double smem[16];
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y0 = blockIdx.y * blockDim.y * rows + threadIdx.y;
+    const double bmean = st->bmean;
     double rs = 0.0, rq = 0.0;
-    if (x < w && y < h) {
+    for (int k = 0; k < rows; ++k) {
+        const int y = y0 + k * blockDim.y;
+        if (x >= w || y >= h) break;
         size_t i = (size_t)y * w + x;
         double c = phi[i];
         double s = 0.0, cnt = 0.0;
@@ -211,9 +217,9 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ phi
         }
         double r = 0.0;
         if (!hole) {
-            r = (b[i] - st->bmean) - (s - cnt * c);
-            rs = r;
-            rq = r * r;
+            r = (b[i] - bmean) - (s - cnt * c);
+            rs += r;
+            rq += r * r;
         }
         if (OUT == 1) r32[i] = (float)r;
         if (OUT == 2) r16[brick_row(bk, (x + y) & 1, y) + brick_col(x >> 1)] = f32_to_bf16((float)r);
@@ -505,9 +511,13 @@ __global__ __launch_bounds__(256) void k_rb_blocked(const float* __restrict__ ei
             rr[c][k] = r[row + col];
         }
     }
-    // the stop flag is read after the loads are in flight; a finished solve
-    // leaves the buffers untouched
-    if (st->done) return;
+    // Max-update rule: the stop flag can be set between the launches of one
+    // batch (k_maxcheck_blocked), and a finished solve leaves the buffers
+    // untouched.  Residual rule: only a check sets it, and no sweeps are
+    // enqueued after a check that stopped the solve, so the flag is not read
+    // (reading it first would hold every load behind two dependent scalar
+    // loads).
+    if (TRACK && st->done) return;
     const float rm = *rmean;
     __shared__ float rlds[BK_RV_LDS ? 2 : 1][BK_RV_LDS ? BK_RY : 1][64];
     typename std::conditional<BK_RV_LDS, RhsLds, RhsRegs>::type rv;
@@ -945,21 +955,25 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     if (blocked) CD_TRY(cd_fill_f32(ctx, e32b, npk, 0.f));
 
     const int fblocks = std::min(div_up((long long)n, CD_BLOCK), 65536);
+    // residual launch of a check: kResidualRows rows per thread
+    constexpr int kResidualRows = 8;
+    const dim3 cgrid(div_up(w, RED_BX), div_up(h, RED_BY * kResidualRows));
+    const int ncheck = cgrid.x * cgrid.y;  // <= nred
     auto enqueue_check = [&](int check_id) -> int {
         CD_TRY(ctx->timer_begin());
         if (blocked) {
             hipLaunchKernelGGL(k_flush_packed, rgrid, rblk, 0, s, phi, e32, e32b, sd, w, h, bk);
-            hipLaunchKernelGGL((k_residual<2>), rgrid, rblk, 0, s, phi, b, mask, sd, r32, r16, bk,
-                               w, h, partials);
+            hipLaunchKernelGGL((k_residual<2>), cgrid, rblk, 0, s, phi, b, mask, sd, r32, r16, bk,
+                               w, h, kResidualRows, partials);
         } else if (mixed) {
             hipLaunchKernelGGL(k_flush, dim3(fblocks), dim3(CD_BLOCK), 0, s, phi, e32, n);
-            hipLaunchKernelGGL((k_residual<1>), rgrid, rblk, 0, s, phi, b, mask, sd, r32, r16, bk,
-                               w, h, partials);
+            hipLaunchKernelGGL((k_residual<1>), cgrid, rblk, 0, s, phi, b, mask, sd, r32, r16, bk,
+                               w, h, kResidualRows, partials);
         } else {
-            hipLaunchKernelGGL((k_residual<0>), rgrid, rblk, 0, s, phi, b, mask, sd, r32, r16, bk,
-                               w, h, partials);
+            hipLaunchKernelGGL((k_residual<0>), cgrid, rblk, 0, s, phi, b, mask, sd, r32, r16, bk,
+                               w, h, kResidualRows, partials);
         }
-        hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, partials, nred, sd, count,
+        hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, partials, ncheck, sd, count,
                            check_id, o->stop, o->tol);
         CD_CHECK_LAUNCH(ctx);
         CD_TRY(ctx->timer_end(CD_KERNEL_RESIDUAL, mixed ? 3 : 2,
