@@ -679,10 +679,20 @@ __global__ __launch_bounds__(256) void k_restrict_r16(const unsigned short* __re
     if (lane == 0) rc[i] = acc / (float)ff;
 }
 
+// (mean, rms) of nc coarse values from stats = (sum, sum^2), as every consumer
+// of stats computes them
+__device__ __forceinline__ void coarse_derived(const double* stats, int nc, double* derived) {
+    const double mean = stats[0] / nc;
+    const double var = stats[1] / nc - mean * mean;
+    derived[0] = mean;
+    derived[1] = sqrt(var > 1e-300 ? var : 1e-300);
+}
+
 // (sum, sum^2) of the coarse residual in one workgroup (fixed order), for
-// coarse grids of up to 64Ki cells
+// coarse grids of up to 64Ki cells; derived (may be NULL) receives (mean, rms)
 __global__ __launch_bounds__(1024) void k_coarse_stats1(const float* __restrict__ rc, int nc,
-                                                        double* __restrict__ stats) {
+                                                        double* __restrict__ stats,
+                                                        double* __restrict__ derived) {
     __shared__ double smem[16];
     double a = 0.0, q = 0.0;
     for (int i = threadIdx.x; i < nc; i += blockDim.x) {
@@ -695,7 +705,14 @@ __global__ __launch_bounds__(1024) void k_coarse_stats1(const float* __restrict_
     if (threadIdx.x == 0) {
         stats[0] = a;
         stats[1] = q;
+        if (derived) coarse_derived(stats, nc, derived);
     }
+}
+
+// (mean, rms) after a multi-workgroup reduction of the coarse statistics
+__global__ void k_coarse_derived(const double* __restrict__ stats, int nc,
+                                 double* __restrict__ derived) {
+    if (threadIdx.x == 0) coarse_derived(stats, nc, derived);
 }
 
 // partials (sum, sum^2) of the coarse residual per workgroup of 256 cells
@@ -733,20 +750,19 @@ __device__ __forceinline__ bool packed_thread(const Brick& bk, int w, int Y, int
     return j < bk.wpb && x < w;
 }
 
-// e = rms * (nmax/pi)^2 * bilinear(y) at cell centres, written in the
+// e = rms * (nmax/pi)^2 * bilinear(y) at cell centres (rms = derived[1], one
+// value for the whole launch), written in the
 // bricked layout of the blocked sweeps (the start of the smoothing); one
 // thread per packed element, coalesced writes
 __global__ __launch_bounds__(256) void k_prolong_bricked(const float* __restrict__ y,
-                                                         const double* __restrict__ stats, int nc,
+                                                         const double* __restrict__ derived,
                                                          int w, int h, int f, int wc, int hc,
                                                          double nscale, Brick bk,
                                                          float* __restrict__ e) {
     const int yy = blockIdx.y;
     int c, j, xx;
     if (!packed_thread(bk, w, yy, c, j, xx)) return;
-    const double mean = stats[0] / nc;
-    const double var = stats[1] / nc - mean * mean;
-    const double sig = sqrt(var > 1e-300 ? var : 1e-300);
+    const double sig = derived[1];  // rms of the coarse residual
     const float u = (xx + 0.5f) / f - 0.5f, v = (yy + 0.5f) / f - 0.5f;
     int i0 = (int)floorf(u), j0 = (int)floorf(v);
     const float tu = u - i0, tv = v - j0;
@@ -1081,19 +1097,21 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     }
     // restrict the residual of the last check to (wc x hc) into lc_r; stats
     // slot receives (sum, sum^2) of the coarse residual
-    auto restrict_stats = [&](int f, int wc, int hc, double* stats) -> int {
+    auto restrict_stats = [&](int f, int wc, int hc, double* stats, double* derived) -> int {
         const int nc = wc * hc;
         const int rb = f <= 8 ? div_up(nc, 256) : div_up(nc, 4);  // thread / wave per cell
         hipLaunchKernelGGL(k_restrict_r16, dim3(rb), dim3(256), 0, s, r16, bk, sd, f, wc, hc,
                            ctx->lc_r.as<float>());
         if (nc <= 65536) {
             hipLaunchKernelGGL(k_coarse_stats1, dim3(1), dim3(1024), 0, s, ctx->lc_r.as<float>(),
-                               nc, stats);
+                               nc, stats, derived);
         } else {
             const int nb = div_up(nc, 256);  // <= nred: the partials buffer is large enough
             hipLaunchKernelGGL(k_coarse_stats, dim3(nb), dim3(256), 0, s, ctx->lc_r.as<float>(),
                                nc, partials);
             CD_TRY(cd_reduce_partials(ctx, partials, nb, 2, stats));
+            if (derived)
+                hipLaunchKernelGGL(k_coarse_derived, dim3(1), dim3(64), 0, s, stats, nc, derived);
         }
         CD_CHECK_LAUNCH(ctx);
         return CD_OK;
@@ -1101,7 +1119,7 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     // restrict the residual of the last check to (wc x hc), normalise into
     // xout; stats slot receives (sum, sum^2) of the coarse residual
     auto restrict_normalize = [&](int f, int wc, int hc, float* xout, double* stats) -> int {
-        CD_TRY(restrict_stats(f, wc, hc, stats));
+        CD_TRY(restrict_stats(f, wc, hc, stats, nullptr));
         const int nc = wc * hc;
         hipLaunchKernelGGL(k_normalize_coarse, dim3(div_up(nc, 256)), dim3(256), 0, s,
                            ctx->lc_r.as<float>(), stats, nc, xout);
@@ -1167,12 +1185,13 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     auto fcg_body = [&](int first_sweep) -> int {
         FcgDev* fd = ctx->fcg_dev.as<FcgDev>();
         CD_TRY(ctx->timer_begin());
+        // slot 0: (sum, sum^2) of the coarse residual; slot 1: (mean, rms)
         double* stats = ctx->lc_stats.as<double>();
-        CD_TRY(restrict_stats(lf, lwc, lhc, stats));
+        CD_TRY(restrict_stats(lf, lwc, lhc, stats, stats + 2));
         CD_TRY(cd_net_forward_coarse(ctx->net, ctx->lc_r.as<float>(), stats,
                                      ctx->lc_y.as<float>(), lhc, lwc));
         hipLaunchKernelGGL(k_prolong_bricked, packed_grid, dim3(256), 0, s,
-                           ctx->lc_y.as<float>(), stats, lwc * lhc, w, h, lf, lwc, lhc, nscale, bk,
+                           ctx->lc_y.as<float>(), stats + 2, w, h, lf, lwc, lhc, nscale, bk,
                            e32);
         CD_CHECK_LAUNCH(ctx);
         CD_TRY(ctx->timer_end(CD_KERNEL_LEARNED, 4, 0.0));
