@@ -303,6 +303,12 @@ int cd_net_get_params(cd_net* net, float* host, size_t n);
 /* Forward pass on a device batch: x, y are [batch][h][w] FP32 (normalised
  * coarse residual in, normalised coarse correction out). */
 int cd_net_forward(cd_net* net, const float* d_x, float* d_y, int batch, int h, int w);
+/* Inference forwards (cd_net_forward and the solver's coarse solve) read the
+ * upsampled U-Net decoder inputs through the convolutions that consume them
+ * (on != 0, the default) or materialise them first, as training does (0).
+ * Both give bit-identical outputs; the switch exists to check that.  No
+ * counterpart in the reference (the network is new). */
+int cd_net_set_fused_forward(cd_net* net, int on);
 /* Mean squared error of the forward pass against d_t and its gradient with
  * respect to every parameter (FP32, host order of cd_net_get_params). */
 int cd_net_loss_grad(cd_net* net, const float* d_x, const float* d_t, int batch, int h, int w,

@@ -60,7 +60,8 @@ __global__ __launch_bounds__(256) void k_conv3x3(const u16* __restrict__ x1,
                                                  const u16* __restrict__ wt, int kpad, int B,
                                                  int H, int W, u16* __restrict__ ybf,
                                                  float* __restrict__ y1, float* __restrict__ y2,
-                                                 int cy, int cy2, int relu, int accumulate) {
+                                                 int cy, int cy2, int relu, int accumulate,
+                                                 int mode1) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const long long npix = (long long)B * H * W;
     const long long p0 = (long long)blockIdx.x * 64 + wave * 16;
@@ -91,8 +92,16 @@ __global__ __launch_bounds__(256) void k_conv3x3(const u16* __restrict__ x1,
             if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
                 const long long q = ((long long)bb * H + yy) * W + xx;
                 if (IN == 0) {
-                    const u16* src = ci < c1 ? x1 + q * c1 + ci : x2 + q * c2 + (ci - c1);
-                    a.u = *reinterpret_cast<const uint4*>(src);
+                    if (ci >= c1) {
+                        a.u = *reinterpret_cast<const uint4*>(x2 + q * c2 + (ci - c1));
+                    } else if (mode1 == 0) {
+                        a.u = *reinterpret_cast<const uint4*>(x1 + q * c1 + ci);
+                    } else {
+                        // x1 holds [B][H/2][W/2][c1]: nearest 2x upsampling
+                        const long long q1 =
+                            ((long long)bb * (H >> 1) + (yy >> 1)) * (W >> 1) + (xx >> 1);
+                        a.u = *reinterpret_cast<const uint4*>(x1 + q1 * c1 + ci);
+                    }
                 } else {
                     const float4 g0 = *reinterpret_cast<const float4*>(g + q * c1 + ci);
                     const float4 g1 = *reinterpret_cast<const float4*>(g + q * c1 + ci + 4);
@@ -536,28 +545,33 @@ struct cd_net {
     std::vector<NetTensor> t;
     int batch = 0, bh = 0, bw = 0;
     bool weights_dirty = true;
+    bool fused_forward = true;  // inference forwards fold the decoder upsampling into the convs
 };
 
 namespace {
 
-int launch_conv(cd_net* n, const NetConv& cv, int B, int H, int W, bool grad_input) {
+// mode1: 0 reads in1 as stored; 1 reads `x1src` (the half-resolution tensor
+// in1 is upsampled from) through nearest 2x upsampling instead
+int launch_conv(cd_net* n, const NetConv& cv, int B, int H, int W, bool grad_input,
+                int mode1 = 0, const u16* x1src = nullptr) {
     cd_context* c = n->ctx;
     hipStream_t s = c->stream;
     const long long npix = (long long)B * H * W;
     const int blocks = div_up(npix, 64);
     if (!grad_input) {
-        const u16* x1 = cv.in1 < 0 ? n->x8.as<u16>() : n->t[cv.in1].act.as<u16>();
+        const u16* x1 = mode1 ? x1src
+                              : (cv.in1 < 0 ? n->x8.as<u16>() : n->t[cv.in1].act.as<u16>());
         const u16* x2 = cv.in2 < 0 ? nullptr : n->t[cv.in2].act.as<u16>();
         const u16* wt = n->wbf.as<u16>() + cv.woff;
         const int nb = cv.coutpad / 16;
         if (cv.out < 0) {  // head: FP32 output [npix][8]
-#define CONV_F(NB) hipLaunchKernelGGL((k_conv3x3<NB, 0, 1>), dim3(blocks), dim3(256), 0, s, x1, x2, cv.c1, cv.c2, (const float*)nullptr, (const u16*)nullptr, wt, cv.kpad, B, H, W, (u16*)nullptr, n->head.as<float>(), (float*)nullptr, 8, 0, 0, 0)
+#define CONV_F(NB) hipLaunchKernelGGL((k_conv3x3<NB, 0, 1>), dim3(blocks), dim3(256), 0, s, x1, x2, cv.c1, cv.c2, (const float*)nullptr, (const u16*)nullptr, wt, cv.kpad, B, H, W, (u16*)nullptr, n->head.as<float>(), (float*)nullptr, 8, 0, 0, 0, mode1)
             if (nb == 1) CONV_F(1);
             else return CD_ERR_BAD_ARG;
 #undef CONV_F
         } else {
             u16* y = n->t[cv.out].act.as<u16>();
-#define CONV_B(NB) hipLaunchKernelGGL((k_conv3x3<NB, 0, 0>), dim3(blocks), dim3(256), 0, s, x1, x2, cv.c1, cv.c2, (const float*)nullptr, (const u16*)nullptr, wt, cv.kpad, B, H, W, y, (float*)nullptr, (float*)nullptr, cv.coutA, 0, cv.relu, 0)
+#define CONV_B(NB) hipLaunchKernelGGL((k_conv3x3<NB, 0, 0>), dim3(blocks), dim3(256), 0, s, x1, x2, cv.c1, cv.c2, (const float*)nullptr, (const u16*)nullptr, wt, cv.kpad, B, H, W, y, (float*)nullptr, (float*)nullptr, cv.coutA, 0, cv.relu, 0, mode1)
             switch (nb) {
                 case 1: CONV_B(1); break;
                 case 2: CONV_B(2); break;
@@ -577,7 +591,7 @@ int launch_conv(cd_net* n, const NetConv& cv, int B, int H, int W, bool grad_inp
         float* d1 = n->t[cv.in1].grad.as<float>();
         float* d2 = cv.in2 < 0 ? nullptr : n->t[cv.in2].grad.as<float>();
         const int nb = cv.cinpad / 16;
-#define CONV_G(NB) hipLaunchKernelGGL((k_conv3x3<NB, 1, 1>), dim3(blocks), dim3(256), 0, s, (const u16*)nullptr, (const u16*)nullptr, cv.coutA, 0, g, mask, wt, cv.kpadT, B, H, W, (u16*)nullptr, d1, d2, cv.c1, cv.c2, 0, 1)
+#define CONV_G(NB) hipLaunchKernelGGL((k_conv3x3<NB, 1, 1>), dim3(blocks), dim3(256), 0, s, (const u16*)nullptr, (const u16*)nullptr, cv.coutA, 0, g, mask, wt, cv.kpadT, B, H, W, (u16*)nullptr, d1, d2, cv.c1, cv.c2, 0, 1, 0)
         switch (nb) {
             case 1: CONV_G(1); break;
             case 2: CONV_G(2); break;
@@ -728,15 +742,24 @@ int refresh_weights(cd_net* n) {
 }
 
 // forward pass from the 8-channel input already in n->x8; head output in n->head
-int forward(cd_net* n, int B, int H, int W) {
+// fused (inference): the upsampled decoder inputs are read through the
+// convolutions that consume them (a pure index remap) instead of being
+// materialised; the output is bit-identical.  Training needs the materialised
+// tensors (weight gradients read them).  Pooling stays a separate pass: read
+// through the convolution (mode 2) it costs more than it saves.
+int forward(cd_net* n, int B, int H, int W, bool fused = false) {
     cd_context* c = n->ctx;
     CD_TRY(refresh_weights(n));
+    const int L = n->arch.levels;
     for (const NetConv& cv : n->convs) {
         const int Hl = H >> cv.level, Wl = W >> cv.level;
-        CD_TRY(launch_conv(n, cv, B, Hl, Wl, false));
-        // after the second conv of an encoder level below the bottleneck: pool
         const int idx = (int)(&cv - &n->convs[0]);
-        const int L = n->arch.levels;
+        if (fused && cv.in2 >= 0) {
+            CD_TRY(launch_conv(n, cv, B, Hl, Wl, false, 1, n->t[n->convs[idx - 1].out].act.as<u16>()));
+        } else {
+            CD_TRY(launch_conv(n, cv, B, Hl, Wl, false));
+        }
+        // after the second conv of an encoder level below the bottleneck: pool
         if (idx < 2 * L && idx % 2 == 1 && cv.level < L - 1) {
             const NetConv& nx = n->convs[idx + 1];  // next level's first conv reads the pooled tensor
             const long long cnt = (long long)B * (Hl / 2) * (Wl / 2) * n->t[nx.in1].C;
@@ -745,7 +768,7 @@ int forward(cd_net* n, int B, int H, int W) {
                                n->t[nx.in1].C);
         }
         // before a decoder concat conv: upsample the previous decoder output
-        if (idx + 1 < (int)n->convs.size()) {
+        if (!fused && idx + 1 < (int)n->convs.size()) {
             const NetConv& nx = n->convs[idx + 1];
             if (nx.in2 >= 0) {
                 const int Hn = H >> nx.level, Wn = W >> nx.level;
@@ -943,14 +966,14 @@ int cd_net_forward_coarse(cd_net* n, const float* rc, const double* stats, float
     hipLaunchKernelGGL(k_prep_normalized, dim3(div_up(h * w, 256)), dim3(256), 0, n->ctx->stream,
                        rc, stats, h * w, n->x8.as<u16>());
     CD_CHECK_LAUNCH(n->ctx);
-    CD_TRY(forward(n, 1, h, w));
+    CD_TRY(forward(n, 1, h, w, n->fused_forward));
     return finish_output(n, 1, h, w, d_y);
 }
 
 // solver-side entry: forward on one coarse field (batch 1), output FP32 [h][w]
 int cd_net_forward_internal(cd_net* n, const float* d_x, float* d_y, int h, int w) {
     CD_TRY(prep(n, d_x, 1, h, w));
-    CD_TRY(forward(n, 1, h, w));
+    CD_TRY(forward(n, 1, h, w, n->fused_forward));
     return finish_output(n, 1, h, w, d_y);
 }
 
@@ -1088,10 +1111,16 @@ int cd_net_forward(cd_net* n, const float* d_x, float* d_y, int batch, int h, in
     if (!n->ctx) return CD_ERR_STATE;
     CD_HIP(n->ctx, hipSetDevice(n->ctx->device));
     CD_TRY(prep(n, d_x, batch, h, w));
-    CD_TRY(forward(n, batch, h, w));
-    const long long px = (long long)batch * h * w;
+    CD_TRY(forward(n, batch, h, w, n->fused_forward));
     CD_TRY(finish_output(n, batch, h, w, d_y));
     CD_HIP(n->ctx, hipStreamSynchronize(n->ctx->stream));
+    return CD_OK;
+}
+
+int cd_net_set_fused_forward(cd_net* n, int on) {
+    if (!n) return CD_ERR_BAD_ARG;
+    if (!n->ctx) return CD_ERR_STATE;
+    n->fused_forward = on != 0;
     return CD_OK;
 }
 

@@ -39,6 +39,26 @@ def test_net_forward_matches_netref(ctx, linear):
 
 
 @pytest.mark.parametrize("linear", [1, 0])
+def test_fused_forward_is_bit_identical(ctx, linear):
+    """Upsampled decoder inputs read through the convolutions (inference)
+    give exactly the output of the materialised tensors (training)."""
+    arch = capi.default_arch(linear=linear, channels=(16, 32, 32, 32, 0, 0))
+    net = capi.Net(ctx, arch)
+    net.set_params(NetRef(4, (16, 32, 32, 32), linear=bool(linear)).random_params(3))
+    x = np.random.default_rng(2).standard_normal((2, 32, 40))
+    dx = dev(x)
+    ys = []
+    for fused in (True, False):
+        net.set_fused_forward(fused)
+        dy = torch.zeros((2, 32, 40), device="cuda:0")
+        torch.cuda.synchronize()
+        net.forward(dx.data_ptr(), dy.data_ptr(), 2, 32, 40)
+        ys.append(dy.cpu().numpy())
+    assert np.abs(ys[0]).max() > 0
+    assert np.array_equal(ys[0], ys[1])
+
+
+@pytest.mark.parametrize("linear", [1, 0])
 def test_net_gradient_matches_netref(ctx, linear):
     net = capi.Net(ctx, small_arch(linear))
     ref = NetRef(3, (8, 16, 16), linear=bool(linear))
