@@ -141,10 +141,17 @@ __global__ __launch_bounds__(256) void k_mesh_init(double* __restrict__ txy,
 
 // image integral over quad slot k of vertex v (polygon_utils.cpp:311-366):
 // one thread per (vertex, adjacent-triangle slot)
-__global__ __launch_bounds__(256) void k_quad_integrals(const double* __restrict__ pixels,
-                                                        int nx, int ny, double W,
-                                                        const double* __restrict__ xy, int mx,
-                                                        int my, double* __restrict__ quadacc) {
+// One thread per (vertex, adjacent-quad slot), 64 threads per workgroup; the
+// clipping work polygons live in LDS (one column per lane) instead of
+// per-thread scratch memory.
+constexpr int kQuadBlock = 64;
+__global__ __launch_bounds__(kQuadBlock) void k_quad_integrals(const double* __restrict__ pixels,
+                                                               int nx, int ny, double W,
+                                                               const double* __restrict__ xy,
+                                                               int mx, int my,
+                                                               double* __restrict__ quadacc) {
+    __shared__ double clipbuf[4 * CLIP_MAX * kQuadBlock];
+    double* buf = clipbuf + threadIdx.x;
     const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long V = (long long)mx * my;
     if (gid >= 6 * V) return;
@@ -167,7 +174,8 @@ __global__ __launch_bounds__(256) void k_quad_integrals(const double* __restrict
         for (int py = (int)ylo; py < yhi; ++py)
             for (int pxi = (int)xlo; pxi < xhi; ++pxi) {
                 const double cx = ((double)pxi + 0.5) * px, cy = ((double)py + 0.5) * px;
-                acc += quad_pixel_area(q, cx, cy, px) * pixels[(size_t)py * nx + pxi];
+                acc += quad_pixel_area_s<kQuadBlock>(q, cx, cy, px, buf) *
+                       pixels[(size_t)py * nx + pxi];
             }
     }
     quadacc[gid] = acc;
@@ -764,7 +772,7 @@ int design_init_from_device_image(cd_design* d, const double* img) {
                        d->txy.as<double>(), d->sxy.as<double>(), d->sz.as<double>(), d->mx,
                        d->my, d->p.width, d->p.height);
     // target areas over the initial dual cells (caustic_design.cpp:341-348)
-    hipLaunchKernelGGL(k_quad_integrals, dim3(div_up(6LL * V, CD_BLOCK)), dim3(CD_BLOCK), 0, s,
+    hipLaunchKernelGGL(k_quad_integrals, dim3(div_up(6LL * V, kQuadBlock)), dim3(kQuadBlock), 0, s,
                        d->pixels.as<double>(), d->nx, d->ny, d->p.width, d->txy.as<double>(),
                        d->mx, d->my, d->quadacc.as<double>());
     hipLaunchKernelGGL(k_cell_sums, dim3(div_up(V, CD_BLOCK)), dim3(CD_BLOCK), 0, s,

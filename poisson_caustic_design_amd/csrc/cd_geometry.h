@@ -132,55 +132,84 @@ __device__ __forceinline__ bool crossing(double ax, double ay, double bx, double
 constexpr int CLIP_MAX = 12;
 
 // Clip polygon (px,py; n vertices) against the directed edge a->b keeping the
-// side `dir`; output into (qx,qy); returns the output count.
-__device__ __forceinline__ int clip_edge(const double* px, const double* py, int n, double ax,
-                                         double ay, double bx, double by, int dir, double* qx,
-                                         double* qy) {
+// side `dir`; output into (qx,qy); returns the output count.  Vertex i of a
+// polygon is stored at index i * S (S = 1: a private array; S = 64: one
+// column per lane of a 64-wide LDS block, bank-conflict free).
+template <int S>
+__device__ __forceinline__ int clip_edge_s(const double* px, const double* py, int n, double ax,
+                                           double ay, double bx, double by, int dir, double* qx,
+                                           double* qy) {
     int m = 0;
     if (n == 0) return 0;
-    double sx = px[n - 1], sy = py[n - 1];
+    double sx = px[(n - 1) * S], sy = py[(n - 1) * S];
     int ss = side_of(ax, ay, bx, by, sx, sy);
     for (int i = 0; i < n; ++i) {
-        const double ex = px[i], ey = py[i];
+        const double ex = px[i * S], ey = py[i * S];
         const int se = side_of(ax, ay, bx, by, ex, ey);
-        if (ss != -dir && m < CLIP_MAX) { qx[m] = sx; qy[m] = sy; ++m; }
+        if (ss != -dir && m < CLIP_MAX) { qx[m * S] = sx; qy[m * S] = sy; ++m; }
         if (ss + se == 0 && ss != 0 && m < CLIP_MAX) {
             double ox, oy;
-            if (crossing(ax, ay, bx, by, sx, sy, ex, ey, ox, oy)) { qx[m] = ox; qy[m] = oy; ++m; }
+            if (crossing(ax, ay, bx, by, sx, sy, ex, ey, ox, oy)) {
+                qx[m * S] = ox;
+                qy[m * S] = oy;
+                ++m;
+            }
         }
         sx = ex; sy = ey; ss = se;
     }
     return m;
 }
 
-__device__ __forceinline__ double shoelace_n(const double* x, const double* y, int n) {
+template <int S>
+__device__ __forceinline__ double shoelace_s(const double* x, const double* y, int n) {
     if (n < 3) return 0.0;
     double s = 0.0;
     for (int i = 0; i < n; ++i) {
         const int k = (i + 1 == n) ? 0 : i + 1;
-        s += x[i] * y[k] - x[k] * y[i];
+        s += x[i * S] * y[k * S] - x[k * S] * y[i * S];
     }
     return 0.5 * s;
 }
 
+__device__ __forceinline__ int clip_edge(const double* px, const double* py, int n, double ax,
+                                         double ay, double bx, double by, int dir, double* qx,
+                                         double* qy) {
+    return clip_edge_s<1>(px, py, n, ax, ay, bx, by, dir, qx, qy);
+}
+
+__device__ __forceinline__ double shoelace_n(const double* x, const double* y, int n) {
+    return shoelace_s<1>(x, y, n);
+}
+
 // Signed area of quad q (4 vertices) intersected with the pixel square of
 // side px centred at (cx, cy), square vertices ordered as
-// polygon_utils.cpp:336-346.
-__device__ __forceinline__ double quad_pixel_area(const double* q, double cx, double cy,
-                                                  double px) {
+// polygon_utils.cpp:336-346.  buf holds the four work polygons (ax, ay, bx,
+// by; CLIP_MAX vertices each, stride S).
+template <int S>
+__device__ __forceinline__ double quad_pixel_area_s(const double* q, double cx, double cy,
+                                                    double px, double* buf) {
     const double sx[4] = {cx - px / 2.0, cx - px / 2.0, cx + px / 2.0, cx + px / 2.0};
     const double sy[4] = {cy - px / 2.0, cy + px / 2.0, cy + px / 2.0, cy - px / 2.0};
     const int dir = side_of(sx[0], sy[0], sx[1], sy[1], sx[2], sy[2]);
-    double ax[CLIP_MAX], ay[CLIP_MAX], bx[CLIP_MAX], by[CLIP_MAX];
-    for (int i = 0; i < 4; ++i) { ax[i] = q[2 * i]; ay[i] = q[2 * i + 1]; }
-    int n = clip_edge(ax, ay, 4, sx[3], sy[3], sx[0], sy[0], dir, bx, by);
+    double* ax = buf;
+    double* ay = buf + CLIP_MAX * S;
+    double* bx = buf + 2 * CLIP_MAX * S;
+    double* by = buf + 3 * CLIP_MAX * S;
+    for (int i = 0; i < 4; ++i) { ax[i * S] = q[2 * i]; ay[i * S] = q[2 * i + 1]; }
+    int n = clip_edge_s<S>(ax, ay, 4, sx[3], sy[3], sx[0], sy[0], dir, bx, by);
     if (n == 0) return 0.0;
-    n = clip_edge(bx, by, n, sx[0], sy[0], sx[1], sy[1], dir, ax, ay);
+    n = clip_edge_s<S>(bx, by, n, sx[0], sy[0], sx[1], sy[1], dir, ax, ay);
     if (n == 0) return 0.0;
-    n = clip_edge(ax, ay, n, sx[1], sy[1], sx[2], sy[2], dir, bx, by);
+    n = clip_edge_s<S>(ax, ay, n, sx[1], sy[1], sx[2], sy[2], dir, bx, by);
     if (n == 0) return 0.0;
-    n = clip_edge(bx, by, n, sx[2], sy[2], sx[3], sy[3], dir, ax, ay);
-    return shoelace_n(ax, ay, n);
+    n = clip_edge_s<S>(bx, by, n, sx[2], sy[2], sx[3], sy[3], dir, ax, ay);
+    return shoelace_s<S>(ax, ay, n);
+}
+
+__device__ __forceinline__ double quad_pixel_area(const double* q, double cx, double cy,
+                                                  double px) {
+    double buf[4 * CLIP_MAX];
+    return quad_pixel_area_s<1>(q, cx, cy, px, buf);
 }
 
 }  // namespace cdgeo
