@@ -291,6 +291,10 @@ typedef struct {
     int out_smooth;    /* smoothing steps y += Lc y / 8 on the output (0)    */
     int loss_smooth;   /* the same steps on the operator-loss residual (0):  */
                        /* weights it towards the low frequencies             */
+    int loss_energy;   /* 1: train on the energy J(y) = y'(-cs Lc)y/2 + x'y  */
+                       /* (same minimiser cs Lc y = x; errors weighted by Lc  */
+                       /* once instead of twice, so smooth error counts more; */
+                       /* loss_smooth is then unused) (0)                    */
 } cd_net_arch;
 
 void cd_net_arch_default(cd_net_arch* a);

@@ -35,12 +35,13 @@ class Conv:
 
 class NetRef:
     def __init__(self, levels=4, channels=(16, 32, 64, 64), linear=False, out_smooth=0,
-                 loss_smooth=0):
+                 loss_smooth=0, loss_energy=False):
         self.L = levels
         self.C = list(channels)[:levels]
         self.act = 0 if linear else 1
         self.out_smooth = out_smooth    # y = S^k head, S = I + Lc/8
         self.loss_smooth = loss_smooth  # operator loss on S^k (cs Lc y - x)
+        self.loss_energy = loss_energy  # energy y'(-cs Lc)y/2 + x'y instead
         self.convs: list[Conv] = []
         self.tensors: list[tuple[int, int]] = []  # (level, channels)
         self.nparam = 0
@@ -221,10 +222,15 @@ class NetRef:
 
     def loss_grad_operator(self, params, x, emulate_bf16=False):
         """Operator loss mean((S^k (cs * Lc y - x))^2), cs = (max(H, W)/pi)^2,
+        (or, with loss_energy, the energy mean(y (x - u) / 2), u = cs Lc y - x),
         k = loss_smooth, and its parameter gradient (cs Lc S^k (2 v / n) into
         y; Lc and S are symmetric)."""
         y, cache = self.forward(params, x, emulate_bf16)
         cs = (max(x.shape[1], x.shape[2]) / np.pi) ** 2
+        if self.loss_energy:
+            # J = mean(y (x - u) / 2), u = cs Lc y - x; dJ/dy = -u / n
+            u = cs * self.lap(y) - x
+            return float(np.mean(0.5 * y * (x - u))), self._backward(cache, -u / y.size)
         v = self.smooth(cs * self.lap(y) - x, self.loss_smooth)
         loss = float(np.mean(v ** 2))
         dv = 2.0 * v / y.size

@@ -441,6 +441,25 @@ __global__ __launch_bounds__(256) void k_sumsq(float* __restrict__ v, long long 
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
+// energy loss per pixel: with u = cs Lc y - x, J = y'(-cs Lc)y/2 + x'y has
+// pixel contributions y (x - u) / 2 and dJ/dy = -u; partial sums of the
+// contributions, gy = -u / npix when want_grad
+__global__ __launch_bounds__(256) void k_energy(const float* __restrict__ y,
+                                                const float* __restrict__ x,
+                                                const float* __restrict__ u,
+                                                float* __restrict__ gy, long long npix,
+                                                int want_grad, double* __restrict__ partials) {
+    __shared__ double smem[16];
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0.0;
+    if (p < npix) {
+        e = 0.5 * (double)y[p] * ((double)x[p] - (double)u[p]);
+        if (want_grad) gy[p] = -u[p] / (float)npix;
+    }
+    const double s = cd_block_sum(e, smem);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
 // gy = cs * Lc du (Lc symmetric)
 __global__ __launch_bounds__(256) void k_op_grad(const float* __restrict__ du,
                                                  float* __restrict__ gy, int B, int H, int W,
@@ -908,7 +927,8 @@ inline float op_scale(int H, int W) {
 }
 
 // operator loss: u = cs Lc y - x, v = S^loss_smooth u, loss = mean(v^2);
-// dL/dy = cs Lc S^loss_smooth (2 v / npix)
+// dL/dy = cs Lc S^loss_smooth (2 v / npix).  With arch.loss_energy the loss is
+// the energy mean(y (x - u) / 2) and dL/dy = -u / npix.
 int op_loss_and_head_grad(cd_net* n, const float* d_x, int B, int H, int W, double* loss,
                           bool want_grad) {
     cd_context* c = n->ctx;
@@ -919,6 +939,14 @@ int op_loss_and_head_grad(cd_net* n, const float* d_x, int B, int H, int W, doub
     hipLaunchKernelGGL(k_op_residual, dim3(nb), dim3(256), 0, c->stream, n->yout.as<float>(), d_x,
                        n->ures.as<float>(), B, H, W, cs);
     CD_CHECK_LAUNCH(c);
+    if (n->arch.loss_energy) {
+        hipLaunchKernelGGL(k_energy, dim3(nb), dim3(256), 0, c->stream, n->yout.as<float>(), d_x,
+                           n->ures.as<float>(), n->gy.as<float>(), px, want_grad ? 1 : 0,
+                           n->partials.as<double>());
+        CD_CHECK_LAUNCH(c);
+        if (want_grad) CD_TRY(head_grad(n, B, H, W));
+        return read_loss(n, nb, px, loss);
+    }
     CD_TRY(smooth_steps(n, n->ures, n->utmp, n->arch.loss_smooth, B, H, W));
     hipLaunchKernelGGL(k_sumsq, dim3(nb), dim3(256), 0, c->stream, n->ures.as<float>(), px,
                        want_grad ? 1 : 0, n->partials.as<double>());
@@ -1007,6 +1035,7 @@ void cd_net_arch_default(cd_net_arch* a) {
     a->linear = 1;
     a->out_smooth = 0;
     a->loss_smooth = 0;
+    a->loss_energy = 0;
 }
 
 int cd_net_create(cd_context* ctx, const cd_net_arch* arch, cd_net** out) {
@@ -1014,7 +1043,7 @@ int cd_net_create(cd_context* ctx, const cd_net_arch* arch, cd_net** out) {
     *out = nullptr;
     if (arch->levels < 2 || arch->levels > 6 || arch->coarsen < 1) return CD_ERR_BAD_ARG;
     if (arch->out_smooth < 0 || arch->out_smooth > 64 || arch->loss_smooth < 0 ||
-        arch->loss_smooth > 64)
+        arch->loss_smooth > 64 || arch->loss_energy < 0 || arch->loss_energy > 1)
         return CD_ERR_BAD_ARG;
     for (int l = 0; l < arch->levels; ++l)
         if (arch->channels[l] <= 0 || arch->channels[l] % 8 || arch->channels[l] > 128)

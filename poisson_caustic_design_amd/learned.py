@@ -265,7 +265,7 @@ def save_params(net: capi.Net, path: str, meta: dict):
                         channels=np.array(list(a.channels)), coarsen=a.coarsen,
                         smooth_sweeps=a.smooth_sweeps, smooth_omega=a.smooth_omega,
                         linear=a.linear, out_smooth=a.out_smooth, loss_smooth=a.loss_smooth,
-                        meta=json.dumps(meta))
+                        loss_energy=a.loss_energy, meta=json.dumps(meta))
 
 
 def load_net(ctx: capi.Context, path: str) -> capi.Net:
@@ -273,7 +273,8 @@ def load_net(ctx: capi.Context, path: str) -> capi.Net:
     arch = capi.default_arch(levels=int(d["levels"]), channels=[int(c) for c in d["channels"]],
                              coarsen=int(d["coarsen"]), smooth_sweeps=int(d["smooth_sweeps"]),
                              smooth_omega=float(d["smooth_omega"]), linear=int(d["linear"]),
-                             out_smooth=int(d["out_smooth"]), loss_smooth=int(d["loss_smooth"]))
+                             out_smooth=int(d["out_smooth"]), loss_smooth=int(d["loss_smooth"]),
+                             loss_energy=int(d["loss_energy"]) if "loss_energy" in d.files else 0)
     net = capi.Net(ctx, arch)
     net.set_params(d["params"])
     return net
@@ -292,6 +293,8 @@ def main(argv=None):
     ap.add_argument("--relu", action="store_true")
     ap.add_argument("--out-smooth", type=int, default=6)
     ap.add_argument("--loss-smooth", type=int, default=0)
+    ap.add_argument("--loss", choices=["operator", "energy"], default="operator",
+                    help="operator: mean((cs Lc y - x)^2); energy: y'(-cs Lc)y/2 + x'y")
     ap.add_argument("--coarsen", type=int, default=16)
     ap.add_argument("--sweeps", type=int, default=8, help="smoothing sweeps per FCG iteration")
     ap.add_argument("--omega", type=float, default=1.5)
@@ -329,6 +332,7 @@ def main(argv=None):
     arch = capi.default_arch(levels=a.levels, channels=[int(c) for c in a.channels.split(",")],
                              linear=0 if a.relu else 1, out_smooth=a.out_smooth,
                              loss_smooth=a.loss_smooth, coarsen=a.coarsen,
+                             loss_energy=int(a.loss == "energy"),
                              smooth_sweeps=a.sweeps, smooth_omega=a.omega)
     t0 = time.time()
     net, losses, pool = fit(ctx, arch, a.mesh_res, range(a.seed0, a.seed0 + a.designs), a.config,
@@ -338,7 +342,7 @@ def main(argv=None):
     os.makedirs(os.path.dirname(out), exist_ok=True)
     meta = {"mesh_res": a.mesh_res, "designs": a.designs, "seed0": a.seed0, "config": a.config,
             "inputs": int(pool.shape[0]), "steps": a.steps, "rounds": a.rounds,
-            "loss": "operator", "final_loss": float(np.mean(losses[-100:])),
+            "loss": a.loss, "final_loss": float(np.mean(losses[-100:])),
             "train_seconds": round(time.time() - t0, 1)}
     save_params(net, out, meta)
     print(json.dumps({"out": out, **meta, "loss_first": float(np.mean(losses[:50]))}), flush=True)

@@ -77,11 +77,13 @@ def test_net_gradient_matches_netref(ctx, linear):
     assert cos > 0.998
 
 
-@pytest.mark.parametrize("linear,out_smooth,loss_smooth", [(1, 0, 0), (0, 0, 0), (1, 2, 3)])
-def test_net_operator_gradient_matches_netref(ctx, linear, out_smooth, loss_smooth):
-    net = capi.Net(ctx, small_arch(linear, out_smooth=out_smooth, loss_smooth=loss_smooth))
+@pytest.mark.parametrize("linear,out_smooth,loss_smooth,energy",
+                         [(1, 0, 0, 0), (0, 0, 0, 0), (1, 2, 3, 0), (1, 2, 0, 1), (0, 0, 0, 1)])
+def test_net_operator_gradient_matches_netref(ctx, linear, out_smooth, loss_smooth, energy):
+    net = capi.Net(ctx, small_arch(linear, out_smooth=out_smooth, loss_smooth=loss_smooth,
+                                   loss_energy=energy))
     ref = NetRef(3, (8, 16, 16), linear=bool(linear), out_smooth=out_smooth,
-                 loss_smooth=loss_smooth)
+                 loss_smooth=loss_smooth, loss_energy=bool(energy))
     net.set_params(ref.random_params(9))
     p = net.get_params()
     rng = np.random.default_rng(5)

@@ -65,11 +65,12 @@ def test_bf16_rounding():
     assert r[2] == 1.0 and r[3] == 1.0 + 2 ** -7 and r[4] == -2.5
 
 
-@pytest.mark.parametrize("out_smooth,loss_smooth", [(0, 0), (2, 3)])
-def test_netref_operator_gradient_finite_differences(out_smooth, loss_smooth):
+@pytest.mark.parametrize("out_smooth,loss_smooth,energy", [(0, 0, False), (2, 3, False),
+                                                           (2, 0, True)])
+def test_netref_operator_gradient_finite_differences(out_smooth, loss_smooth, energy):
     for linear in (True, False):
         net = NetRef(levels=3, channels=(8, 16, 16), linear=linear, out_smooth=out_smooth,
-                     loss_smooth=loss_smooth)
+                     loss_smooth=loss_smooth, loss_energy=energy)
         p = net.random_params(4).astype(np.float64)
         rng = np.random.default_rng(2)
         x = rng.standard_normal((2, 8, 8))
@@ -98,6 +99,28 @@ def test_netref_operator_matches_independent_stencil_and_exact_inverse():
     y = dct_solve(x) / cs
     u = cs * NetRef.lap(y[None])[0] - x
     assert np.abs(u).max() < 1e-9
+
+
+def test_energy_loss_is_minimal_at_the_inverse_operator():
+    """J(y) = mean(y (x - u) / 2), u = cs Lc y - x, is the energy
+    y'(-cs Lc)y/2 + x'y per pixel: J(y* + d) - J(y*) = mean(d (-cs Lc) d) / 2
+    >= 0 at y* = Lc^{-1} x / cs (exact DCT inverse), for any d."""
+    from tests.helpers import dct_solve
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal((12, 20))
+    x -= x.mean()
+    cs = (20 / np.pi) ** 2
+
+    def J(y):
+        u = cs * NetRef.lap(y[None])[0] - x
+        return np.mean(0.5 * y * (x - u))
+
+    ys = dct_solve(x) / cs
+    for k in range(5):
+        d = rng.standard_normal((12, 20)) * 10.0 ** -k
+        expect = np.mean(0.5 * d * (-cs * NetRef.lap(d[None])[0]))
+        assert expect > 0
+        assert J(ys + d) - J(ys) == pytest.approx(expect, rel=1e-6, abs=1e-12)
 
 
 def test_linear_netref_is_linear():
