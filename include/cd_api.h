@@ -295,6 +295,11 @@ typedef struct {
                        /* (same minimiser cs Lc y = x; errors weighted by Lc  */
                        /* once instead of twice, so smooth error counts more; */
                        /* loss_smooth is then unused) (0)                    */
+    int mid_sweeps;    /* > 0 with coarsen a multiple of 4, >= 8: the coarse  */
+                       /* correction is prolonged to an intermediate grid     */
+                       /* (coarsen/4 fine cells per cell), smoothed there by  */
+                       /* this many red-black sweeps on the restricted        */
+                       /* residual, then prolonged to the fine grid (0)       */
 } cd_net_arch;
 
 void cd_net_arch_default(cd_net_arch* a);

@@ -75,7 +75,7 @@ class RunStats(C.Structure):
 class NetArch(C.Structure):
     _fields_ = [("levels", _I), ("channels", _I * 6), ("coarsen", _I), ("smooth_sweeps", _I),
                 ("smooth_omega", _D), ("linear", _I), ("out_smooth", _I), ("loss_smooth", _I),
-                ("loss_energy", _I)]
+                ("loss_energy", _I), ("mid_sweeps", _I)]
 
 
 class CDError(RuntimeError):

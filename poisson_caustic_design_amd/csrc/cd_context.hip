@@ -199,6 +199,10 @@ void cd_context_destroy(cd_context* ctx) {
     ctx->fcg_p.release();
     ctx->fcg_q.release();
     ctx->fcg_dev.release();
+    ctx->fcg_em.release();
+    ctx->fcg_emb.release();
+    ctx->fcg_rm.release();
+    ctx->fcg_zero.release();
     ctx->cap_phi.release();
     if (ctx->host_status) (void)hipHostFree(ctx->host_status);
     for (auto& p : ctx->pending) {

@@ -103,12 +103,12 @@ struct cd_context {
     bool graphs = true;
     hipGraphExec_t fcg_exec = nullptr;
     struct FcgKey {
-        int w = 0, h = 0, nsmooth = 0;
+        int w = 0, h = 0, nsmooth = 0, mid = 0;
         const void* net = nullptr;
         unsigned long long cgen = 0, ngen = 0;
         bool operator==(const FcgKey& o) const {
-            return w == o.w && h == o.h && nsmooth == o.nsmooth && net == o.net &&
-                   cgen == o.cgen && ngen == o.ngen;
+            return w == o.w && h == o.h && nsmooth == o.nsmooth && mid == o.mid &&
+                   net == o.net && cgen == o.cgen && ngen == o.ngen;
         }
     } fcg_key;
     hipStream_t stream = nullptr;
@@ -128,6 +128,9 @@ struct cd_context {
     cd_net* net = nullptr;
     DevBuf lc_r, lc_x, lc_y, lc_stats;
     DevBuf fcg_p, fcg_q, fcg_dev;  // learned FCG: direction, L direction, coefficients
+    // learned FCG intermediate grid: correction ping-pong, BF16 residual, a
+    // device 0.0f (the mean the intermediate sweeps subtract)
+    DevBuf fcg_em, fcg_emb, fcg_rm, fcg_zero;
 
     // training-pair capture (device buffers owned by the caller)
     float* cap_x = nullptr;

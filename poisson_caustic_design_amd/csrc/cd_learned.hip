@@ -1036,6 +1036,7 @@ void cd_net_arch_default(cd_net_arch* a) {
     a->out_smooth = 0;
     a->loss_smooth = 0;
     a->loss_energy = 0;
+    a->mid_sweeps = 0;
 }
 
 int cd_net_create(cd_context* ctx, const cd_net_arch* arch, cd_net** out) {
@@ -1043,7 +1044,8 @@ int cd_net_create(cd_context* ctx, const cd_net_arch* arch, cd_net** out) {
     *out = nullptr;
     if (arch->levels < 2 || arch->levels > 6 || arch->coarsen < 1) return CD_ERR_BAD_ARG;
     if (arch->out_smooth < 0 || arch->out_smooth > 64 || arch->loss_smooth < 0 ||
-        arch->loss_smooth > 64 || arch->loss_energy < 0 || arch->loss_energy > 1)
+        arch->loss_smooth > 64 || arch->loss_energy < 0 || arch->loss_energy > 1 ||
+        arch->mid_sweeps < 0 || arch->mid_sweeps > 64)
         return CD_ERR_BAD_ARG;
     for (int l = 0; l < arch->levels; ++l)
         if (arch->channels[l] <= 0 || arch->channels[l] % 8 || arch->channels[l] > 128)

@@ -763,7 +763,8 @@ __global__ __launch_bounds__(256) void k_prolong_bricked(const float* __restrict
     int c, j, xx;
     if (!packed_thread(bk, w, yy, c, j, xx)) return;
     const double sig = derived[1];  // rms of the coarse residual
-    const float u = (xx + 0.5f) / f - 0.5f, v = (yy + 0.5f) / f - 0.5f;
+    const float inv = 1.f / f;  // exact for the power-of-two coarsenings in use
+    const float u = (xx + 0.5f) * inv - 0.5f, v = (yy + 0.5f) * inv - 0.5f;
     int i0 = (int)floorf(u), j0 = (int)floorf(v);
     const float tu = u - i0, tv = v - j0;
     const int i1 = min(max(i0 + 1, 0), wc - 1), j1 = min(max(j0 + 1, 0), hc - 1);
@@ -772,6 +773,47 @@ __global__ __launch_bounds__(256) void k_prolong_bricked(const float* __restrict
     const float ev = (y[j0 * wc + i0] * (1.f - tu) + y[j0 * wc + i1] * tu) * (1.f - tv) +
                      (y[j1 * wc + i0] * (1.f - tu) + y[j1 * wc + i1] * tu) * tv;
     e[brick_row(bk, c, yy) + brick_col(j)] = (float)(sig * nscale * (double)ev);
+}
+
+// Intermediate grid of the learned preconditioner (fm x fm fine cells per
+// cell): rm = the block sum of the projected fine residual (r16 - rmean32),
+// i.e. fm^2 times its block mean, the right-hand side of the unit-spacing
+// operator on that grid; BF16, bricked like the fine right-hand side.  One
+// thread per packed element of the intermediate grid.
+__global__ __launch_bounds__(256) void k_restrict_to_mid(const unsigned short* __restrict__ r16,
+                                                         Brick bk, const SolveDev* __restrict__ st,
+                                                         int fm, int wm, Brick bkm,
+                                                         unsigned short* __restrict__ rm) {
+    const int Ym = blockIdx.y;
+    int c, j, xm;
+    if (!packed_thread(bkm, wm, Ym, c, j, xm)) return;
+    const float rmean = st->rmean32;
+    float acc = 0.f;
+    for (int dy = 0; dy < fm; ++dy)
+        for (int dx = 0; dx < fm; ++dx)
+            acc += bf16_to_f32(r16[brick_at(bk, xm * fm + dx, Ym * fm + dy)]) - rmean;
+    rm[brick_row(bkm, c, Ym) + brick_col(j)] = f32_to_bf16(acc);
+}
+
+// fine e = bilinear interpolation of the intermediate-grid correction em
+// (bricked) at fine cell centres; one thread per fine packed element
+__global__ __launch_bounds__(256) void k_prolong_mid(const float* __restrict__ em, Brick bkm,
+                                                     int wm, int hm, int fm, int w, int h,
+                                                     Brick bk, float* __restrict__ e) {
+    const int yy = blockIdx.y;
+    int c, j, xx;
+    if (!packed_thread(bk, w, yy, c, j, xx)) return;
+    const float inv = 1.f / fm;
+    const float u = (xx + 0.5f) * inv - 0.5f, v = (yy + 0.5f) * inv - 0.5f;
+    int i0 = (int)floorf(u), j0 = (int)floorf(v);
+    const float tu = u - i0, tv = v - j0;
+    const int i1 = min(max(i0 + 1, 0), wm - 1), j1 = min(max(j0 + 1, 0), hm - 1);
+    i0 = min(max(i0, 0), wm - 1);
+    j0 = min(max(j0, 0), hm - 1);
+    const float a = em[brick_at(bkm, i0, j0)], b = em[brick_at(bkm, i1, j0)];
+    const float cc = em[brick_at(bkm, i0, j1)], d = em[brick_at(bkm, i1, j1)];
+    e[brick_row(bk, c, yy) + brick_col(j)] = (a * (1.f - tu) + b * tu) * (1.f - tv) +
+                                             (cc * (1.f - tu) + d * tu) * tv;
 }
 
 // ---------------------------------------------------------------------------
@@ -1089,6 +1131,24 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         CD_TRY(ctx->ensure(ctx->fcg_q, sizeof(float) * npk));
         CD_TRY(ctx->ensure(ctx->fcg_dev, sizeof(FcgDev)));
     }
+    // intermediate grid of the preconditioner (coarsening a multiple of 4, >= 8)
+    const int mid_sweeps = arch ? arch->mid_sweeps : 0;
+    const bool mid = learned && mid_sweeps > 0 && lf >= 8 && lf % 4 == 0;
+    const int fm = mid ? lf / 4 : 1;
+    const int wm = w / fm, hm = h / fm;
+    const Brick bkm = make_brick(wm, hm);
+    if (mid) {
+        const size_t npm = 2 * bkm.plane;
+        CD_TRY(ctx->ensure(ctx->fcg_em, sizeof(float) * npm));
+        CD_TRY(ctx->ensure(ctx->fcg_emb, sizeof(float) * npm));
+        CD_TRY(ctx->ensure(ctx->fcg_rm, sizeof(unsigned short) * npm));
+        CD_TRY(ctx->ensure(ctx->fcg_zero, sizeof(float) * 4));
+        // padding cells stay finite; the intermediate sweeps subtract a zero mean
+        CD_TRY(cd_fill_f32(ctx, ctx->fcg_em.as<float>(), npm, 0.f));
+        CD_TRY(cd_fill_f32(ctx, ctx->fcg_emb.as<float>(), npm, 0.f));
+        CD_TRY(cd_fill_f32(ctx, ctx->fcg_rm.as<float>(), npm / 2, 0.f));
+        CD_TRY(cd_fill_f32(ctx, ctx->fcg_zero.as<float>(), 4, 0.f));
+    }
     if (learned || capture)
         CD_TRY(ctx->ensure(ctx->lc_stats, sizeof(double) * 2 * (kMaxPairsPerSolve + 2)));
     if (capture) {
@@ -1190,9 +1250,34 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         CD_TRY(restrict_stats(lf, lwc, lhc, stats, stats + 2));
         CD_TRY(cd_net_forward_coarse(ctx->net, ctx->lc_r.as<float>(), stats,
                                      ctx->lc_y.as<float>(), lhc, lwc));
-        hipLaunchKernelGGL(k_prolong_bricked, packed_grid, dim3(256), 0, s,
-                           ctx->lc_y.as<float>(), stats + 2, w, h, lf, lwc, lhc, nscale, bk,
-                           e32);
+        if (mid) {
+            // coarse -> intermediate grid, smoothing there, -> fine grid
+            const int pbm = div_up(bkm.wpb, 256);
+            unsigned short* rm = ctx->fcg_rm.as<unsigned short>();
+            float* em[2] = {ctx->fcg_em.as<float>(), ctx->fcg_emb.as<float>()};
+            hipLaunchKernelGGL(k_restrict_to_mid, dim3(pbm, hm, 2), dim3(256), 0, s, r16, bk, sd,
+                               fm, wm, bkm, rm);
+            hipLaunchKernelGGL(k_prolong_bricked, dim3(pbm, hm, 2), dim3(256), 0, s,
+                               ctx->lc_y.as<float>(), stats + 2, wm, hm, lf / fm, lwc, lhc, nscale,
+                               bkm, em[0]);
+            const int tx = div_up(wm, bix), ntm = tx * div_up(hm, biy);
+            const int gm = div_up(ntm, 8) * 8;
+            int cur_m = 0;
+            for (int done_sw = 0; done_sw < mid_sweeps;) {
+                const int nsw = std::min(BH / 2, mid_sweeps - done_sw);
+                hipLaunchKernelGGL((k_rb_blocked<BH, false>), dim3(gm), dim3(256), 0, s, em[cur_m],
+                                   em[1 - cur_m], rm, bkm, wm, hm, tx, ntm, 2 * nsw, oms / 2.f,
+                                   oms / 3.f, oms / 4.f, ctx->fcg_zero.as<float>(), sd, 0);
+                cur_m = 1 - cur_m;
+                done_sw += nsw;
+            }
+            hipLaunchKernelGGL(k_prolong_mid, packed_grid, dim3(256), 0, s, em[cur_m], bkm, wm,
+                               hm, fm, w, h, bk, e32);
+        } else {
+            hipLaunchKernelGGL(k_prolong_bricked, packed_grid, dim3(256), 0, s,
+                               ctx->lc_y.as<float>(), stats + 2, w, h, lf, lwc, lhc, nscale, bk,
+                               e32);
+        }
         CD_CHECK_LAUNCH(ctx);
         CD_TRY(ctx->timer_end(CD_KERNEL_LEARNED, 4, 0.0));
         CD_TRY(enqueue_blocked_om(first_sweep, nsmooth, oms / 2.f, oms / 3.f, oms / 4.f));
@@ -1214,7 +1299,9 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         // profiling keeps the launches separate so that every timer records
         const bool replay = ctx->graphs && !ctx->profiling && fcg_iter > 0;
         if (replay) {
-            const cd_context::FcgKey key{w, h, nsmooth, ctx->net, ctx->gen, cd_net_gen(ctx->net)};
+            const cd_context::FcgKey key{w,        h,        nsmooth,
+                                         mid ? mid_sweeps : 0,   ctx->net, ctx->gen,
+                                         cd_net_gen(ctx->net)};
             if (!(ctx->fcg_exec && ctx->fcg_key == key)) {
                 if (ctx->fcg_exec) {
                     CD_HIP(ctx, hipGraphExecDestroy(ctx->fcg_exec));

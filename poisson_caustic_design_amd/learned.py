@@ -265,7 +265,8 @@ def save_params(net: capi.Net, path: str, meta: dict):
                         channels=np.array(list(a.channels)), coarsen=a.coarsen,
                         smooth_sweeps=a.smooth_sweeps, smooth_omega=a.smooth_omega,
                         linear=a.linear, out_smooth=a.out_smooth, loss_smooth=a.loss_smooth,
-                        loss_energy=a.loss_energy, meta=json.dumps(meta))
+                        loss_energy=a.loss_energy, mid_sweeps=a.mid_sweeps,
+                        meta=json.dumps(meta))
 
 
 def load_net(ctx: capi.Context, path: str) -> capi.Net:
@@ -274,7 +275,8 @@ def load_net(ctx: capi.Context, path: str) -> capi.Net:
                              coarsen=int(d["coarsen"]), smooth_sweeps=int(d["smooth_sweeps"]),
                              smooth_omega=float(d["smooth_omega"]), linear=int(d["linear"]),
                              out_smooth=int(d["out_smooth"]), loss_smooth=int(d["loss_smooth"]),
-                             loss_energy=int(d["loss_energy"]) if "loss_energy" in d.files else 0)
+                             loss_energy=int(d["loss_energy"]) if "loss_energy" in d.files else 0,
+                             mid_sweeps=int(d["mid_sweeps"]) if "mid_sweeps" in d.files else 0)
     net = capi.Net(ctx, arch)
     net.set_params(d["params"])
     return net
@@ -298,6 +300,8 @@ def main(argv=None):
     ap.add_argument("--coarsen", type=int, default=16)
     ap.add_argument("--sweeps", type=int, default=8, help="smoothing sweeps per FCG iteration")
     ap.add_argument("--omega", type=float, default=1.5)
+    ap.add_argument("--mid-sweeps", type=int, default=0,
+                    help="smoothing sweeps on the intermediate grid (coarsen/4 cells per cell)")
     ap.add_argument("--random", type=int, default=1200)
     ap.add_argument("--capacity", type=int, default=4000)
     ap.add_argument("--steps", type=int, default=8000)
@@ -332,7 +336,7 @@ def main(argv=None):
     arch = capi.default_arch(levels=a.levels, channels=[int(c) for c in a.channels.split(",")],
                              linear=0 if a.relu else 1, out_smooth=a.out_smooth,
                              loss_smooth=a.loss_smooth, coarsen=a.coarsen,
-                             loss_energy=int(a.loss == "energy"),
+                             loss_energy=int(a.loss == "energy"), mid_sweeps=a.mid_sweeps,
                              smooth_sweeps=a.sweeps, smooth_omega=a.omega)
     t0 = time.time()
     net, losses, pool = fit(ctx, arch, a.mesh_res, range(a.seed0, a.seed0 + a.designs), a.config,

@@ -173,6 +173,32 @@ def test_learned_solve_with_untrained_net_still_converges(ctx):
     assert np.abs((phi - phi.mean()) - ex).max() <= 1e-4 * np.abs(ex).max()
 
 
+def test_intermediate_grid_preconditioner_converges(ctx):
+    """mid_sweeps > 0: coarse correction -> intermediate grid (coarsen/4 cells
+    per cell) smoothed there -> fine grid.  A non-square grid, an untrained
+    small net: the solve still reaches 1e-6 and the DCT solution, and the
+    captured graph replays it bit-identically."""
+    arch = capi.default_arch(levels=3, channels=[8, 16, 16], linear=1, coarsen=8,
+                             smooth_sweeps=4, smooth_omega=1.5, mid_sweeps=4)
+    net = capi.Net(ctx, arch)
+    net.set_params(NetRef(3, (8, 16, 16), linear=True).random_params(21, head_scale=0.05))
+    b = smooth_rhs(np.random.default_rng(8), 256, 192)
+    o = capi.default_opts(tol=1e-6, learned_init=1)
+    ctx.set_net(net)
+    try:
+        ctx.set_graphs(False)
+        phi_e, st_e = ctx.poisson_solve(b, np.zeros_like(b), o)
+        ctx.set_graphs(True)
+        phi_g, st_g = ctx.poisson_solve(b, np.zeros_like(b), o)
+    finally:
+        ctx.set_net(None)
+    assert st_e.status == capi.OK and st_e.rel_residual <= 1e-6
+    assert st_e.learned_applied > 2
+    assert np.array_equal(phi_e, phi_g) and st_e.sweeps == st_g.sweeps
+    ex = dct_solve(b)
+    assert np.abs((phi_e - phi_e.mean()) - ex).max() <= 1e-4 * np.abs(ex).max()
+
+
 def test_capture_pairs(ctx):
     x, t = learned.generate_pairs(ctx, [5, 6], mesh_res=32, capacity=256, log=None)
     assert x.shape[0] > 4
