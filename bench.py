@@ -588,7 +588,8 @@ def main(argv=None):
             "max_abs_height_over_max_height": float(
                 np.abs(sp[:, 2] - plain_final["source_points"][:, 2]).max() / max(zr, 1e-300)),
         }
-        if a.tol == 1e-6 and prec == capi.PREC_MIXED:
+        # the FP64 reference needs an attainable 1e-9 (rounding floor ~ 4 N^2 / pi^2 * 2^-53)
+        if a.tol == 1e-6 and prec == capi.PREC_MIXED and N <= 2048:
             out["learned"]["equal_error"] = equal_error_leg(ctx, cd, one_design, wpath, a, value,
                                                             plain_final, img, prm)
     if a.batch > 1:

@@ -321,10 +321,10 @@ __global__ __launch_bounds__(256) void k_rb_sweep(T* __restrict__ e, const T* __
 // ---------------------------------------------------------------------------
 // Temporally blocked red-black SOR (FP32, packed layout, no holes).
 //
-// A workgroup of 4 waves owns a region of 4*R rows x 128 columns; the halo is
+// A workgroup of W waves owns a region of W*R rows x 128 columns; the halo is
 // H cells on every side.  Lane l of wave v keeps packed column l (cells
-// 2l and 2l+1 of every row, one per colour) of rows v*R .. v*R+R-1 and their
-// right-hand side in registers.  Up/down neighbours of a cell are the other
+// 2l and 2l+1 of every row, one per colour) of rows v*R .. v*R+R-1 in
+// registers, the region's right-hand side sits in LDS.  Up/down neighbours of a cell are the other
 // colour in the rows above/below: registers of the same lane, except at a
 // wave's first/last row, which comes through a small LDS exchange.  Left/right
 // neighbours are the other colour in the same row at packed column l-1+p or
@@ -334,10 +334,14 @@ __global__ __launch_bounds__(256) void k_rb_sweep(T* __restrict__ e, const T* __
 // point; after nhalf <= H half-sweeps the interior (distance >= H) equals
 // nhalf global half-sweeps and is written to the other ping-pong buffer.
 // ---------------------------------------------------------------------------
-// Geometry is fixed at compile time; CD_BK_R / CD_BK_H / CD_BK_RV_LDS exist to
-// compare variants on the hardware.
+// Geometry is fixed at compile time; CD_BK_W / CD_BK_R / CD_BK_H / CD_BK_RV_LDS
+// exist to compare variants on the hardware.  Measured at 2048^2 (4 sweeps per
+// launch, same 128 x 64 region): 4 waves x 16 rows 22.1 us, 8 x 8 17.0 us,
+// 12 x 6 17.7 us, 16 x 4 16.8 us -- more, shorter waves overlap the loads and
+// the sweeps of the one round of resident workgroups better; 8 x 12 and
+// 8 x 16 (taller regions, fewer workgroups) 21.7 and 27.0 us.
 #ifndef CD_BK_R
-#define CD_BK_R 16
+#define CD_BK_R 4
 #endif
 #ifndef CD_BK_H
 #define CD_BK_H 8
@@ -345,9 +349,14 @@ __global__ __launch_bounds__(256) void k_rb_sweep(T* __restrict__ e, const T* __
 #ifndef CD_BK_RV_LDS
 #define CD_BK_RV_LDS 1
 #endif
+#ifndef CD_BK_W
+#define CD_BK_W 16
+#endif
 constexpr int BK_RX = 128;      // region columns (64 packed columns, one per lane)
 constexpr int BK_R = CD_BK_R;   // rows per wave
-constexpr int BK_RY = 4 * BK_R;
+constexpr int BK_NW = CD_BK_W;  // waves per workgroup
+constexpr int BK_THREADS = 64 * BK_NW;
+constexpr int BK_RY = BK_NW * BK_R;
 constexpr bool BK_RV_LDS = CD_BK_RV_LDS != 0;  // right-hand side in LDS (else registers)
 
 // Right-hand side of the region: registers of the owning thread, or LDS
@@ -429,7 +438,7 @@ __device__ __forceinline__ void blocked_half(float (&ev)[2][BK_R], const RV& rv,
 
 template <int C, int H, bool EDGE, bool TRACK, typename RV>
 __device__ __forceinline__ void blocked_step(float (&ev)[2][BK_R], const RV& rv,
-                                             float (*xch)[4][2][64], int lane, int wave, int s,
+                                             float (*xch)[BK_NW][2][64], int lane, int wave, int s,
                                              int dx0, int dx1, int dyw, bool l0, bool l1, bool r0,
                                              bool r1, bool in0, bool in1, int Yw, int h,
                                              float om2, float om3, float om4, float& mx) {
@@ -438,14 +447,14 @@ __device__ __forceinline__ void blocked_step(float (&ev)[2][BK_R], const RV& rv,
     xch[1 - C][wave][1][lane] = ev[1 - C][BK_R - 1];
     __syncthreads();
     const float up_row = xch[1 - C][wave > 0 ? wave - 1 : 0][1][lane];
-    const float down_row = xch[1 - C][wave < 3 ? wave + 1 : 3][0][lane];
+    const float down_row = xch[1 - C][wave < BK_NW - 1 ? wave + 1 : BK_NW - 1][0][lane];
     blocked_half<C, H, EDGE, TRACK>(ev, rv, lane, up_row, down_row, s, dx0, dx1, dyw, l0, l1, r0,
                                     r1, in0, in1, Yw, h, om2, om3, om4, mx);
 }
 
 template <int H, bool EDGE, bool TRACK, typename RV>
 __device__ __forceinline__ void blocked_run(float (&ev)[2][BK_R], const RV& rv,
-                                            float (*xch)[4][2][64], int lane, int wave, int nhalf,
+                                            float (*xch)[BK_NW][2][64], int lane, int wave, int nhalf,
                                             int dx0, int dx1, int dyw, bool l0, bool l1, bool r0,
                                             bool r1, bool in0, bool in1, int Yw, int h, float om2,
                                             float om3, float om4, float (&mxs)[H / 2]) {
@@ -464,7 +473,7 @@ __device__ __forceinline__ void blocked_run(float (&ev)[2][BK_R], const RV& rv,
 }
 
 template <int H, bool TRACK>
-__global__ __launch_bounds__(256) void k_rb_blocked(const float* __restrict__ ein,
+__global__ __launch_bounds__(BK_THREADS) void k_rb_blocked(const float* __restrict__ ein,
                                                     float* __restrict__ eout,
                                                     const unsigned short* __restrict__ r,
                                                     Brick bk, int w, int h,
@@ -472,9 +481,9 @@ __global__ __launch_bounds__(256) void k_rb_blocked(const float* __restrict__ ei
                                                     float om2, float om3, float om4,
                                                     const float* __restrict__ rmean,
                                                     SolveDev* st, int out_index) {
-    static_assert(H % 2 == 0 && H >= 2 && 2 * H < BK_R * 2 && BK_R % 2 == 0, "geometry");
-    __shared__ float xch[2][4][2][64];
-    __shared__ float smax[4][H / 2];
+    static_assert(H % 2 == 0 && H >= 2 && 2 * H < BK_RY && BK_R % 2 == 0, "geometry");
+    __shared__ float xch[2][BK_NW][2][64];
+    __shared__ float smax[BK_NW][H / 2];
     // XCD-aware tile order: gridDim.x is a multiple of 8 and consecutive
     // workgroups go round-robin to the 8 XCDs; give each XCD a contiguous band
     // of tiles so halo re-reads hit its own L2.
@@ -564,7 +573,8 @@ __global__ __launch_bounds__(256) void k_rb_blocked(const float* __restrict__ ei
         __syncthreads();
         if (threadIdx.x < H / 2 && 2 * (int)threadIdx.x < nhalf) {
             const int j = threadIdx.x;
-            const float m = fmaxf(fmaxf(smax[0][j], smax[1][j]), fmaxf(smax[2][j], smax[3][j]));
+            float m = smax[0][j];
+            for (int wv = 1; wv < BK_NW; ++wv) m = fmaxf(m, smax[wv][j]);
             atomicMax(&st->maxupd_bits[j], __float_as_uint(m));
         }
     }
@@ -1062,13 +1072,13 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
             float* ein = in ? e32b : e32;
             float* eout = in ? e32 : e32b;
             if (track) {
-                hipLaunchKernelGGL((k_rb_blocked<BH, true>), dim3(bgrid), dim3(256), 0, s, ein,
+                hipLaunchKernelGGL((k_rb_blocked<BH, true>), dim3(bgrid), dim3(BK_THREADS), 0, s, ein,
                                    eout, r16, bk, w, h, tiles_x, ntiles, 2 * nsw, om2, om3, om4,
                                    &sd->rmean32, sd, 1 - in);
                 hipLaunchKernelGGL(k_maxcheck_blocked, dim3(1), dim3(64), 0, s, sd,
                                    first + done_sw, nsw, o->tol);
             } else {
-                hipLaunchKernelGGL((k_rb_blocked<BH, false>), dim3(bgrid), dim3(256), 0, s, ein,
+                hipLaunchKernelGGL((k_rb_blocked<BH, false>), dim3(bgrid), dim3(BK_THREADS), 0, s, ein,
                                    eout, r16, bk, w, h, tiles_x, ntiles, 2 * nsw, om2, om3, om4,
                                    &sd->rmean32, sd, 1 - in);
             }
@@ -1265,7 +1275,7 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
             int cur_m = 0;
             for (int done_sw = 0; done_sw < mid_sweeps;) {
                 const int nsw = std::min(BH / 2, mid_sweeps - done_sw);
-                hipLaunchKernelGGL((k_rb_blocked<BH, false>), dim3(gm), dim3(256), 0, s, em[cur_m],
+                hipLaunchKernelGGL((k_rb_blocked<BH, false>), dim3(gm), dim3(BK_THREADS), 0, s, em[cur_m],
                                    em[1 - cur_m], rm, bkm, wm, hm, tx, ntm, 2 * nsw, oms / 2.f,
                                    oms / 3.f, oms / 4.f, ctx->fcg_zero.as<float>(), sd, 0);
                 cur_m = 1 - cur_m;
