@@ -273,9 +273,10 @@ int cd_design_export_svg(cd_design* d, const char* path, int what, double stroke
  * activations, FP32 accumulation (gfx950 MFMA), FP32 master weights and Adam.
  *
  * Solves with learned_init = 1 (FP32-mixed precision, no holes) start with
- * flexible conjugate-gradient iterations preconditioned by one two-grid
- * cycle: restrict the residual (block mean), apply the network, prolong
- * (bilinear) and run `smooth_sweeps` red-black SOR sweeps.  The first cycle
+ * flexible conjugate-gradient iterations preconditioned by one cycle:
+ * restrict the residual (block mean), apply the network, prolong (bilinear),
+ * with mid_sweeps > 0 through an intermediate grid smoothed by that many
+ * red-black sweeps, and run `smooth_sweeps` red-black SOR sweeps.  The first cycle
  * is the learned initial correction.  Every iteration recomputes the FP64
  * residual; when eight iterations bring no halving of it the solve continues
  * with plain SOR from the current iterate, so the stop rule and tolerance are

@@ -1,8 +1,9 @@
 """Training and use of the learned coarse solve (cd_net).
 
 The network approximates the inverse of the coarse Neumann operator; solves
-with learned_init use it inside a two-grid preconditioner of flexible
-conjugate gradients.  It is trained on the operator loss
+with learned_init use it as the coarse solve of a multigrid-like
+preconditioner (optionally through an intermediate grid, mid_sweeps) of
+flexible conjugate gradients.  It is trained on the operator loss
 mean((cs * Lc net(x) - x)^2), which needs inputs only:
 
 * coarse residuals captured from the solver itself: designs over seeded
