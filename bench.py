@@ -559,7 +559,9 @@ def main(argv=None):
             "per_design": per_design,
         },
         "roofline": {
-            "kernel": "k_rb_blocked (temporally blocked red-black SOR, 4 sweeps per launch)",
+            "kernel": "k_rb_blocked (temporally blocked red-black SOR, "
+                      f"{bytes_per_launch / (SWEEP_BYTES_PER_CELL * N * N):.2f} sweeps per launch "
+                      "on average)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

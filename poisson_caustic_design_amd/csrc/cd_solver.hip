@@ -339,12 +339,15 @@ __global__ __launch_bounds__(256) void k_rb_sweep(T* __restrict__ e, const T* __
 // launch, same 128 x 64 region): 4 waves x 16 rows 22.1 us, 8 x 8 17.0 us,
 // 12 x 6 17.7 us, 16 x 4 16.8 us -- more, shorter waves overlap the loads and
 // the sweeps of the one round of resident workgroups better; 8 x 12 and
-// 8 x 16 (taller regions, fewer workgroups) 21.7 and 27.0 us.
+// 8 x 16 (taller regions, fewer workgroups) 21.7 and 27.0 us.  Halo (= half-
+// sweeps per launch) with 16 x 4: H = 6 3.6 us per sweep inside a design,
+// H = 8 3.9, H = 10 3.7, H = 12 3.7 (plain designs 177 / 154 / 148 / 147 ms);
+// H = 10 keeps the learned iteration's 8 smoothing sweeps at two launches.
 #ifndef CD_BK_R
 #define CD_BK_R 4
 #endif
 #ifndef CD_BK_H
-#define CD_BK_H 8
+#define CD_BK_H 10
 #endif
 #ifndef CD_BK_RV_LDS
 #define CD_BK_RV_LDS 1
