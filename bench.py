@@ -466,7 +466,7 @@ def main(argv=None):
         return precision_report(a)
 
     if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
+        print(f"bench.py: measures one GPU only (asked for --gpus {a.gpus}); "
               "independent designs shard one per GPU with no communication, so N GPUs give "
               "N x the one-GPU value.", file=sys.stderr)
         return 2
