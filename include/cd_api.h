@@ -27,12 +27,13 @@
 extern "C" {
 #endif
 
-#define CD_API_VERSION 1
+#define CD_API_VERSION 2
 
 typedef enum {
     CD_OK = 0,
     CD_ERR_RASTER_MISS = 1,     /* a raster sample / source vertex lies in no triangle */
-    CD_ERR_FOLDED = 2,          /* a deformed mesh cell has non-positive area           */
+    CD_ERR_FOLDED = 2,          /* a transport step inverted mesh triangles
+                                   (only with cd_design_set_fold_check(d, 1))          */
     CD_ERR_NOT_CONVERGED = 3,   /* sweep cap reached before the stop criterion          */
     CD_ERR_NAN = 4,             /* NaN/Inf in a result                                  */
     CD_ERR_BAD_ARG = 5,
@@ -150,6 +151,16 @@ int cd_design_transport_step(cd_design* d, double* step_out);
 int cd_design_height_step(cd_design* d, double* update_out);
 /* Mesh::laplacian_smoothing(target_points, factor) (mesh.cpp:534-606). */
 int cd_design_laplacian_smoothing(cd_design* d, double factor);
+/* Folded meshes.  The reference keeps stepping a mesh whose triangles have
+ * inverted (mesh.cpp:485-532 does not clamp interior vertices) and reports
+ * nothing.  Every cd_design_transport_step counts the triangles of the
+ * stepped target mesh whose orientation (b-a)x(c-a) is <= 0 (all are > 0 in
+ * the initial mesh, mesh.cpp:57-63); cd_run_stats carries the counts.
+ * fail != 0: a step that leaves inverted triangles returns CD_ERR_FOLDED
+ * (the step is applied); default 0 keeps the reference's behaviour. */
+int cd_design_set_fold_check(cd_design* d, int fail);
+/* Inverted triangles of the current target mesh (e.g. after cd_design_set). */
+int cd_design_count_folds(cd_design* d, long long* folded_triangles);
 /* on != 0: a raster miss in cd_design_transport_step smooths the target mesh
  * with factor 0.1f and returns CD_OK with a NaN step, the recovery of
  * caustic_design.cpp:215-218 (in the reference an exit(0) at mesh.cpp:279
@@ -167,6 +178,9 @@ typedef struct {
     double max_phi_residual, max_h_residual;
     double elapsed_ms;
     int recoveries;               /* raster misses recovered by smoothing            */
+    long long folded_triangles;   /* inverted triangles after the last OT step       */
+    int folded_steps;             /* OT steps that ended with a folded mesh          */
+    int first_folded_step;        /* index of the first of them (-1: none)           */
 } cd_run_stats;
 
 /* The reference driver loop (main.cpp:243-262): up to max_transport_steps

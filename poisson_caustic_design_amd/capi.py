@@ -66,7 +66,8 @@ class RunStats(C.Structure):
                 ("last_step", _D), ("phi_sweeps", _LL), ("h_sweeps", _LL), ("checks", _LL),
                 ("learned_applied", _I), ("learned_rejected", _I),
                 ("max_phi_residual", _D), ("max_h_residual", _D), ("elapsed_ms", _D),
-                ("recoveries", _I)]
+                ("recoveries", _I), ("folded_triangles", _LL), ("folded_steps", _I),
+                ("first_folded_step", _I)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -130,6 +131,8 @@ def load(path: str = LIB_PATH):
         "cd_context_set_graphs": ([_P, _I], _I),
         "cd_design_laplacian_smoothing": ([_P, _D], _I),
         "cd_design_set_miss_recovery": ([_P, _I], _I),
+        "cd_design_set_fold_check": ([_P, _I], _I),
+        "cd_design_count_folds": ([_P, C.POINTER(_LL)], _I),
         "cd_png_read_rgba8": ([C.c_char_p, C.POINTER(_I), C.POINTER(_I), _P, _SZ], _I),
         "cd_image_gray_resize": ([_P, _P, _I, _I, _I, _I, _P], _I),
         "cd_image_gray_resize_device": ([_P, _P, _I, _I, _I, _I, _P], _I),
@@ -184,7 +187,8 @@ def exported_symbols():
         "cd_json_write_heightmap", "cd_svg_write_grid", "cd_svg_write_triangles",
         "cd_design_save_solid_obj", "cd_design_save_heightmap_json", "cd_design_export_svg",
         "cd_design_laplacian_smoothing", "cd_design_set_miss_recovery",
-        "cd_designs_run_batch", "cd_net_set_fused_forward",
+        "cd_designs_run_batch", "cd_net_set_fused_forward", "cd_design_set_fold_check",
+        "cd_design_count_folds",
     ]
 
 
@@ -338,6 +342,7 @@ class CausticDesign:
         self.nthreads = 0
         self.phi_opts: Optional[SolveOpts] = None
         self.h_opts: Optional[SolveOpts] = None
+        self.step_size: Optional[float] = None  # None: the reference's (double)0.05f
         self.d = None
 
     # setters of caustic_design.cpp:28-53
@@ -367,6 +372,8 @@ class CausticDesign:
             p.phi_solve = self.phi_opts
         if self.h_opts is not None:
             p.h_solve = self.h_opts
+        if self.step_size is not None:
+            p.step_size = self.step_size
         return p
 
     def create(self):
@@ -429,6 +436,16 @@ class CausticDesign:
     def laplacian_smoothing(self, factor: float):
         """mesh->laplacian_smoothing(mesh->target_points, factor) (mesh.cpp:534-606)."""
         self.ctx.check(self.L.cd_design_laplacian_smoothing(self.d, factor), "laplacian_smoothing")
+
+    def set_fold_check(self, fail: bool):
+        """fail: a transport step that leaves inverted triangles raises CDError(FOLDED)."""
+        self.ctx.check(self.L.cd_design_set_fold_check(self.d, int(fail)), "set_fold_check")
+
+    def count_folds(self) -> int:
+        """Inverted triangles of the current target mesh."""
+        n = _LL()
+        self.ctx.check(self.L.cd_design_count_folds(self.d, C.byref(n)), "count_folds")
+        return n.value
 
     def set_miss_recovery(self, on: bool):
         self.ctx.check(self.L.cd_design_set_miss_recovery(self.d, int(on)), "set_miss_recovery")

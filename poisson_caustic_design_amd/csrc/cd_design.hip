@@ -469,6 +469,28 @@ __global__ __launch_bounds__(256) void k_vertex_step(const double* __restrict__ 
     if (threadIdx.x == 0) partials[blockIdx.x] = m;
 }
 
+// Folded triangles of a deformed mesh: the structured triangulation
+// (mesh.cpp:57-63) has every triangle counter-clockwise in the undeformed
+// mesh, (b-a) x (c-a) = +dx*dy > 0; a fold reverses (or collapses) at least
+// one, so its orientation is <= 0 (or not finite).  The reference neither
+// detects nor reports folds; it keeps stepping.  partials: count per block.
+__global__ __launch_bounds__(256) void k_fold_count(const double* __restrict__ xy, int mx,
+                                                    int my, double* __restrict__ partials) {
+    __shared__ double smem[16];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double folded = 0.0;
+    if (t < 2 * (mx - 1) * (my - 1)) {
+        int a, b, c;
+        tri_vertices(mx, t, a, b, c);
+        const double ax = xy[2 * a], ay = xy[2 * a + 1];
+        const double cr = (xy[2 * b] - ax) * (xy[2 * c + 1] - ay) -
+                          (xy[2 * b + 1] - ay) * (xy[2 * c] - ax);
+        if (!(cr > 0.0)) folded = 1.0;
+    }
+    const double s = cd_block_sum(folded, smem);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
 // ---------------------------------------------------------------------------
 // height step (caustic_design.cpp:269-332)
 // ---------------------------------------------------------------------------
@@ -657,7 +679,9 @@ struct cd_design {
     DevBuf pixels, image, raster, phi, h, div, normx, normy, owner, owner_src, owner_v;
     DevBuf partials, scalars, tmp3;
     cd_solve_stats last_phi{}, last_h{};
-    long long folded_cells = 0;
+    long long folded_cells = 0;      // dual cells of area <= 0 at the last transport step's start
+    long long folded_triangles = 0;  // inverted triangles after the last transport step
+    bool fold_fail = false;          // a transport step that folds the mesh fails (CD_ERR_FOLDED)
 };
 
 namespace {
@@ -694,6 +718,18 @@ int alloc_design(cd_design* d) {
     CD_TRY(c->ensure(d->partials, 8 * 3 * nb));
     CD_TRY(c->ensure(d->scalars, 8 * 16));
     CD_TRY(c->ensure(d->tmp3, 24 * V));
+    return CD_OK;
+}
+
+// folded-triangle count of the target mesh into scalars[slot]
+int count_folds(cd_design* d, int slot) {
+    const int tb = div_up(d->T, CD_BLOCK);
+    hipLaunchKernelGGL(k_fold_count, dim3(tb), dim3(CD_BLOCK), 0, d->ctx->stream,
+                       d->txy.as<double>(), d->mx, d->my, d->partials.as<double>());
+    CD_CHECK_LAUNCH(d->ctx);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, d->ctx->stream, d->partials.as<double>(),
+                       tb, 1, (int)R_SUM, (int)R_SUM, (int)R_SUM, d->scalars.as<double>() + slot);
+    CD_CHECK_LAUNCH(d->ctx);
     return CD_OK;
 }
 
@@ -801,6 +837,7 @@ int design_init_from_device_image(cd_design* d, const double* img) {
     d->recoveries = 0;
     d->initialized = true;
     d->folded_cells = 0;
+    d->folded_triangles = 0;
     return CD_OK;
 }
 
@@ -918,6 +955,25 @@ int cd_design_laplacian_smoothing(cd_design* d, double factor) {
     return CD_OK;
 }
 
+int cd_design_set_fold_check(cd_design* d, int fail) {
+    if (!d) return CD_ERR_BAD_ARG;
+    if (!d->ctx) return CD_ERR_STATE;
+    d->fold_fail = fail != 0;
+    return CD_OK;
+}
+
+int cd_design_count_folds(cd_design* d, long long* folded_triangles) {
+    if (!d || !folded_triangles) return CD_ERR_BAD_ARG;
+    if (!d->ctx) return CD_ERR_STATE;
+    if (!d->initialized) return CD_ERR_STATE;
+    CD_HIP(d->ctx, hipSetDevice(d->ctx->device));
+    CD_TRY(count_folds(d, 9));
+    double sc[10];
+    CD_TRY(read_scalars(d, sc, 10));
+    *folded_triangles = (long long)sc[9];
+    return CD_OK;
+}
+
 int cd_design_set_miss_recovery(cd_design* d, int on) {
     if (!d) return CD_ERR_BAD_ARG;
     if (!d->ctx) return CD_ERR_STATE;
@@ -972,11 +1028,19 @@ int cd_design_transport_step(cd_design* d, double* step_out) {
                        d->txy.as<double>(), d->vgrad.as<double>(), d->partials.as<double>());
     CD_CHECK_LAUNCH(c);
     CD_TRY(finish(d, vb, 1, R_MAX, R_SUM, R_SUM, 3));
-    CD_TRY(c->timer_end(CD_KERNEL_VERTEX, 1, 64.0 * V));
-    double sc[5];
-    CD_TRY(read_scalars(d, sc, 5));
+    // orientation of every triangle of the stepped mesh
+    CD_TRY(count_folds(d, 8));
+    CD_TRY(c->timer_end(CD_KERNEL_VERTEX, 2, 64.0 * V + 48.0 * d->T));
+    double sc[9];
+    CD_TRY(read_scalars(d, sc, 9));
     d->folded_cells = (long long)sc[4];
+    d->folded_triangles = (long long)sc[8];
     if (step_out) *step_out = sc[3] / d->p.width;
+    if (d->fold_fail && d->folded_triangles > 0) {
+        c->set_error("transport step folded the mesh: " + std::to_string(d->folded_triangles) +
+                     " inverted triangles");
+        return CD_ERR_FOLDED;
+    }
     return CD_OK;
 }
 
@@ -1060,6 +1124,7 @@ int cd_design_run(cd_design* d, int max_transport_steps, double conv_tres, int h
     auto t0 = std::chrono::steady_clock::now();
     cd_run_stats r;
     std::memset(&r, 0, sizeof(r));
+    r.first_folded_step = -1;
     int status = CD_OK;
     const int recoveries0 = d->recoveries;
     // main.cpp:243-256: OT steps, stop after the first step below conv_tres
@@ -1068,6 +1133,13 @@ int cd_design_run(cd_design* d, int max_transport_steps, double conv_tres, int h
         int st = cd_design_transport_step(d, &step);
         r.transport_steps++;
         r.last_step = step;
+        if (st == CD_OK || st == CD_ERR_FOLDED) {
+            r.folded_triangles = d->folded_triangles;
+            if (d->folded_triangles > 0) {
+                r.folded_steps++;
+                if (r.first_folded_step < 0) r.first_folded_step = it;
+            }
+        }
         r.phi_sweeps += d->last_phi.sweeps;
         r.checks += d->last_phi.checks;
         r.learned_applied += d->last_phi.learned_applied;

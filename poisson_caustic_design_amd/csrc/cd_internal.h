@@ -199,12 +199,38 @@ int cd_fill_f32(cd_context* ctx, float* p, size_t n, float v);
 int cd_fill_f64(cd_context* ctx, double* p, size_t n, double v);
 int cd_fill_i32(cd_context* ctx, int* p, size_t n, int v);
 
+// Device buffers of one plain Poisson solve, supplied by the caller instead
+// of the context's grow-only workspace (the allocation-free entry point).
+// Sizes for a w x h grid: partials 4*nred doubles (nred = ceil(w/64)*ceil(h/4)),
+// e32/e32b 2*plane floats each in the bricked layout (mixed precision without
+// holes; e32 n floats with holes), r BF16 2*plane (no holes) or FP32 n (holes).
+struct SolveWork {
+    double* partials = nullptr;
+    SolveDev* sd = nullptr;
+    float* e32 = nullptr;
+    float* e32b = nullptr;
+    void* r = nullptr;
+};
+
 // Poisson solve on device-resident data.  b is the right-hand side (not
 // modified), phi the warm start/result (FP64).  mask: optional per-cell
-// neighbour-validity bits (NULL: no holes).
+// neighbour-validity bits (NULL: no holes).  work: NULL takes the context's
+// workspace (allocated on demand); otherwise those buffers, and the solve
+// allocates nothing (no learned iterations, no training-pair capture).
 int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, const uint8_t* mask,
                              long long valid_cells, int w, int h, const cd_solve_opts* o,
-                             cd_solve_stats* st);
+                             cd_solve_stats* st, const SolveWork* work = nullptr);
+
+// Sizes of the solve buffers of a w x h grid: nred reduction blocks (the
+// partials hold 4*nred doubles) and npk elements of the bricked layout.
+void cd_solve_work_sizes(int w, int h, size_t* nred, size_t* npk);
+
+// Hole mask of a right-hand side (per-cell neighbour-validity bits, NaN
+// cells marked) into mask (w*h bytes); the number of holes is reduced into
+// fill[0] through partials (nred doubles) and read back through host_buf
+// (pinned host memory).
+int cd_hole_mask(cd_context* ctx, const double* d_rhs, int w, int h, uint8_t* mask,
+                 double* partials, double* fill, double* host_buf, long long* holes);
 
 // Learned network forward on one coarse field (cd_learned.hip).
 int cd_net_forward_internal(cd_net* n, const float* d_x, float* d_y, int h, int w);

@@ -584,7 +584,9 @@ __global__ __launch_bounds__(BK_THREADS) void k_rb_blocked(const float* __restri
     if (threadIdx.x == 0) st->cur = out_index;  // same value from every workgroup
 }
 
-// max-update stop rule over the sweeps of one blocked launch (solver.cpp:142)
+// max-update stop rule over the sweeps of one blocked launch (solver.cpp:142).
+// The launch has applied all of its nsweeps sweeps, so a stop at any of them
+// counts the whole launch.
 __global__ void k_maxcheck_blocked(SolveDev* st, int first_sweep, int nsweeps, double tol) {
     if (threadIdx.x != 0) return;
     if (!st->done) {
@@ -593,11 +595,20 @@ __global__ void k_maxcheck_blocked(SolveDev* st, int first_sweep, int nsweeps, d
             st->last_max_update = (double)mu;
             if ((double)mu < tol) {
                 st->done = 1;
-                st->sweeps_at_done = first_sweep + j + 1;
+                st->sweeps_at_done = first_sweep + nsweeps;
                 break;
             }
         }
     }
+    for (int j = 0; j < 16; ++j) st->maxupd_bits[j] = 0u;
+}
+
+// Reopen a solve whose max-update stop is to be confirmed on a fresh
+// right-hand side (the sweeps that stopped it read r rounded at the last check).
+__global__ void k_reopen(SolveDev* st) {
+    if (threadIdx.x != 0) return;
+    st->done = 0;
+    st->sweeps_at_done = -1;
     for (int j = 0; j < 16; ++j) st->maxupd_bits[j] = 0u;
 }
 
@@ -975,7 +986,7 @@ __global__ void k_capture_finish(const double* __restrict__ final_c,
 
 int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, const uint8_t* mask,
                              long long valid_cells, int w, int h, const cd_solve_opts* o,
-                             cd_solve_stats* st_out) {
+                             cd_solve_stats* st_out, const SolveWork* work) {
     auto t_start = std::chrono::steady_clock::now();
     if (w < 2 || h < 2 || !b || !phi || !o) return CD_ERR_BAD_ARG;
     if (o->stop != CD_STOP_RESIDUAL && o->stop != CD_STOP_MAX_UPDATE) return CD_ERR_BAD_ARG;
@@ -988,26 +999,36 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     const dim3 rblk(RED_BX, RED_BY);
     const dim3 rgrid(div_up(w, RED_BX), div_up(h, RED_BY));
     const int nred = rgrid.x * rgrid.y;
-    // (sum, sum^2) per block for checks, four dot products per block for FCG
-    CD_TRY(ctx->ensure(ctx->partials, sizeof(double) * 4 * (size_t)nred));
-    CD_TRY(ctx->ensure(ctx->solve_dev, sizeof(SolveDev)));
     // FP32 sweeps without holes take the temporally blocked kernel on the
     // packed layout; FP64 sweeps and hole masks take the per-half-sweep kernel.
     const bool blocked = mixed && mask == nullptr;
     const Brick bk = make_brick(w, h);
     const size_t npk = 2 * bk.plane;
-    if (mixed) {
-        CD_TRY(ctx->ensure(ctx->e32, sizeof(float) * (blocked ? npk : n)));
-        // FP32 rhs (natural layout) or BF16 rhs (bricked layout)
-        CD_TRY(ctx->ensure(ctx->r32, blocked ? sizeof(unsigned short) * npk : sizeof(float) * n));
-        if (blocked) CD_TRY(ctx->ensure(ctx->e32b, sizeof(float) * npk));
+    SolveWork ws;
+    if (work) {
+        ws = *work;
+    } else {
+        // (sum, sum^2) per block for checks, four dot products per block for FCG
+        CD_TRY(ctx->ensure(ctx->partials, sizeof(double) * 4 * (size_t)nred));
+        CD_TRY(ctx->ensure(ctx->solve_dev, sizeof(SolveDev)));
+        if (mixed) {
+            CD_TRY(ctx->ensure(ctx->e32, sizeof(float) * (blocked ? npk : n)));
+            // FP32 rhs (natural layout) or BF16 rhs (bricked layout)
+            CD_TRY(ctx->ensure(ctx->r32, blocked ? sizeof(unsigned short) * npk : sizeof(float) * n));
+            if (blocked) CD_TRY(ctx->ensure(ctx->e32b, sizeof(float) * npk));
+        }
+        ws.partials = ctx->partials.as<double>();
+        ws.sd = ctx->solve_dev.as<SolveDev>();
+        ws.e32 = mixed ? ctx->e32.as<float>() : nullptr;
+        ws.e32b = blocked ? ctx->e32b.as<float>() : nullptr;
+        ws.r = mixed ? ctx->r32.ptr : nullptr;
     }
-    double* partials = ctx->partials.as<double>();
-    SolveDev* sd = ctx->solve_dev.as<SolveDev>();
-    float* e32 = mixed ? ctx->e32.as<float>() : nullptr;
-    float* e32b = blocked ? ctx->e32b.as<float>() : nullptr;
-    float* r32 = mixed ? ctx->r32.as<float>() : nullptr;
-    unsigned short* r16 = blocked ? ctx->r32.as<unsigned short>() : nullptr;
+    double* partials = ws.partials;
+    SolveDev* sd = ws.sd;
+    float* e32 = mixed ? ws.e32 : nullptr;
+    float* e32b = blocked ? ws.e32b : nullptr;
+    float* r32 = mixed ? static_cast<float*>(ws.r) : nullptr;
+    unsigned short* r16 = blocked ? static_cast<unsigned short*>(ws.r) : nullptr;
     if (mixed && blocked) CD_TRY(cd_fill_f32(ctx, r32, npk / 2, 0.f));  // padding stays finite
 
     const double omega = cd_sor_omega(o->omega, w, h);
@@ -1128,11 +1149,11 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
                (h / f) % (1 << (lv - 1)) == 0 && m > 0;
     };
     bool learned = o->learned_init && arch && blocked && o->stop == CD_STOP_RESIDUAL &&
-                   coarse_ok(arch->coarsen, arch->levels);
+                   coarse_ok(arch->coarsen, arch->levels) && !work;
     const int lf = arch ? arch->coarsen : 1;
     const int lwc = w / std::max(lf, 1), lhc = h / std::max(lf, 1);
     const bool capture = ctx->cap_x && ctx->cap_coarsen > 0 && w == ctx->cap_coarsen * ctx->cap_w &&
-                         h == ctx->cap_coarsen * ctx->cap_h && blocked;
+                         h == ctx->cap_coarsen * ctx->cap_h && blocked && !work;
     const int cwc = capture ? ctx->cap_w : 0, chc = capture ? ctx->cap_h : 0;
     constexpr int kMaxPairsPerSolve = 64;
     std::vector<int> cap_slots;
@@ -1233,11 +1254,9 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     const int obuf_index = (smooth_launches & 1) ? 0 : 1;
     int learned_applied = 0, learned_rejected = 0;
 
-    std::vector<int> cum_at_check;
     int check_id = 0;
     int cum = 0;
     CD_TRY(enqueue_check(check_id));
-    cum_at_check.push_back(0);
     if (capture && !ctx->host_status->done) CD_TRY(capture_pair());
     double prev_rel = ctx->host_status->rel;
     int k = kmax;
@@ -1341,7 +1360,6 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         ++fcg_iter;
         cum += nsmooth;
         CD_TRY(enqueue_check(++check_id));
-        cum_at_check.push_back(cum);
         ++learned_applied;
         FcgDev* fd = ctx->fcg_dev.as<FcgDev>();
         (void)fd;
@@ -1369,16 +1387,36 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         }
         prev_rel = hs.rel;
     }
+    // Max-update rule in mixed precision: the sweeps see the right-hand side
+    // rounded (BF16 / FP32) at the last check, which perturbs every update by
+    // up to omega/4 times that rounding of the check's residual.  A stop is
+    // therefore confirmed on a fresh FP64 residual: the check after the
+    // stopping batch folds the correction and rounds the new residual, and
+    // one more launch of sweeps must meet the rule again; if none of its
+    // sweeps does, the solve goes on (and a later stop is confirmed the same
+    // way).  FP64 sweeps read b itself and stop at once.
+    const bool confirm_stop = track && mixed;
+    const int confirm_sweeps = blocked ? BH / 2 : 4;
+    bool confirming = false;
     while (!done && cum < o->max_sweeps) {
-        const int nsw = std::min(k, o->max_sweeps - cum);
+        const int nsw = std::min(confirming ? confirm_sweeps : k, o->max_sweeps - cum);
         CD_TRY(enqueue_sweeps(cum, nsw));
         cum += nsw;
         ++check_id;
         CD_TRY(enqueue_check(check_id));
-        cum_at_check.push_back(cum);
         const SolveDev& hs = *ctx->host_status;
         done = hs.done != 0;
         if (!std::isfinite(hs.rel)) break;
+        if (confirm_stop) {
+            if (done && !confirming && cum < o->max_sweeps) {
+                hipLaunchKernelGGL(k_reopen, dim3(1), dim3(64), 0, s, sd);
+                CD_CHECK_LAUNCH(ctx);
+                confirming = true;
+                done = false;
+                continue;
+            }
+            confirming = false;  // confirmed (done) or not: back to full batches
+        }
         if (capture && !done && check_id % std::max(ctx->cap_every, 1) == 0)
             CD_TRY(capture_pair());
         if (adaptive && !done) {
@@ -1419,10 +1457,8 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     if (st_out) {
         std::memset(st_out, 0, sizeof(*st_out));
         st_out->status = status;
-        if (hs.done && o->stop == CD_STOP_MAX_UPDATE) st_out->sweeps = hs.sweeps_at_done;
-        else if (hs.done && hs.done_check >= 0 && hs.done_check < (int)cum_at_check.size())
-            st_out->sweeps = cum_at_check[hs.done_check];
-        else st_out->sweeps = cum;
+        // the residual rule stops at the last check, after all enqueued sweeps
+        st_out->sweeps = (hs.done && o->stop == CD_STOP_MAX_UPDATE) ? hs.sweeps_at_done : cum;
         st_out->checks = hs.checks;
         st_out->rel_residual = hs.rel;
         st_out->initial_rel_residual = hs.rel0;
@@ -1435,6 +1471,28 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     return status;
 }
 
+// Sizes of the solve buffers of a w x h grid (see SolveWork).
+void cd_solve_work_sizes(int w, int h, size_t* nred, size_t* npk) {
+    *nred = (size_t)div_up(w, RED_BX) * div_up(h, RED_BY);
+    *npk = 2 * make_brick(w, h).plane;
+}
+
+// Hole mask of a right-hand side (NaN cells and their neighbours' validity
+// bits) and the number of holes, read back through host_buf (pinned).
+int cd_hole_mask(cd_context* ctx, const double* d_rhs, int w, int h, uint8_t* mask,
+                 double* partials, double* fill, double* host_buf, long long* holes) {
+    const dim3 rblk(RED_BX, RED_BY);
+    const dim3 rgrid(div_up(w, RED_BX), div_up(h, RED_BY));
+    const int nred = rgrid.x * rgrid.y;
+    hipLaunchKernelGGL(k_build_mask, rgrid, rblk, 0, ctx->stream, d_rhs, mask, w, h, partials);
+    CD_CHECK_LAUNCH(ctx);
+    CD_TRY(cd_reduce_partials(ctx, partials, nred, 1, fill));
+    CD_HIP(ctx, hipMemcpyAsync(host_buf, fill, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *holes = (long long)host_buf[0];
+    return CD_OK;
+}
+
 extern "C" {
 
 int cd_poisson_solve_device(cd_context* ctx, const double* d_rhs, double* d_phi, int width,
@@ -1444,24 +1502,19 @@ int cd_poisson_solve_device(cd_context* ctx, const double* d_rhs, double* d_phi,
     if (!opts) { cd_solve_opts_default(&def); opts = &def; }
     CD_HIP(ctx, hipSetDevice(ctx->device));
     const size_t n = (size_t)width * height;
-    // hole mask: only materialised when the right-hand side contains NaN
+    size_t nred, npk;
+    cd_solve_work_sizes(width, height, &nred, &npk);
+    // hole mask: only used by the solve when the right-hand side contains NaN
     CD_TRY(ctx->ensure(ctx->mask, n));
-    const dim3 rblk(RED_BX, RED_BY);
-    const dim3 rgrid(div_up(width, RED_BX), div_up(height, RED_BY));
-    const int nred = rgrid.x * rgrid.y;
-    CD_TRY(ctx->ensure(ctx->partials, sizeof(double) * 4 * (size_t)nred));  // as the solve
+    CD_TRY(ctx->ensure(ctx->partials, sizeof(double) * 4 * nred));  // as the solve
     CD_TRY(ctx->ensure(ctx->fill, sizeof(double) * 4));
-    hipLaunchKernelGGL(k_build_mask, rgrid, rblk, 0, ctx->stream, d_rhs, ctx->mask.as<uint8_t>(),
-                       width, height, ctx->partials.as<double>());
-    CD_CHECK_LAUNCH(ctx);
-    CD_TRY(cd_reduce_partials(ctx, ctx->partials.as<double>(), nred, 1, ctx->fill.as<double>()));
-    double holes = 0.0;
-    CD_HIP(ctx, hipMemcpyAsync(&holes, ctx->fill.ptr, sizeof(double), hipMemcpyDeviceToHost,
-                               ctx->stream));
-    CD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    long long holes = 0;
+    CD_TRY(cd_hole_mask(ctx, d_rhs, width, height, ctx->mask.as<uint8_t>(),
+                        ctx->partials.as<double>(), ctx->fill.as<double>(),
+                        reinterpret_cast<double*>(ctx->host_status), &holes));
     const uint8_t* mask = holes > 0 ? ctx->mask.as<uint8_t>() : nullptr;
-    return cd_solve_device_internal(ctx, d_rhs, d_phi, mask, (long long)(n - (size_t)holes), width,
-                                    height, opts, stats);
+    return cd_solve_device_internal(ctx, d_rhs, d_phi, mask, (long long)n - holes, width, height,
+                                    opts, stats);
 }
 
 int cd_context_set_capture(cd_context* ctx, float* d_x, float* d_t, int capacity, int h, int w,

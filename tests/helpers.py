@@ -55,3 +55,26 @@ def smooth_rhs(rng, w, h, n_modes=6):
 def small_image(seed, n):
     from poisson_caustic_design_amd import synth
     return synth.resize_nearest(synth.gray_from_u8(synth.gauss5(seed, 64)), n, n)
+
+
+def mesh_triangles(mx, my):
+    """Vertex indices of the structured triangulation of mesh.cpp:57-63:
+    cell (i, j) with base idx = i*mx + j holds {idx, idx+1, idx+mx} and
+    {idx+mx, idx+1, idx+mx+1}."""
+    i, j = np.meshgrid(np.arange(my - 1), np.arange(mx - 1), indexing="ij")
+    idx = (i * mx + j).ravel()
+    t = np.empty((2 * idx.size, 3), dtype=np.int64)
+    t[0::2] = np.stack([idx, idx + 1, idx + mx], 1)
+    t[1::2] = np.stack([idx + mx, idx + 1, idx + mx + 1], 1)
+    return t
+
+
+def fold_count(xy, mx, my):
+    """Triangles whose signed shoelace area is not positive (inverted or
+    collapsed); every triangle of the undeformed mesh has area +dx*dy/2."""
+    xy = np.asarray(xy, dtype=np.float64).reshape(-1, xy.shape[-1])[:, :2]
+    t = mesh_triangles(mx, my)
+    x, y = xy[t, 0], xy[t, 1]
+    area2 = (x[:, 0] * y[:, 1] - x[:, 1] * y[:, 0] + x[:, 1] * y[:, 2] - x[:, 2] * y[:, 1]
+             + x[:, 2] * y[:, 0] - x[:, 0] * y[:, 2])
+    return int(np.count_nonzero(~(area2 > 0)))

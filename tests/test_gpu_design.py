@@ -164,3 +164,70 @@ def test_c1_design_matches_oracle_fixture(ctx):
     print(f"C1: {len(steps)} OT steps, max |dxy|/W {dxy:.2e}, max |dz|/max|z| {dz:.2e}")
     assert dxy < 1e-11 and dz < 1e-10  # measured 1.0e-13 and 7.4e-13
     cd.close()
+
+
+def test_fold_count_matches_host_counter(ctx, oracle, rng):
+    """cd_design_count_folds on host-forced meshes: none on the initial mesh,
+    two after an interior vertex is pushed past its right neighbour (the
+    hand-computed case of tests/test_folds.py), and the host counter's count
+    on random perturbations."""
+    from tests.helpers import fold_count
+    M, W = 12, 0.5
+    cd, _ = make_pair(ctx, oracle, M)
+    tp = cd.get("target_points").copy()
+    assert cd.count_folds() == 0 == fold_count(tp, M, M)
+    dx = W / (M - 1)
+    forced = tp.copy()
+    v = 5 * M + 5
+    forced[v, 0] += 1.5 * dx
+    cd.set("target_points", forced)
+    assert cd.count_folds() == 2 == fold_count(forced, M, M)
+    for amp in (0.2, 0.6, 1.2):
+        p = tp.copy()
+        p[:, :2] += amp * dx * rng.standard_normal((M * M, 2))
+        cd.set("target_points", p)
+        assert cd.count_folds() == fold_count(p, M, M), amp
+
+
+def test_folding_step_is_reported(ctx, oracle):
+    """A transport step with a step size 400x the reference's folds the mesh:
+    the run reports the inverted triangles (the count the host counter finds
+    on the final mesh) and, with the fold check on, the step fails with
+    CD_ERR_FOLDED."""
+    from tests.helpers import fold_count
+    M, W = 12, 0.5
+    N = 4 * M
+    img = small_image(3, N)
+
+    def make(fail):
+        cd = capi.CausticDesign(ctx)
+        cd.set_mesh_resolution(M, M)
+        cd.set_domain_resolution(N, N)
+        cd.set_mesh_size(W, W)
+        cd.set_lens_focal_length(1.5)
+        cd.set_lens_thickness(0.1)
+        o = capi.default_opts(tol=1e-10)
+        cd.set_solve_opts(o, o)
+        cd.step_size = 400 * float(np.float32(0.05))
+        cd.initialize_solvers(img)
+        cd.set_fold_check(fail)
+        return cd
+
+    cd = make(False)
+    rs = cd.run(3, 0.0, 0, allow=(capi.OK, capi.RASTER_MISS))
+    tp = cd.get("target_points")
+    assert rs.folded_steps >= 1 and rs.first_folded_step == 0
+    if rs.status == capi.OK:
+        assert rs.folded_triangles == fold_count(tp, M, M) > 0
+    cd = make(True)
+    with pytest.raises(capi.CDError) as e:
+        cd.perform_transport_iteration()
+    assert e.value.status == capi.FOLDED
+    assert cd.count_folds() == fold_count(cd.get("target_points"), M, M) > 0
+
+
+def test_reference_step_on_smooth_target_does_not_fold(ctx, oracle):
+    cd, _ = make_pair(ctx, oracle, 16)
+    cd.set_fold_check(True)
+    rs = cd.run(5, 0.0, 1)
+    assert rs.status == capi.OK and rs.folded_steps == 0 and rs.first_folded_step == -1
