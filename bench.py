@@ -56,8 +56,8 @@ def cpu_baseline(config: str, seed: int, per_design: dict, timeout: int) -> dict
     m = json.loads(r.stdout.strip().splitlines()[-1])
     t_design = (m["t_init"] + per_design["ot_steps"] * m["t_ot_ns"]
                 + per_design["h_steps"] * m["t_h_ns"] + per_design["sweeps"] * m["t_sweep"])
-    sample = (f"{m['kind']} code, {m['tiles']} SOR tile threads on {m['nproc']} host cpus "
-              f"({m['cpu_model']}): measured initialize_solvers ({m['t_init']:.2f}s), one transport "
+    sample = (f"{m['kind']} code built {m['flags']}, {m['tiles']} SOR tile threads on "
+              f"{m['nproc']} host cpus ({m['cpu_model']}): measured initialize_solvers ({m['t_init']:.2f}s), one transport "
               f"step without its solve ({m['t_ot_ns']:.2f}s), one height step without its solve "
               f"({m['t_h_ns']:.2f}s) and {m['sweeps_timed']} SOR sweeps ({m['t_sweep']*1e3:.2f} ms/"
               f"sweep) at {m['grid']}^2; extrapolated to one design with the GPU run's per-design "
@@ -67,6 +67,61 @@ def cpu_baseline(config: str, seed: int, per_design: dict, timeout: int) -> dict
     return {"value": 1.0 / t_design, "unit": "solves/s", "cores": m["tiles"],
             "kind": m["kind"], "sample": sample, "seconds_per_design": t_design,
             "components": m}
+
+
+def c1_end_to_end(timeout: int) -> dict | None:
+    """C1 end to end on both sides, the reference CLI's flow and defaults
+    (main.cpp:224-262: max-update stops 1e-7 / 1e-8, the reference's omega,
+    <= 50 transport steps with the break at conv_tres 0.01f, 3 height steps):
+    the reference's own code on the host cores, measured in full (not
+    extrapolated), and the GPU library with the same settings
+    (cd_design_params_default), median of 3 designs."""
+    import numpy as np
+
+    from poisson_caustic_design_amd import capi, synth
+
+    threads = min(os.cpu_count() or 1, 16)
+    cmd = [sys.executable, "-m", "oracle.cpu_baseline", "--config", "gauss5", "--seed", "0",
+           "--threads", str(threads), "--full"]
+    try:
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, cwd=ROOT)
+    except subprocess.TimeoutExpired:
+        log("C1 end to end: CPU run timed out")
+        return None
+    if r.returncode != 0:
+        log("C1 end to end: CPU run failed:", r.stderr[-1000:])
+        return None
+    m = json.loads(r.stdout.strip().splitlines()[-1])
+    img, prm = synth.target_for("gauss5", 0)
+    ctx = capi.Context(0)
+    cd = capi.CausticDesign(ctx)     # the drop-in defaults: the reference's solve settings
+    cd.set_mesh_resolution(prm["mesh_res"], prm["mesh_res"])
+    cd.set_domain_resolution(prm["res"], prm["res"])
+    cd.set_mesh_size(prm["width"], prm["height"])
+    cd.set_lens_focal_length(prm["focal_l"])
+    cd.set_lens_thickness(float(np.float32(0.1)))
+    cd.initialize_solvers(img)
+    d_img = cd.device_ptr("image")
+    conv = float(np.float32(0.01))
+    times, rs = [], None
+    for _ in range(4):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        cd.reinitialize_device(d_img)
+        rs = cd.run(50, conv, 3)
+        ctx.synchronize()
+        times.append(time.perf_counter() - t0)
+    cd.close()
+    t_gpu = float(np.median(times[1:]))
+    log(f"C1 end to end: CPU {m['t_design']:.1f} s ({m['ot_steps']} OT steps), GPU "
+        f"{t_gpu * 1e3:.1f} ms ({rs.transport_steps} OT steps)")
+    return {"workload": "C1: 100x100 mesh, 400x400 grid, gauss5 seed 0, the reference CLI's "
+                        "solver defaults (max-update 1e-7 / 1e-8, reference omega)",
+            "cpu_seconds": m["t_design"], "cpu_ot_steps": m["ot_steps"],
+            "cpu": f"reference code built {m['flags']}, {m['tiles']} SOR tile threads on "
+                   f"{m['nproc']} host cpus ({m['cpu_model']}), measured end to end",
+            "gpu_seconds": t_gpu, "gpu_ot_steps": rs.transport_steps,
+            "gpu_sweeps": rs.phi_sweeps + rs.h_sweeps, "speedup": m["t_design"] / t_gpu}
 
 
 PROFILE_SUMMARY = os.path.join(ROOT, "profiles", "k_rb_blocked_design_512mesh_2048grid.md")
@@ -442,6 +497,9 @@ def main(argv=None):
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-timeout", type=int, default=240)
+    ap.add_argument("--no-c1", action="store_true",
+                    help="skip the C1 end-to-end comparison (reference code on the host "
+                         "against the GPU, the reference CLI's settings)")
     ap.add_argument("--mode", choices=["design", "sweeps", "copy", "learned", "precision"],
                     default="design",
                     help="design: the benchmark; sweeps: a fixed number of SOR sweeps at the "
@@ -552,7 +610,7 @@ def main(argv=None):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if prec == capi.PREC_MIXED else "f64",
+        "dtype": "f32 sweeps, bf16 rhs, f64 residual" if prec == capi.PREC_MIXED else "f64",
         "data": "synthetic",
         "config": {
             "workload": (f"full caustic design (init + <=50 OT steps + 3 height steps), "
@@ -617,6 +675,8 @@ def main(argv=None):
         cb = cpu_baseline(a.config, a.seed, per_design, a.cpu_timeout)
     out["cpu_baseline"] = ({k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
                            if cb else None)
+    if not a.no_cpu_baseline and not a.no_c1:
+        out["c1_end_to_end"] = c1_end_to_end(a.cpu_timeout)
     if cb:
         out["speedup_vs_cpu"] = value / cb["value"]
     print(json.dumps(out), flush=True)

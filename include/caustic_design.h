@@ -28,7 +28,12 @@ typedef std::vector<point_t> polygon_t; /* src/mesh.h:18 */
 struct cd_context;
 struct cd_design;
 
-/* The parts of class Mesh (src/mesh.h:31-104) that callers read. */
+/* The parts of class Mesh (src/mesh.h:31-104) that callers read: the point
+ * sets, sizes, the structured triangulation and the index helper.  The
+ * reference's other Mesh methods are the internals of the two steps
+ * (dual cells, BVH rasterization, step_grid, inverse map, normals, height
+ * setting), which run on the device inside perform_transport_iteration and
+ * perform_height_map_iteration; their results are the Caustic_design members. */
 class Mesh {
 public:
     std::vector<point_t> source_points;
@@ -38,6 +43,10 @@ public:
     int res_x = 0;
     int res_y = 0;
     std::vector<std::vector<int>> triangles; /* structured, src/mesh.cpp:57-63 */
+
+    /* left/right/top/bottom neighbour of a vertex, -1 at the border (mesh.cpp:761-788) */
+    void get_vertex_neighbor_ids(int vertex_id, int &left_vertex, int &right_vertex,
+                                 int &top_vertex, int &bottom_vertex) const;
 };
 
 class Caustic_design {
@@ -48,9 +57,16 @@ public:
     Mesh *mesh;
     std::vector<std::vector<double>> phi;
     std::vector<double> errors;
+    /* partitioned dual cells of the target mesh as the last transport step
+     * found it (caustic_design.cpp:195-196), and of the initial mesh
+     * (caustic_design.cpp:343-346); filled while host mirroring is on */
+    std::vector<std::vector<polygon_t>> target_cells;
+    std::vector<std::vector<polygon_t>> first_target_cells;
     std::vector<double> target_areas;
     std::vector<std::vector<double>> pixels;
     std::vector<std::vector<double>> raster;
+    /* gradient[0], gradient[1] of phi (caustic_design.cpp:225) */
+    std::vector<std::vector<std::vector<double>>> gradient;
     std::vector<std::vector<double>> h;
     std::vector<std::vector<double>> divergence;
     std::vector<std::vector<double>> norm_x;
@@ -67,6 +83,11 @@ public:
     double focal_l;
     double thickness;
     int nthreads;
+
+    /* Angle-weighted normal of vertex vertex_index of a res_x x res_y grid
+     * of points (caustic_design.cpp:63-143); computed on the host from the
+     * caller's points (one vertex; no device state involved). */
+    std::vector<double> calculate_vertex_normal(std::vector<point_t> &points, int vertex_index);
 
     double perform_transport_iteration();
     void perform_height_map_iteration(int itr);
@@ -100,8 +121,11 @@ private:
     bool exit_on_miss_ = false;
     bool residual_stop_ = false;
     double rel_tol_ = 1e-6;
+    std::vector<unsigned char> cell_order_;  /* per vertex: quad slots in the
+                                               reference's listing order */
     void refresh_vertices();
     void refresh_grids();
+    void refresh_cells(int initial, std::vector<std::vector<polygon_t>> &cells);
 };
 
 #endif  // CAUSTIC_DESIGN_H

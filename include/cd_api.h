@@ -221,7 +221,10 @@ typedef enum {
     CD_FIELD_PIXELS = 12,         /* N     pixels (min-max scaled image)   */
     CD_FIELD_INVERSE_MAP = 13,    /* V x 2 inverted transport map          */
     CD_FIELD_IMAGE = 14,          /* N     input image as last initialised */
-    CD_FIELD_COUNT = 15
+    CD_FIELD_GRADIENT = 15,       /* 2 x N gradient[0], gradient[1] of phi
+                                     (calculate_gradient, utils.cpp:3-20;
+                                     caustic_design.cpp:225), read only    */
+    CD_FIELD_COUNT = 16
 } cd_field;
 
 /* number of doubles of a field (N = res_x*res_y grid, V = vertices) */
@@ -231,6 +234,18 @@ int cd_design_set(cd_design* d, int field, const double* host_in, size_t count);
 /* device pointer of a field's storage (NULL for composite fields) */
 const void* cd_design_device_ptr(cd_design* d, int field);
 int cd_design_last_solve_stats(const cd_design* d, cd_solve_stats* phi, cd_solve_stats* h);
+/* Partitioned dual cells of the target mesh, the reference's target_cells
+ * (initial == 0; rebuilt each transport step, caustic_design.cpp:195-196) or
+ * first_target_cells (initial != 0; the initial mesh, caustic_design.cpp:
+ * 343-346), built by Mesh::build_target_partitioned_dual_cells
+ * (mesh.cpp:121-231).  For vertex v and slot k < 6: triangles[6v+k] is its
+ * k-th adjacent triangle in ascending index (-1 past the last) and
+ * quads[(6v+k)*12 ...] that triangle's quad as 4 points {x, y, z}.  (The
+ * reference lists a vertex's quads in the iteration order of a
+ * std::unordered_set of these indices; the C++ drop-in reproduces it.)
+ * vertices >= V; triangles holds 6V ints, quads 72V doubles. */
+int cd_design_get_dual_cells(cd_design* d, int initial, int* triangles, double* quads,
+                             size_t vertices);
 
 /* ---- input and output files (main.cpp, src/utils.cpp) -------------------- */
 /* Parameters a design was created with. */

@@ -19,6 +19,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libcdref.so")
+# the reference's unmodified main.cpp built against include/ and libcaustic_amd.so
+REF_MAIN_DROPIN = os.path.join(HERE, "_ref", "ref_main_dropin")
 REFERENCE_TREE = "/root/reference"
 
 _D = C.c_double
@@ -31,6 +33,9 @@ def build(with_reference: bool = True) -> None:
     targets = ["oracle"]
     if with_reference and os.path.isdir(os.path.join(REFERENCE_TREE, "src")):
         targets.append("ref")
+        if os.path.exists(os.path.join(os.path.dirname(HERE), "poisson_caustic_design_amd",
+                                       "libcaustic_amd.so")):
+            targets.append("dropin")
     subprocess.run(["make", "-s", "-C", HERE, *targets], check=True)
 
 
@@ -298,6 +303,9 @@ class RefLib:
         L.ref_design_destroy.argtypes = [_P]
         L.ref_design_set_threads.argtypes = [_P, _I]
         L.ref_design_init.argtypes = [_P, _P]
+        L.ref_get_target_cells.argtypes = [_P, _I, _P]
+        L.ref_get_target_cells.restype = C.c_long
+        L.ref_vertex_normal.argtypes = [_P, _P, _I, _P]
         L.ref_transport_step.argtypes = [_P]
         L.ref_transport_step.restype = _D
         L.ref_height_step.argtypes = [_P]
@@ -416,6 +424,19 @@ class RefDesign:
 
     def laplacian_smoothing(self, factor):
         self.L.ref_laplacian_smoothing(self.h, factor)
+
+    def target_cells(self, initial: bool) -> np.ndarray:
+        """first_target_cells (initial) or target_cells, flattened: per vertex
+        its quad count, then the quads' 4 x {x, y, z}, in the reference's order."""
+        buf = np.empty(self.mx * self.my * (1 + 6 * 12))
+        n = self.L.ref_get_target_cells(self.h, int(initial), _ptr(buf))
+        return buf[:n].copy()
+
+    def vertex_normal(self, xyz, vertex: int) -> np.ndarray:
+        xyz = _f64(xyz)
+        out = np.empty(3)
+        self.L.ref_vertex_normal(self.h, _ptr(xyz), vertex, _ptr(out))
+        return out
 
     def grid(self, which):
         a = np.empty((self.ny, self.nx))

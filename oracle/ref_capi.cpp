@@ -264,6 +264,32 @@ static std::vector<std::vector<double>> to_points(const double* a, int n, int st
 }
 
 // Mesh::laplacian_smoothing (mesh.cpp:534-606) on the design's target mesh
+// first_target_cells (which 1) or target_cells (which 0), flattened: for each
+// vertex its quad count, then that many quads of 4 points {x, y, z}, in the
+// reference's own listing order; returns the number of doubles written.
+long ref_get_target_cells(void* h, int which, double* out) {
+    auto* cd = static_cast<Caustic_design*>(h);
+    const auto& cells = which ? cd->first_target_cells : cd->target_cells;
+    long o = 0;
+    for (const auto& cell : cells) {
+        out[o++] = (double)cell.size();
+        for (const auto& quad : cell)
+            for (const auto& pt : quad)
+                for (int k = 0; k < 3; ++k) out[o++] = pt[k];
+    }
+    return o;
+}
+
+// Caustic_design::calculate_vertex_normal on caller points (V x 3)
+void ref_vertex_normal(void* h, const double* xyz, int vertex, double* out) {
+    auto* cd = static_cast<Caustic_design*>(h);
+    std::vector<std::vector<double>> pts(cd->mesh->source_points.size(), std::vector<double>(3));
+    for (size_t i = 0; i < pts.size(); ++i)
+        for (int k = 0; k < 3; ++k) pts[i][k] = xyz[i * 3 + k];
+    const std::vector<double> n = cd->calculate_vertex_normal(pts, vertex);
+    for (int k = 0; k < 3; ++k) out[k] = n[k];
+}
+
 void ref_laplacian_smoothing(void* h, double factor) {
     auto* cd = static_cast<Caustic_design*>(h);
     cd->mesh->laplacian_smoothing(cd->mesh->target_points, factor);

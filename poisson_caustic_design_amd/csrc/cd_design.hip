@@ -650,6 +650,49 @@ __global__ __launch_bounds__(256) void k_laplacian_smooth(const double* __restri
     dst[2 * i + 1] = ay * f + src[2 * i + 1] * (1.0 - f);
 }
 
+// gradient of a grid by central differences, clamped at the borders
+// (calculate_gradient, utils.cpp:3-20): out[0..G) = d/dx, out[G..2G) = d/dy
+__global__ __launch_bounds__(256) void k_gradient_grid(const double* __restrict__ g, int w,
+                                                       int h, double* __restrict__ out) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= w) return;
+    const size_t row = (size_t)y * w;
+    out[row + x] = (g[row + min(x + 1, w - 1)] - g[row + max(x - 1, 0)]) / 2.0;
+    out[(size_t)w * h + row + x] =
+        (g[(size_t)min(y + 1, h - 1) * w + x] - g[(size_t)max(y - 1, 0) * w + x]) / 2.0;
+}
+
+// Partitioned dual cells (Mesh::build_target_partitioned_dual_cells,
+// mesh.cpp:201-231): for vertex v, slot k < 6 holds its k-th adjacent
+// triangle in ascending index (tri[6v+k], -1 past the last) and that
+// triangle's quad (v_i, mid(ij), centroid(ijk), mid(ik)) as 4 points of
+// {x, y, 0} (mesh.cpp:174-199; target points have z = 0).
+__global__ __launch_bounds__(256) void k_dual_cells(const double* __restrict__ xy, int mx, int my,
+                                                    int* __restrict__ tri,
+                                                    double* __restrict__ quads) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= mx * my) return;
+    const int y = v / mx, x = v - y * mx;
+    int adj[6];
+    const int n = adjacent_triangles(mx, my, x, y, adj);
+    for (int k = 0; k < 6; ++k) {
+        double* o = quads + ((size_t)v * 6 + k) * 12;
+        if (k < n) {
+            double q[8];
+            triangle_quad(mx, xy, v, adj[k], q);
+            for (int c = 0; c < 4; ++c) {
+                o[3 * c] = q[2 * c];
+                o[3 * c + 1] = q[2 * c + 1];
+                o[3 * c + 2] = 0.0;
+            }
+        } else {
+            for (int c = 0; c < 12; ++c) o[c] = 0.0;
+        }
+        tri[(size_t)v * 6 + k] = k < n ? adj[k] : -1;
+    }
+}
+
 __global__ void k_points3(const double* __restrict__ xy, const double* __restrict__ z, int V,
                           double* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -678,6 +721,7 @@ struct cd_design {
     DevBuf big, nbig;  // scatter: triangles spanning many sample rows, and their count
     DevBuf pixels, image, raster, phi, h, div, normx, normy, owner, owner_src, owner_v;
     DevBuf partials, scalars, tmp3;
+    DevBuf grad, cells_t, cells_q;  // on demand: gradient grid, dual-cell export
     cd_solve_stats last_phi{}, last_h{};
     long long folded_cells = 0;      // dual cells of area <= 0 at the last transport step's start
     long long folded_triangles = 0;  // inverted triangles after the last transport step
@@ -1263,6 +1307,7 @@ size_t cd_design_field_size(const cd_design* d, int field) {
         case CD_FIELD_NORM_Y:
         case CD_FIELD_PIXELS:
         case CD_FIELD_IMAGE: return d->G;
+        case CD_FIELD_GRADIENT: return 2 * d->G;
         default: return 0;
     }
 }
@@ -1310,6 +1355,13 @@ int cd_design_get(cd_design* d, int field, double* out, size_t count) {
             CD_CHECK_LAUNCH(c);
             src = d->tmp3.ptr;
             break;
+        case CD_FIELD_GRADIENT:
+            CD_TRY(c->ensure(d->grad, 16 * d->G));
+            hipLaunchKernelGGL(k_gradient_grid, dim3(div_up(d->nx, 256), d->ny), dim3(256), 0, s,
+                               d->phi.as<double>(), d->nx, d->ny, d->grad.as<double>());
+            CD_CHECK_LAUNCH(c);
+            src = d->grad.ptr;
+            break;
         case CD_FIELD_NORMALS:
             CD_HIP(c, hipMemcpyAsync(out, d->nrm_x.ptr, 8 * d->V, hipMemcpyDeviceToHost, s));
             CD_HIP(c, hipMemcpyAsync(out + d->V, d->nrm_y.ptr, 8 * d->V, hipMemcpyDeviceToHost, s));
@@ -1320,6 +1372,29 @@ int cd_design_get(cd_design* d, int field, double* out, size_t count) {
     }
     if (!src) return CD_ERR_BAD_ARG;
     CD_HIP(c, hipMemcpyAsync(out, src, 8 * n, hipMemcpyDeviceToHost, s));
+    CD_HIP(c, hipStreamSynchronize(s));
+    return CD_OK;
+}
+
+int cd_design_get_dual_cells(cd_design* d, int initial, int* triangles, double* quads,
+                             size_t vertices) {
+    if (!d || !triangles || !quads) return CD_ERR_BAD_ARG;
+    if (!d->ctx || !d->initialized) return CD_ERR_STATE;
+    if (vertices < d->V) return CD_ERR_BAD_ARG;
+    cd_context* c = d->ctx;
+    hipStream_t s = c->stream;
+    CD_HIP(c, hipSetDevice(c->device));
+    CD_TRY(c->ensure(d->cells_t, sizeof(int) * 6 * d->V));
+    CD_TRY(c->ensure(d->cells_q, sizeof(double) * 72 * d->V));
+    // the initial target mesh is the regular grid the source points keep
+    const double* xy = initial ? d->sxy.as<double>() : d->txy.as<double>();
+    hipLaunchKernelGGL(k_dual_cells, dim3(div_up((int)d->V, 256)), dim3(256), 0, s, xy, d->mx,
+                       d->my, d->cells_t.as<int>(), d->cells_q.as<double>());
+    CD_CHECK_LAUNCH(c);
+    CD_HIP(c, hipMemcpyAsync(triangles, d->cells_t.ptr, sizeof(int) * 6 * d->V,
+                             hipMemcpyDeviceToHost, s));
+    CD_HIP(c, hipMemcpyAsync(quads, d->cells_q.ptr, sizeof(double) * 72 * d->V,
+                             hipMemcpyDeviceToHost, s));
     CD_HIP(c, hipStreamSynchronize(s));
     return CD_OK;
 }

@@ -203,7 +203,8 @@ int cd_fill_i32(cd_context* ctx, int* p, size_t n, int v);
 // of the context's grow-only workspace (the allocation-free entry point).
 // Sizes for a w x h grid: partials 4*nred doubles (nred = ceil(w/64)*ceil(h/4)),
 // e32/e32b 2*plane floats each in the bricked layout (mixed precision without
-// holes; e32 n floats with holes), r BF16 2*plane (no holes) or FP32 n (holes).
+// holes; e32 n floats with holes), r BF16 2*plane (no holes), FP32 2*plane
+// (no holes, max-update rule) or FP32 n (holes).
 struct SolveWork {
     double* partials = nullptr;
     SolveDev* sd = nullptr;

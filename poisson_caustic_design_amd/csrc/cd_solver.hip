@@ -156,6 +156,9 @@ __device__ __forceinline__ unsigned short f32_to_bf16(float x) {
 __device__ __forceinline__ float bf16_to_f32(unsigned short v) {
     return __uint_as_float((unsigned)v << 16);
 }
+// right-hand side element of the blocked sweeps: BF16 or FP32 storage
+__device__ __forceinline__ float rhs_to_f32(unsigned short v) { return bf16_to_f32(v); }
+__device__ __forceinline__ float rhs_to_f32(float v) { return v; }
 
 // blocked path: fold the current ping-pong correction into phi64, zero both
 __global__ __launch_bounds__(256) void k_flush_packed(double* __restrict__ phi,
@@ -178,7 +181,8 @@ __global__ __launch_bounds__(256) void k_flush_packed(double* __restrict__ phi,
 // partials[block] = (sum r, sum r^2)
 // ---------------------------------------------------------------------------
 // OUT: 0 norms only, 1 FP32 copy of r (natural layout), 2 BF16 copy of r
-// (bricked layout, blocked sweeps).  A thread takes `rows` rows (stride
+// (bricked layout, blocked sweeps), 3 FP32 copy of r in the bricked layout
+// (blocked sweeps under the max-update rule).  A thread takes `rows` rows (stride
 // blockDim.y) of its column before the two block reductions, so a launch
 // reduces over few workgroups.
 template <int OUT>
@@ -223,6 +227,7 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ phi
         }
         if (OUT == 1) r32[i] = (float)r;
         if (OUT == 2) r16[brick_row(bk, (x + y) & 1, y) + brick_col(x >> 1)] = f32_to_bf16((float)r);
+        if (OUT == 3) r32[brick_row(bk, (x + y) & 1, y) + brick_col(x >> 1)] = (float)r;
     }
     double a = cd_block_sum(rs, smem);
     double q = cd_block_sum(rq, smem);
@@ -475,10 +480,14 @@ __device__ __forceinline__ void blocked_run(float (&ev)[2][BK_R], const RV& rv,
     }
 }
 
+// The right-hand side is BF16 under the residual rule (every FP64 check
+// re-rounds the current residual, so the rounding never reaches the result)
+// and FP32 under the max-update rule (TRACK), which cannot see the smooth
+// error a rounded right-hand side leaves behind.
 template <int H, bool TRACK>
 __global__ __launch_bounds__(BK_THREADS) void k_rb_blocked(const float* __restrict__ ein,
                                                     float* __restrict__ eout,
-                                                    const unsigned short* __restrict__ r,
+                                                    const void* __restrict__ rhs,
                                                     Brick bk, int w, int h,
                                                     int tiles_x, int ntiles, int nhalf,
                                                     float om2, float om3, float om4,
@@ -511,8 +520,10 @@ __global__ __launch_bounds__(BK_THREADS) void k_rb_blocked(const float* __restri
     // packed column of both cells, clamped into the row: every load below is
     // a uniform row offset plus this one 32-bit lane offset
     const unsigned col = brick_col(min(max(X0 / 2 + lane, 0), wp - 1));
+    using RT = typename std::conditional<TRACK, float, unsigned short>::type;
+    const RT* __restrict__ r = static_cast<const RT*>(rhs);
     float ev[2][BK_R];
-    unsigned short rr[2][BK_R];
+    RT rr[2][BK_R];
 #pragma unroll
     for (int k = 0; k < BK_R; ++k) {
         const int Yc = min(max(Yw + k, 0), h - 1);
@@ -541,7 +552,7 @@ __global__ __launch_bounds__(BK_THREADS) void k_rb_blocked(const float* __restri
         for (int c = 0; c < 2; ++c) {
             const bool in = iny && (((k + c) & 1) ? in1 : in0);
             ev[c][k] = in ? ev[c][k] : 0.f;
-            rv.set(c, k, dyw, lane, in ? bf16_to_f32(rr[c][k]) - rm : 0.f);
+            rv.set(c, k, dyw, lane, in ? rhs_to_f32(rr[c][k]) - rm : 0.f);
         }
     }
     float mxs[H / 2];
@@ -1013,8 +1024,11 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         CD_TRY(ctx->ensure(ctx->solve_dev, sizeof(SolveDev)));
         if (mixed) {
             CD_TRY(ctx->ensure(ctx->e32, sizeof(float) * (blocked ? npk : n)));
-            // FP32 rhs (natural layout) or BF16 rhs (bricked layout)
-            CD_TRY(ctx->ensure(ctx->r32, blocked ? sizeof(unsigned short) * npk : sizeof(float) * n));
+            // FP32 rhs (natural layout), BF16 rhs (bricked layout) or, under
+            // the max-update rule, FP32 rhs (bricked layout)
+            const size_t rb = blocked ? (o->stop == CD_STOP_MAX_UPDATE ? sizeof(float) : sizeof(unsigned short)) * npk
+                                      : sizeof(float) * n;
+            CD_TRY(ctx->ensure(ctx->r32, rb));
             if (blocked) CD_TRY(ctx->ensure(ctx->e32b, sizeof(float) * npk));
         }
         ws.partials = ctx->partials.as<double>();
@@ -1029,12 +1043,18 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     float* e32b = blocked ? ws.e32b : nullptr;
     float* r32 = mixed ? static_cast<float*>(ws.r) : nullptr;
     unsigned short* r16 = blocked ? static_cast<unsigned short*>(ws.r) : nullptr;
-    if (mixed && blocked) CD_TRY(cd_fill_f32(ctx, r32, npk / 2, 0.f));  // padding stays finite
+    const int track = o->stop == CD_STOP_MAX_UPDATE;
+    // kernel timers record HIP events (created on demand): not with caller buffers
+    auto timer_begin = [&]() -> int { return work ? (int)CD_OK : ctx->timer_begin(); };
+    auto timer_end = [&](int kernel, long long launches, double bytes) -> int {
+        return work ? (int)CD_OK : ctx->timer_end(kernel, launches, bytes);
+    };
+    // padding stays finite
+    if (mixed && blocked) CD_TRY(cd_fill_f32(ctx, r32, track ? npk : npk / 2, 0.f));
 
     const double omega = cd_sor_omega(o->omega, w, h);
     const int kmax = o->check_every > 0 ? o->check_every : auto_check_every(w, h);
     const bool adaptive = o->check_every <= 0 && o->stop == CD_STOP_RESIDUAL;
-    const int track = o->stop == CD_STOP_MAX_UPDATE;
     hipStream_t s = ctx->stream;
 
     // right-hand-side statistics: mean over non-hole cells, ||P b||
@@ -1052,11 +1072,15 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     const dim3 cgrid(div_up(w, RED_BX), div_up(h, RED_BY * kResidualRows));
     const int ncheck = cgrid.x * cgrid.y;  // <= nred
     auto enqueue_check = [&](int check_id) -> int {
-        CD_TRY(ctx->timer_begin());
+        CD_TRY(timer_begin());
         if (blocked) {
             hipLaunchKernelGGL(k_flush_packed, rgrid, rblk, 0, s, phi, e32, e32b, sd, w, h, bk);
-            hipLaunchKernelGGL((k_residual<2>), cgrid, rblk, 0, s, phi, b, mask, sd, r32, r16, bk,
-                               w, h, kResidualRows, partials);
+            if (track)
+                hipLaunchKernelGGL((k_residual<3>), cgrid, rblk, 0, s, phi, b, mask, sd, r32, r16,
+                                   bk, w, h, kResidualRows, partials);
+            else
+                hipLaunchKernelGGL((k_residual<2>), cgrid, rblk, 0, s, phi, b, mask, sd, r32, r16,
+                                   bk, w, h, kResidualRows, partials);
         } else if (mixed) {
             hipLaunchKernelGGL(k_flush, dim3(fblocks), dim3(CD_BLOCK), 0, s, phi, e32, n);
             hipLaunchKernelGGL((k_residual<1>), cgrid, rblk, 0, s, phi, b, mask, sd, r32, r16, bk,
@@ -1068,7 +1092,7 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, partials, ncheck, sd, count,
                            check_id, o->stop, o->tol);
         CD_CHECK_LAUNCH(ctx);
-        CD_TRY(ctx->timer_end(CD_KERNEL_RESIDUAL, mixed ? 3 : 2,
+        CD_TRY(timer_end(CD_KERNEL_RESIDUAL, mixed ? 3 : 2,
                               (blocked ? 48.0 : (mixed ? 44.0 : 16.0)) * (double)n));
         CD_HIP(ctx, hipMemcpyAsync(ctx->host_status, sd, sizeof(SolveDev), hipMemcpyDeviceToHost, s));
         CD_HIP(ctx, hipStreamSynchronize(s));
@@ -1088,7 +1112,7 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     const float om2 = om / 2.f, om3 = om / 3.f, om4 = om / 4.f;
     auto enqueue_blocked_om = [&](int first, int count_sweeps, float om2, float om3,
                                   float om4) -> int {
-        CD_TRY(ctx->timer_begin());
+        CD_TRY(timer_begin());
         int launches = 0;
         for (int done_sw = 0; done_sw < count_sweeps;) {
             const int nsw = std::min(BH / 2, count_sweeps - done_sw);
@@ -1097,7 +1121,7 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
             float* eout = in ? e32 : e32b;
             if (track) {
                 hipLaunchKernelGGL((k_rb_blocked<BH, true>), dim3(bgrid), dim3(BK_THREADS), 0, s, ein,
-                                   eout, r16, bk, w, h, tiles_x, ntiles, 2 * nsw, om2, om3, om4,
+                                   eout, r32, bk, w, h, tiles_x, ntiles, 2 * nsw, om2, om3, om4,
                                    &sd->rmean32, sd, 1 - in);
                 hipLaunchKernelGGL(k_maxcheck_blocked, dim3(1), dim3(64), 0, s, sd,
                                    first + done_sw, nsw, o->tol);
@@ -1112,7 +1136,7 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         CD_CHECK_LAUNCH(ctx);
         // algorithmic bytes: 12 B per cell per red+black sweep (e read, e write,
         // rhs read), whatever the number of sweeps fused into one launch
-        CD_TRY(ctx->timer_end(CD_KERNEL_SOR_SWEEP, launches, 12.0 * (double)n * count_sweeps));
+        CD_TRY(timer_end(CD_KERNEL_SOR_SWEEP, launches, 12.0 * (double)n * count_sweeps));
         return CD_OK;
     };
     auto enqueue_blocked = [&](int first, int count_sweeps) -> int {
@@ -1120,7 +1144,7 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     };
     auto enqueue_sweeps = [&](int first, int count_sweeps) -> int {
         if (blocked) return enqueue_blocked(first, count_sweeps);
-        CD_TRY(ctx->timer_begin());
+        CD_TRY(timer_begin());
         for (int k = 0; k < count_sweeps; ++k) {
             const int sw = first + k;
             for (int color = 0; color < 2; ++color) {
@@ -1135,7 +1159,7 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
             if (track) hipLaunchKernelGGL(k_maxcheck, dim3(1), dim3(64), 0, s, sd, sw, o->tol);
         }
         CD_CHECK_LAUNCH(ctx);
-        CD_TRY(ctx->timer_end(CD_KERNEL_SOR_SWEEP, 2LL * count_sweeps,
+        CD_TRY(timer_end(CD_KERNEL_SOR_SWEEP, 2LL * count_sweeps,
                               (mixed ? 12.0 : 24.0) * (double)n * count_sweeps));
         return CD_OK;
     };
@@ -1153,7 +1177,7 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     const int lf = arch ? arch->coarsen : 1;
     const int lwc = w / std::max(lf, 1), lhc = h / std::max(lf, 1);
     const bool capture = ctx->cap_x && ctx->cap_coarsen > 0 && w == ctx->cap_coarsen * ctx->cap_w &&
-                         h == ctx->cap_coarsen * ctx->cap_h && blocked && !work;
+                         h == ctx->cap_coarsen * ctx->cap_h && blocked && !track && !work;
     const int cwc = capture ? ctx->cap_w : 0, chc = capture ? ctx->cap_h : 0;
     constexpr int kMaxPairsPerSolve = 64;
     std::vector<int> cap_slots;
@@ -1276,7 +1300,7 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     // (which also sizes every buffer), then replayed as one captured graph.
     auto fcg_body = [&](int first_sweep) -> int {
         FcgDev* fd = ctx->fcg_dev.as<FcgDev>();
-        CD_TRY(ctx->timer_begin());
+        CD_TRY(timer_begin());
         // slot 0: (sum, sum^2) of the coarse residual; slot 1: (mean, rms)
         double* stats = ctx->lc_stats.as<double>();
         CD_TRY(restrict_stats(lf, lwc, lhc, stats, stats + 2));
@@ -1311,9 +1335,9 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
                                e32);
         }
         CD_CHECK_LAUNCH(ctx);
-        CD_TRY(ctx->timer_end(CD_KERNEL_LEARNED, 4, 0.0));
+        CD_TRY(timer_end(CD_KERNEL_LEARNED, 4, 0.0));
         CD_TRY(enqueue_blocked_om(first_sweep, nsmooth, oms / 2.f, oms / 3.f, oms / 4.f));
-        CD_TRY(ctx->timer_begin());
+        CD_TRY(timer_begin());
         hipLaunchKernelGGL(k_fcg_dots, dot_grid, dim3(256), 0, s, zbuf,
                            ctx->fcg_p.as<float>(), ctx->fcg_q.as<float>(), r16, sd, bk, w, h,
                            partials);
@@ -1322,7 +1346,7 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
                            ctx->fcg_p.as<float>(), ctx->fcg_q.as<float>(), obuf, fd, bk, w, h,
                            sd, obuf_index);
         CD_CHECK_LAUNCH(ctx);
-        CD_TRY(ctx->timer_end(CD_KERNEL_LEARNED, 3, 26.0 * (double)n));
+        CD_TRY(timer_end(CD_KERNEL_LEARNED, 3, 26.0 * (double)n));
         return CD_OK;
     };
     int fcg_iter = 0;

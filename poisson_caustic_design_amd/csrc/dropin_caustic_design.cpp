@@ -1,10 +1,12 @@
 // Caustic_design drop-in (src/caustic_design.h:7-66, src/caustic_design.cpp)
 // over the C ABI.  All state lives on the device; public members are host
 // mirrors refreshed after each call.
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <stdexcept>
 #include <string>
+#include <unordered_set>
 #include <vector>
 
 #include "caustic_design.h"
@@ -97,6 +99,12 @@ void Caustic_design::initialize_solvers(std::vector<std::vector<double>> image) 
             mesh->triangles.push_back({idx + mesh_res_x, idx + 1, idx + mesh_res_x + 1});
         }
     refresh_vertices();
+    cell_order_.clear();
+    gradient.clear();
+    if (mirror_) {
+        refresh_cells(1, first_target_cells);
+        target_cells = first_target_cells;
+    }
     std::vector<double> ta(cd_design_field_size(design_, CD_FIELD_TARGET_AREAS));
     check(ctx_, cd_design_get(design_, CD_FIELD_TARGET_AREAS, ta.data(), ta.size()), "get");
     target_areas = ta;
@@ -120,6 +128,106 @@ void Caustic_design::refresh_vertices() {
             mesh->target_points[i][k] = t[3 * i + k];
             mesh->source_points[i][k] = s[3 * i + k];
         }
+}
+
+// Dual cells from the device, each vertex's quads listed as the reference
+// lists them: Mesh::find_adjacent_elements (mesh.cpp:121-147) inserts the
+// vertex's triangle indices, ascending, into a std::unordered_set<int> and
+// copies it out in iteration order.  The same container on the same
+// insertion sequence gives that order here; it depends on the topology only,
+// so it is worked out once per design.
+void Caustic_design::refresh_cells(int initial, std::vector<std::vector<polygon_t>> &cells) {
+    const size_t V = (size_t)mesh_res_x * mesh_res_y;
+    std::vector<int> tri(6 * V);
+    std::vector<double> q(72 * V);
+    check(ctx_, cd_design_get_dual_cells(design_, initial, tri.data(), q.data(), V),
+          "cd_design_get_dual_cells");
+    if (cell_order_.size() != 6 * V) {
+        cell_order_.assign(6 * V, 0xFF);
+        for (size_t v = 0; v < V; ++v) {
+            std::unordered_set<int> adjacent;
+            for (int k = 0; k < 6 && tri[6 * v + k] >= 0; ++k) adjacent.insert(tri[6 * v + k]);
+            int pos = 0;
+            for (int t : adjacent)
+                for (int k = 0; k < 6; ++k)
+                    if (tri[6 * v + k] == t) cell_order_[6 * v + pos++] = (unsigned char)k;
+        }
+    }
+    cells.assign(V, std::vector<polygon_t>());
+    for (size_t v = 0; v < V; ++v) {
+        for (int pos = 0; pos < 6 && cell_order_[6 * v + pos] != 0xFF; ++pos) {
+            const double* o = &q[(6 * v + cell_order_[6 * v + pos]) * 12];
+            polygon_t quad(4, point_t(3));
+            for (int c = 0; c < 4; ++c)
+                for (int k = 0; k < 3; ++k) quad[c][k] = o[3 * c + k];
+            cells[v].push_back(quad);
+        }
+    }
+}
+
+// Mesh::get_vertex_neighbor_ids (mesh.cpp:761-788)
+void Mesh::get_vertex_neighbor_ids(int vertex_id, int &left_vertex, int &right_vertex,
+                                   int &top_vertex, int &bottom_vertex) const {
+    const int y = vertex_id / res_x, x = vertex_id % res_x;
+    left_vertex = x != 0 ? y * res_x + (x - 1) : -1;
+    top_vertex = y != 0 ? (y - 1) * res_x + x : -1;
+    right_vertex = x != res_x - 1 ? y * res_x + (x + 1) : -1;
+    bottom_vertex = y != res_y - 1 ? (y + 1) * res_x + x : -1;
+}
+
+namespace {
+
+// unit vector of a 3-vector, each component divided by the length
+// (normalize, utils.cpp:370-384)
+void unit3(const double a[3], double out[3]) {
+    const double len = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    for (int i = 0; i < 3; ++i) out[i] = a[i] / len;
+}
+
+// normal of triangle (p1, p2, p3) and its angle at p1
+// (calculate_angle_and_normal_from_triangle, utils.cpp:386-405)
+void angle_and_normal(const point_t &p1, const point_t &p2, const point_t &p3, double n[3],
+                      double &angle) {
+    const double u[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+    const double v[3] = {p3[0] - p1[0], p3[1] - p1[1], p3[2] - p1[2]};
+    const double c[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2],
+                         u[0] * v[1] - u[1] * v[0]};
+    unit3(c, n);
+    double nu[3], nv[3];
+    unit3(u, nu);
+    unit3(v, nv);
+    const double r = nu[0] * nv[0] + nu[1] * nv[1] + nu[2] * nv[2];
+    angle = r <= -1.0 ? M_PI : (r >= 1.0 ? 0.0 : std::acos(r));
+}
+
+}  // namespace
+
+// Caustic_design::calculate_vertex_normal (caustic_design.cpp:63-143): the
+// angle-weighted mean of the normals of the four corner triangles around the
+// vertex that exist, with x and y negated after normalising.
+std::vector<double> Caustic_design::calculate_vertex_normal(std::vector<point_t> &points,
+                                                            int vertex_index) {
+    if (!mesh) throw std::logic_error("calculate_vertex_normal before initialize_solvers");
+    int l = 0, r = 0, t = 0, b = 0;
+    mesh->get_vertex_neighbor_ids(vertex_index, l, r, t, b);
+    double acc[3] = {0.0, 0.0, 0.0};
+    const int pairs[4][2] = {{l, t}, {b, l}, {r, b}, {t, r}};
+    const bool have[4] = {l != -1 && t != -1, l != -1 && b != -1, r != -1 && b != -1,
+                          r != -1 && t != -1};
+    for (int k = 0; k < 4; ++k) {
+        if (!have[k]) continue;
+        double n[3], angle;
+        angle_and_normal(points[vertex_index], points[pairs[k][0]], points[pairs[k][1]], n, angle);
+        for (int i = 0; i < 3; ++i) acc[i] += n[i] * angle;
+    }
+    std::vector<double> out = {acc[0], acc[1], acc[2]};
+    const double mag = std::sqrt(acc[0] * acc[0] + acc[1] * acc[1] + acc[2] * acc[2]);
+    if (mag > 1e-12) {
+        out[0] /= -mag;
+        out[1] /= -mag;
+        out[2] /= mag;
+    }
+    return out;
 }
 
 void Caustic_design::refresh_grids() {
@@ -154,6 +262,8 @@ void Caustic_design::refresh_grids() {
 
 double Caustic_design::perform_transport_iteration() {
     if (!design_) throw std::logic_error("perform_transport_iteration before initialize_solvers");
+    // the step's dual cells are those of the mesh it starts from (caustic_design.cpp:195-196)
+    if (mirror_) refresh_cells(0, target_cells);
     double step = 0.0;
     int rc = cd_design_transport_step(design_, &step);
     if (rc == CD_ERR_RASTER_MISS && exit_on_miss_) {
@@ -162,7 +272,14 @@ double Caustic_design::perform_transport_iteration() {
     }
     check(ctx_, rc, "perform_transport_iteration");
     refresh_vertices();
-    if (mirror_) refresh_grids();
+    if (mirror_) {
+        refresh_grids();
+        const size_t G = (size_t)resolution_x * resolution_y;
+        std::vector<double> g(2 * G);
+        check(ctx_, cd_design_get(design_, CD_FIELD_GRADIENT, g.data(), g.size()), "get");
+        std::vector<double> gx(g.begin(), g.begin() + G), gy(g.begin() + G, g.end());
+        gradient = {to_grid(gx, resolution_x, resolution_y), to_grid(gy, resolution_x, resolution_y)};
+    }
     return step;
 }
 

@@ -43,14 +43,55 @@ int main(int argc, char** argv) {
     caustic_design.set_solver_max_threads(1);
     caustic_design.initialize_solvers(pixels);
 
+    // the dual cells of the initial mesh, flattened as: per vertex its quad
+    // count, then the quads' 4 points {x, y, z}
+    auto dump_cells = [&](const std::vector<std::vector<polygon_t>>& cells, const char* name) {
+        FILE* f = std::fopen((output_path + name).c_str(), "wb");
+        if (!f) std::exit(1);
+        for (const auto& cell : cells) {
+            const double n = (double)cell.size();
+            std::fwrite(&n, sizeof(double), 1, f);
+            for (const auto& quad : cell)
+                for (const auto& pt : quad) std::fwrite(pt.data(), sizeof(double), 3, f);
+        }
+        std::fclose(f);
+    };
+    auto dump_grid = [&](const std::vector<std::vector<double>>& g, FILE* f) {
+        for (const auto& row : g) std::fwrite(row.data(), sizeof(double), row.size(), f);
+    };
+    dump_cells(caustic_design.first_target_cells, "cells_first.f64");
+
     int steps = 0;
     for (int itr = 0; itr < 50; itr++) {
         double step_size = caustic_design.perform_transport_iteration();
         ++steps;
+        if (itr == 0) {  // the first step's cells, phi and gradient
+            dump_cells(caustic_design.target_cells, "cells_step1.f64");
+            FILE* f = std::fopen((output_path + "phi_grad1.f64").c_str(), "wb");
+            if (!f) return 1;
+            dump_grid(caustic_design.phi, f);
+            dump_grid(caustic_design.gradient[0], f);
+            dump_grid(caustic_design.gradient[1], f);
+            std::fclose(f);
+        }
         if (step_size < (double)0.01f) break;
     }
     for (int itr = 0; itr < 3; itr++) caustic_design.perform_height_map_iteration(itr);
     caustic_design.save_solid_obj_source(output_path + "output.obj");
+    {   // vertex normals of the final lens surface at every 7th vertex
+        FILE* f = std::fopen((output_path + "normals.f64").c_str(), "wb");
+        if (!f) return 1;
+        auto& pts = caustic_design.mesh->source_points;
+        for (int v = 0; v < (int)pts.size(); v += 7) {
+            const std::vector<double> n = caustic_design.calculate_vertex_normal(pts, v);
+            std::fwrite(n.data(), sizeof(double), 3, f);
+        }
+        std::fclose(f);
+        FILE* g = std::fopen((output_path + "source_points.f64").c_str(), "wb");
+        if (!g) return 1;
+        for (auto& p : pts) std::fwrite(p.data(), sizeof(double), 3, g);
+        std::fclose(g);
+    }
 
     double hs = 0.0;
     for (auto& row : caustic_design.h)

@@ -47,3 +47,38 @@ def test_dropin_program_matches_c_abi(ctx, tmp_path):
     cd.save_solid_obj_source(str(tmp_path / "api.obj"))
     assert (tmp_path / "output.obj").read_bytes() == (tmp_path / "api.obj").read_bytes()
     cd.close()
+
+
+def test_dropin_members_match_reference(ctx, tmp_path, reflib):
+    """Members of the C++ drop-in that the reference's Caustic_design also
+    exposes, against the reference's own code (oracle/_ref) on the same input:
+    first_target_cells bit-equal, quads listed in the reference's order
+    (mesh.cpp:121-231, caustic_design.cpp:343-346; that order is not the
+    ascending triangle order for 254 of the 256 vertices here); the first step's
+    target_cells are the initial mesh's (caustic_design.cpp:195-196); the
+    gradient member is the reference's calculate_gradient of the phi member
+    (utils.cpp:3-20, caustic_design.cpp:225); calculate_vertex_normal equals
+    the reference's on the same points (caustic_design.cpp:63-143)."""
+    from oracle.bind import RefDesign
+    M = 16
+    N = 4 * M
+    r = subprocess.run([EXE, str(M), str(tmp_path) + "/"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr[-500:]
+    img = np.fromfile(tmp_path / "image.f64", dtype=np.float64).reshape(N, N)
+    rd = RefDesign(reflib, M, M, N, N, 1.0, 1.0, 1.5, 0.2, 1)
+    rd.init(img)
+    ref_cells = rd.target_cells(initial=True)
+    first = np.fromfile(tmp_path / "cells_first.f64", dtype=np.float64)
+    step1 = np.fromfile(tmp_path / "cells_step1.f64", dtype=np.float64)
+    assert first.size == ref_cells.size and np.array_equal(first, ref_cells)
+    assert np.array_equal(step1, first)
+    pg = np.fromfile(tmp_path / "phi_grad1.f64", dtype=np.float64).reshape(3, N, N)
+    gx, gy = reflib.gradient(pg[0])
+    assert np.array_equal(pg[1], gx) and np.array_equal(pg[2], gy)
+    assert np.abs(pg[1]).max() > 0
+    pts = np.fromfile(tmp_path / "source_points.f64", dtype=np.float64).reshape(M * M, 3)
+    nrm = np.fromfile(tmp_path / "normals.f64", dtype=np.float64).reshape(-1, 3)
+    ref_n = np.array([rd.vertex_normal(pts, v) for v in range(0, M * M, 7)])
+    assert np.array_equal(nrm, ref_n)
+    assert np.abs(nrm[:, :2]).max() > 0  # a curved surface, not the flat default

@@ -3,11 +3,16 @@ rule (solver.cpp:92-146, stop at :142) with the reference's omega
 (solver.cpp:71), mixed precision (FP32 sweeps on a BF16-rounded right-hand
 side, FP64 defect correction) against the same iteration in FP64 sweeps.
 
-A stop decided on sweeps that read the rounded right-hand side is confirmed
-on a fresh FP64 residual before the solve ends; these tests show that the
-mixed result is as accurate as the FP64 one at N = 1024, from a cold start
-and from a warm start that stops within its first batch of sweeps (the case
-where the rounding of the initial residual would otherwise remain)."""
+Under the max-update rule the blocked sweeps read an FP32 right-hand side
+(the FP64 residual of the last check rounded to FP32): the rule cannot see
+smooth error, and the smooth part of a BF16 rounding of the initial residual
+(2^-9 relative per cell) would stay in the result when the stop falls within
+the first batch of sweeps (an FP64 restatement of that case ends 20x further
+from the exact solution than the FP64 solve: 3.3e-5 against 1.7e-6 at this
+size and warm start).  A stop is also confirmed on a fresh FP64 residual.
+These tests show that the mixed result is as accurate as the FP64 one at
+N = 1024, from a cold start and from a warm start whose whole solve lies in
+the first batch of sweeps."""
 import numpy as np
 import pytest
 
@@ -33,6 +38,9 @@ def test_max_update_mixed_as_accurate_as_fp64_at_1024(ctx, start):
         # updates exceed the threshold, and it is met again within one batch
         phi0 = ex + 1e-5 * rng.standard_normal(b.shape)
     kw = dict(stop=capi.STOP_MAX_UPDATE, tol=1e-7, omega=-1.0)
+    if start == "warm":
+        # one batch of sweeps between FP64 checks holds the whole solve
+        kw["check_every"] = 4096
     g64, s64 = ctx.poisson_solve(b, phi0, capi.default_opts(precision=capi.PREC_FP64, **kw))
     gm, sm = ctx.poisson_solve(b, phi0, capi.default_opts(precision=capi.PREC_MIXED, **kw))
     e64, em = mf_err(g64, ex), mf_err(gm, ex)
@@ -41,7 +49,7 @@ def test_max_update_mixed_as_accurate_as_fp64_at_1024(ctx, start):
     assert s64.status == capi.OK and sm.status == capi.OK
     assert sm.last_max_update < 1e-7
     if start == "warm":
-        assert s64.sweeps < 200  # the threshold is met within the first batch
+        assert s64.sweeps < 4096  # the threshold is met within the first batch
     assert em <= 2.0 * e64 + 1e-12 * np.abs(ex).max()
 
 
