@@ -119,6 +119,26 @@ int cd_poisson_solve(cd_context* ctx, const double* rhs, double* phi, int width,
 int cd_poisson_solve_device(cd_context* ctx, const double* d_rhs, double* d_phi, int width,
                             int height, const cd_solve_opts* opts, cd_solve_stats* stats);
 
+/* Allocation-free variant of cd_poisson_solve_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory during a
+ * solve (tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels, results and statuses as cd_poisson_solve_device; its
+ * scratch (reduction partials, solve state, the FP32 corrections, the
+ * right-hand side of the sweeps, the hole mask) is one static device array
+ * per GPU of CD_NOALLOC_SCRATCH_MIB MiB, reserved when the library's code
+ * object is loaded on that GPU and held until the process ends, which covers
+ * every grid up to CD_NOALLOC_MAX_SIDE x CD_NOALLOC_MAX_SIDE; a larger grid
+ * fails with CD_ERR_BAD_ARG before any work.  learned_init must be 0.  All
+ * calls on one GPU share that array (it keeps the last call's contents; each
+ * call initialises what it reads): calls that overlap on one GPU, from any
+ * context or thread, race and must be serialised by the caller.  Kernel
+ * timers (cd_context_set_profiling) do not cover it. */
+#define CD_NOALLOC_MAX_SIDE 2048
+#define CD_NOALLOC_SCRATCH_MIB 53
+int cd_poisson_solve_device_noalloc(cd_context* ctx, const double* d_rhs, double* d_phi,
+                                    int width, int height, const cd_solve_opts* opts,
+                                    cd_solve_stats* stats);
+
 /* ---- design (class Caustic_design, src/caustic_design.h:7-66) ----------- */
 typedef struct {
     int mesh_res_x, mesh_res_y;   /* set_mesh_resolution   (caustic_design.cpp:28-31) */

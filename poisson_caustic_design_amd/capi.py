@@ -111,6 +111,8 @@ def load(path: str = LIB_PATH):
         "cd_poisson_solve": ([_P, _P, _P, _I, _I, C.POINTER(SolveOpts), C.POINTER(SolveStats)], _I),
         "cd_poisson_solve_device": ([_P, _P, _P, _I, _I, C.POINTER(SolveOpts),
                                      C.POINTER(SolveStats)], _I),
+        "cd_poisson_solve_device_noalloc": ([_P, _P, _P, _I, _I, C.POINTER(SolveOpts),
+                                             C.POINTER(SolveStats)], _I),
         "cd_design_params_default": ([C.POINTER(DesignParams)], None),
         "cd_design_create": ([_P, C.POINTER(DesignParams), C.POINTER(_P)], _I),
         "cd_design_destroy": ([_P], None),
@@ -172,7 +174,8 @@ def exported_symbols():
         "cd_get_api_version", "cd_status_string", "cd_solve_opts_default", "cd_context_create",
         "cd_context_destroy", "cd_context_synchronize", "cd_context_last_error",
         "cd_context_set_profiling", "cd_context_kernel_stats", "cd_context_reset_kernel_stats",
-        "cd_poisson_solve", "cd_poisson_solve_device", "cd_design_params_default",
+        "cd_poisson_solve", "cd_poisson_solve_device", "cd_poisson_solve_device_noalloc",
+        "cd_design_params_default",
         "cd_design_create", "cd_design_destroy", "cd_design_set_solve_opts", "cd_design_init",
         "cd_design_init_device", "cd_design_transport_step", "cd_design_height_step",
         "cd_design_run", "cd_design_field_size", "cd_design_get", "cd_design_set",
@@ -306,6 +309,18 @@ class Context:
         rc = self.L.cd_poisson_solve_device(self.h, d_rhs, d_phi, width, height,
                                             C.byref(opts or default_opts()), C.byref(st))
         self.check(rc, "cd_poisson_solve_device", allow)
+        return st
+
+    def poisson_solve_device_noalloc(self, d_rhs: int, d_phi: int, width: int, height: int,
+                                     opts: Optional[SolveOpts] = None,
+                                     allow=(OK, NOT_CONVERGED)):
+        """cd_poisson_solve_device_noalloc: the same solve with the library's
+        static per-device scratch instead of allocated buffers (grids up to
+        2048 x 2048; overlapping calls on one GPU race)."""
+        st = SolveStats()
+        rc = self.L.cd_poisson_solve_device_noalloc(self.h, d_rhs, d_phi, width, height,
+                                                    C.byref(opts or default_opts()), C.byref(st))
+        self.check(rc, "cd_poisson_solve_device_noalloc", allow)
         return st
 
 

@@ -23,6 +23,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <type_traits>
 
 #include "cd_internal.h"
@@ -1517,7 +1519,115 @@ int cd_hole_mask(cd_context* ctx, const double* d_rhs, int w, int h, uint8_t* ma
     return CD_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free solve.  Its buffers (reduction partials, solve state, the
+// two ping-pong FP32 corrections, the right-hand side of the sweeps, the hole
+// mask and the reduction slots) live in one static device array per GPU,
+// sized for grids of up to CD_NOALLOC_MAX_SIDE x CD_NOALLOC_MAX_SIDE cells and
+// held until the process ends.  The array's address on each device is taken
+// once (hipGetSymbolAddress, with that device current) and kept per device;
+// host code passes that address, never the array's name.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr size_t al256(size_t x) { return (x + 255) / 256 * 256; }
+
+struct NoallocLayout {
+    size_t partials, sd, e32, e32b, r, mask, fill, total;
+};
+
+constexpr NoallocLayout noalloc_layout(int w, int h) {
+    // the bricked layout of make_brick and the reduction grid of the checks
+    const size_t wp = (size_t)(w + 1) / 2;
+    const size_t plane = ((size_t)(h + 7) / 8 * 8) * ((wp + 63) / 64 * 64);
+    const size_t npk = 2 * plane;  // >= w*h (natural layout with holes fits)
+    const size_t nred = (size_t)((w + RED_BX - 1) / RED_BX) * ((h + RED_BY - 1) / RED_BY);
+    NoallocLayout L{};
+    size_t o = 0;
+    L.partials = o; o += al256(sizeof(double) * 4 * nred);
+    L.sd = o;       o += al256(sizeof(SolveDev));
+    L.e32 = o;      o += al256(sizeof(float) * npk);
+    L.e32b = o;     o += al256(sizeof(float) * npk);
+    L.r = o;        o += al256(sizeof(float) * npk);  // FP32 (max-update rule) or BF16
+    L.mask = o;     o += al256((size_t)w * h);
+    L.fill = o;     o += al256(sizeof(double) * 4);
+    L.total = o;
+    return L;
+}
+
+constexpr size_t kNoallocScratchBytes =
+    noalloc_layout(CD_NOALLOC_MAX_SIDE, CD_NOALLOC_MAX_SIDE).total;
+static_assert(kNoallocScratchBytes <= ((size_t)CD_NOALLOC_SCRATCH_MIB << 20) &&
+                  kNoallocScratchBytes > ((size_t)(CD_NOALLOC_SCRATCH_MIB - 1) << 20),
+              "CD_NOALLOC_SCRATCH_MIB states the scratch size");
+
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this device's scratch (set up on the first call per
+// device; the caller has made ctx->device current).
+int noalloc_scratch(cd_context* ctx, unsigned char** out) {
+    if (ctx->device < 0 || ctx->device >= kMaxDevices) return CD_ERR_BAD_ARG;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[ctx->device]) {
+        void* p = nullptr;
+        CD_HIP(ctx, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) {
+            ctx->set_error("hipGetSymbolAddress returned no address for the solve scratch");
+            return CD_ERR_HIP;
+        }
+        g_scratch_ptr[ctx->device] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch_ptr[ctx->device];
+    return CD_OK;
+}
+
+}  // namespace
+
 extern "C" {
+
+int cd_poisson_solve_device_noalloc(cd_context* ctx, const double* d_rhs, double* d_phi,
+                                    int width, int height, const cd_solve_opts* opts,
+                                    cd_solve_stats* stats) {
+    if (!ctx || !d_rhs || !d_phi || width < 2 || height < 2) return CD_ERR_BAD_ARG;
+    cd_solve_opts def;
+    if (!opts) { cd_solve_opts_default(&def); opts = &def; }
+    if (opts->learned_init) {
+        ctx->set_error("cd_poisson_solve_device_noalloc: learned_init needs the network's "
+                       "buffers; use cd_poisson_solve_device");
+        return CD_ERR_BAD_ARG;
+    }
+    const NoallocLayout L = noalloc_layout(width, height);
+    if (L.total > kNoallocScratchBytes) {
+        ctx->set_error("cd_poisson_solve_device_noalloc: a " + std::to_string(width) + " x " +
+                       std::to_string(height) + " grid needs " + std::to_string(L.total) +
+                       " bytes of scratch, more than the " +
+                       std::to_string(kNoallocScratchBytes) + " held per device (grids up to " +
+                       std::to_string(CD_NOALLOC_MAX_SIDE) + " x " +
+                       std::to_string(CD_NOALLOC_MAX_SIDE) + ")");
+        return CD_ERR_BAD_ARG;
+    }
+    CD_HIP(ctx, hipSetDevice(ctx->device));
+    unsigned char* base = nullptr;
+    CD_TRY(noalloc_scratch(ctx, &base));
+    SolveWork ws;
+    ws.partials = reinterpret_cast<double*>(base + L.partials);
+    ws.sd = reinterpret_cast<SolveDev*>(base + L.sd);
+    ws.e32 = reinterpret_cast<float*>(base + L.e32);
+    ws.e32b = reinterpret_cast<float*>(base + L.e32b);
+    ws.r = base + L.r;
+    uint8_t* mask = base + L.mask;
+    double* fill = reinterpret_cast<double*>(base + L.fill);
+    long long holes = 0;
+    CD_TRY(cd_hole_mask(ctx, d_rhs, width, height, mask, ws.partials, fill,
+                        reinterpret_cast<double*>(ctx->host_status), &holes));
+    const long long n = (long long)width * height;
+    return cd_solve_device_internal(ctx, d_rhs, d_phi, holes > 0 ? mask : nullptr, n - holes,
+                                    width, height, opts, stats, &ws);
+}
 
 int cd_poisson_solve_device(cd_context* ctx, const double* d_rhs, double* d_phi, int width,
                             int height, const cd_solve_opts* opts, cd_solve_stats* stats) {
