@@ -158,13 +158,25 @@ def fixed_sweeps(a) -> int:
     d_phi = torch.zeros_like(d_b)
     torch.cuda.synchronize()
     # residual rule with tol 0 never stops early: exactly a.sweeps sweeps of the
-    # production (non-tracking) kernel, one FP64 check at the end
+    # production (non-tracking) kernel, one FP64 check at the end; the first
+    # solve warms up, the last `steps` are timed
     o = capi.default_opts(max_sweeps=a.sweeps, stop=capi.STOP_RESIDUAL, tol=0.0,
                           check_every=a.sweeps)
-    ctx.set_profiling(True)
-    st = ctx.poisson_solve_device(d_b.data_ptr(), d_phi.data_ptr(), n, n, o)
-    ms, launches, nbytes = ctx.kernel_stats(capi.KERNEL_SOR_SWEEP)
-    print(json.dumps({"mode": "sweeps", "grid": n, "sweeps": st.sweeps, "sor_ms": ms,
+    best = None
+    for i in range(1 + max(a.steps, 1)):
+        d_phi.zero_()
+        torch.cuda.synchronize()
+        ctx.reset_kernel_stats()
+        ctx.set_profiling(i > 0)
+        st = ctx.poisson_solve_device(d_b.data_ptr(), d_phi.data_ptr(), n, n, o)
+        if i == 0:
+            continue
+        ms, launches, nbytes = ctx.kernel_stats(capi.KERNEL_SOR_SWEEP)
+        if best is None or ms < best[0]:
+            best = (ms, launches, nbytes)
+    ms, launches, nbytes = best
+    print(json.dumps({"mode": "sweeps", "grid": n,
+                      "sweeps": st.sweeps, "sor_ms": ms, "us_per_sweep": 1e3 * ms / st.sweeps,
                       "launches": launches, "avg_launch_us": 1e3 * ms / max(launches, 1),
                       "bytes_per_launch": nbytes / max(launches, 1),
                       "achieved_GBs": nbytes / max(ms, 1e-9) / 1e6}), flush=True)

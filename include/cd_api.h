@@ -201,6 +201,10 @@ typedef struct {
     long long folded_triangles;   /* inverted triangles after the last OT step       */
     int folded_steps;             /* OT steps that ended with a folded mesh          */
     int first_folded_step;        /* index of the first of them (-1: none)           */
+    int unconverged_solves;       /* Poisson solves that reached their sweep cap; the
+                                     design goes on with that solution, as the
+                                     reference does after max_iterations
+                                     (solver.cpp:92), and the run still returns CD_OK */
 } cd_run_stats;
 
 /* The reference driver loop (main.cpp:243-262): up to max_transport_steps

@@ -67,7 +67,7 @@ class RunStats(C.Structure):
                 ("learned_applied", _I), ("learned_rejected", _I),
                 ("max_phi_residual", _D), ("max_h_residual", _D), ("elapsed_ms", _D),
                 ("recoveries", _I), ("folded_triangles", _LL), ("folded_steps", _I),
-                ("first_folded_step", _I)]
+                ("first_folded_step", _I), ("unconverged_solves", _I)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

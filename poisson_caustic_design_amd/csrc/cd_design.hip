@@ -1189,6 +1189,7 @@ int cd_design_run(cd_design* d, int max_transport_steps, double conv_tres, int h
         r.learned_applied += d->last_phi.learned_applied;
         r.learned_rejected += d->last_phi.learned_rejected;
         if (d->last_phi.rel_residual > r.max_phi_residual) r.max_phi_residual = d->last_phi.rel_residual;
+        if (d->last_phi.status == CD_ERR_NOT_CONVERGED) r.unconverged_solves++;
         if (st != CD_OK) { status = st; break; }
         if (step < conv_tres) break;
     }
@@ -1202,6 +1203,7 @@ int cd_design_run(cd_design* d, int max_transport_steps, double conv_tres, int h
         r.learned_applied += d->last_h.learned_applied;
         r.learned_rejected += d->last_h.learned_rejected;
         if (d->last_h.rel_residual > r.max_h_residual) r.max_h_residual = d->last_h.rel_residual;
+        if (d->last_h.status == CD_ERR_NOT_CONVERGED) r.unconverged_solves++;
         if (st != CD_OK) { status = st; break; }
     }
     r.status = status;

@@ -128,6 +128,7 @@ struct cd_context {
     cd_net* net = nullptr;
     DevBuf lc_r, lc_x, lc_y, lc_stats;
     DevBuf fcg_p, fcg_q, fcg_dev;  // learned FCG: direction, L direction, coefficients
+    DevBuf phi_start;              // learned FCG: the warm start (FP64)
     // learned FCG intermediate grid: correction ping-pong, BF16 residual, a
     // device 0.0f (the mean the intermediate sweeps subtract)
     DevBuf fcg_em, fcg_emb, fcg_rm, fcg_zero;

@@ -254,7 +254,9 @@ __global__ void k_check(const double* __restrict__ partials, int nblocks, SolveD
     if (threadIdx.x == 0) {
         double mean = count > 0 ? a / count : 0.0;
         double rn2 = q - count * mean * mean;
-        double rn = sqrt(rn2 > 0 ? rn2 : 0.0);
+        // rounding can make rn2 slightly negative; a NaN must stay NaN (a
+        // non-finite iterate never meets the stop rule)
+        double rn = rn2 > 0.0 ? sqrt(rn2) : (rn2 == rn2 ? 0.0 : rn2);
         double rel = st->bnorm > 0 ? rn / st->bnorm : (rn > 0 ? INFINITY : 0.0);
         st->rsum = a;
         st->rnorm = rn;
@@ -370,15 +372,17 @@ constexpr int BK_RY = BK_NW * BK_R;
 constexpr bool BK_RV_LDS = CD_BK_RV_LDS != 0;  // right-hand side in LDS (else registers)
 
 // Right-hand side of the region: registers of the owning thread, or LDS
-// (frees VGPRs for occupancy).  Element (colour c, region row y, lane).
-struct RhsRegs {
-    float v[2][BK_R];
-    float (*lds)[BK_RY][64];  // unused
+// (frees VGPRs for occupancy).  Element (colour c, region row y, lane);
+// R rows per wave, RY rows per region.
+template <int R, int RY>
+struct RhsRegsT {
+    float v[2][R];
     __device__ __forceinline__ float get(int c, int k, int, int) const { return v[c][k]; }
     __device__ __forceinline__ void set(int c, int k, int, int, float x) { v[c][k] = x; }
 };
-struct RhsLds {
-    float (*lds)[BK_RY][64];
+template <int RY>
+struct RhsLdsT {
+    float (*lds)[RY][64];
     __device__ __forceinline__ float get(int c, int k, int dyw, int lane) const {
         return lds[c][dyw + k][lane];
     }
@@ -386,6 +390,8 @@ struct RhsLds {
         lds[c][dyw + k][lane] = x;
     }
 };
+using RhsRegs = RhsRegsT<BK_R, BK_RY>;
+using RhsLds = RhsLdsT<BK_RY>;
 
 // value of lane l-1 (0 in lane 0) / lane l+1 (0 in lane 63): DPP wave shifts
 __device__ __forceinline__ float lane_prev(float v) {
@@ -395,37 +401,83 @@ __device__ __forceinline__ float lane_next(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, true));
 }
 
-template <int C, int H, bool EDGE, bool TRACK, typename RV>
-__device__ __forceinline__ void blocked_half(float (&ev)[2][BK_R], const RV& rv, int lane,
-                                             float up_row, float down_row, int s, int dx0,
-                                             int dx1, int dyw, bool l0, bool l1, bool r0, bool r1,
-                                             bool in0, bool in1, int Yw, int h, float om2,
-                                             float om3, float om4, float& mx) {
+// Per-lane facts of a region for the sweep helpers: packed column `lane`
+// holds cells x0 = X0 + 2 lane and x0 + 1 of rows Yw .. Yw + R - 1.
+struct SweepGeom {
+    int lane, dx0, dx1;     // region-border distances of the lane's two columns
+    int dyw, Yw, h;         // first region row and grid row of the wave, grid height
+    bool l0, l1, r0, r1;    // the two cells have a left / right neighbour in the grid
+    bool in0, in1;          // the two cells lie in the grid
+    float om2, om3, om4;    // omega / cnt
+};
+
+__device__ __forceinline__ SweepGeom sweep_geom(int X0, int RX, int Yw, int dyw, int w, int h,
+                                                float om2, float om3, float om4) {
+    SweepGeom g;
+    g.lane = threadIdx.x & 63;
+    const int x0 = X0 + 2 * g.lane;
+    g.dx0 = min(2 * g.lane, RX - 1 - 2 * g.lane);
+    g.dx1 = min(2 * g.lane + 1, RX - 2 - 2 * g.lane);
+    g.dyw = dyw;
+    g.Yw = Yw;
+    g.h = h;
+    g.in0 = (unsigned)x0 < (unsigned)w;
+    g.in1 = (unsigned)(x0 + 1) < (unsigned)w;
+    g.l0 = x0 > 0;
+    g.l1 = x0 + 1 > 0;
+    g.r0 = x0 < w - 1;
+    g.r1 = x0 + 1 < w - 1;
+    g.om2 = om2;
+    g.om3 = om3;
+    g.om4 = om4;
+    return g;
+}
+
+// Rows K0 <= k < K1 of one half-sweep of colour C.  Only rows 0 and R-1 read
+// up_row / down_row (the neighbouring waves' rows).
+template <int C, int H, bool EDGE, bool TRACK, int R, int RY, int K0, int K1, typename RV>
+__device__ __forceinline__ void blocked_half(float (&ev)[2][R], const RV& rv, const SweepGeom g,
+                                             float up_row, float down_row, float& mx) {
 #pragma unroll
-    for (int k = 0; k < BK_R; ++k) {
-        constexpr int dummy = 0;
-        (void)dummy;
+    for (int k = K0; k < K1; ++k) {
         const int p = (k + C) & 1;  // column parity of colour-C cells in row k (R even)
         const float o = ev[1 - C][k];
         const float left = p ? o : lane_prev(o);
         const float right = p ? lane_next(o) : o;
         const float up = k > 0 ? ev[1 - C][k - 1] : up_row;
-        const float down = k < BK_R - 1 ? ev[1 - C][k + 1] : down_row;
-        const int dy = min(dyw + k, BK_RY - 1 - (dyw + k));
-        const int dx = p ? dx1 : dx0;
+        const float down = k < R - 1 ? ev[1 - C][k + 1] : down_row;
+        const int dy = min(g.dyw + k, RY - 1 - (g.dyw + k));
+        const int dx = p ? g.dx1 : g.dx0;
         const float v = ev[C][k];
         if (EDGE) {
-            const int Y = Yw + k;
-            const bool lu = Y > 0, ld = Y < h - 1;
-            const bool ll = p ? l1 : l0, lr = p ? r1 : r0;
-            const bool upd = dx >= s && dy >= s && (p ? in1 : in0) && Y >= 0 && Y < h;
-            const float sum =
-                (ll ? left : 0.f) + (lu ? up : 0.f) + (lr ? right : 0.f) + (ld ? down : 0.f);
-            const int n = (int)ll + (int)lu + (int)lr + (int)ld;
-            const float oc = n == 4 ? om4 : (n == 3 ? om3 : om2);
-            const float dl = oc * (sum - (float)n * v - rv.get(C, k, dyw, lane));
-            ev[C][k] = upd ? v + dl : v;
-            if (TRACK && upd && dx >= H && dy >= H) mx = fmaxf(mx, fabsf(dl));
+            // A neighbour outside the grid contributes the cell's own value:
+            // the sum is then the sum over the in-grid neighbours plus
+            // (4 - cnt) v, so fma(-4, v, sum) = sum_in - cnt v, the reference
+            // operator (solver.cpp:29-47), and omega/cnt (om4 / om3 / om2)
+            // scales the update.  Cells outside the grid keep their value (0);
+            // they are never read by cells inside it.  The grid border lies at
+            // least H cells inside the region, so, as in the interior path,
+            // every in-grid cell is updated: stale values near the region
+            // border never reach the stored interior.  A cell with four
+            // neighbours gets the same operations, hence the same bits, as in
+            // the interior path.  (Branch-free: uniform per-row branches here
+            // keep the compiler from interleaving the rows' dependent chains
+            // and measured 13% slower.)
+            const int Y = g.Yw + k;
+            const bool lu = Y > 0, ld = Y < g.h - 1;  // uniform over the wave
+            const bool ll = p ? g.l1 : g.l0, lr = p ? g.r1 : g.r0;
+            const bool in = (p ? g.in1 : g.in0) && Y >= 0 && Y < g.h;
+            const float sum = (ll ? left : v) + (lu ? up : v) + (lr ? right : v) + (ld ? down : v);
+            const int miss = (int)!ll + (int)!lr + (int)!lu + (int)!ld;
+            const float oc = miss == 0 ? g.om4 : (miss == 1 ? g.om3 : g.om2);
+            const float t = __builtin_fmaf(-4.f, v, sum) - rv.get(C, k, g.dyw, g.lane);
+            if (TRACK) {
+                const float dl = oc * t;
+                ev[C][k] = in ? v + dl : v;
+                if (in && dx >= H && dy >= H) mx = fmaxf(mx, fabsf(dl));
+            } else {
+                ev[C][k] = in ? __builtin_fmaf(oc, t, v) : v;
+            }
         } else {
             // Interior region: every cell is updated.  A cell at distance d < s
             // from the region border takes stale neighbours, but after
@@ -434,50 +486,50 @@ __device__ __forceinline__ void blocked_half(float (&ev)[2][BK_R], const RV& rv,
             // reach the stored interior (distance >= H >= nhalf) and stay
             // finite (each update is a bounded combination of finite values).
             const float t = __builtin_fmaf(-4.f, v, left + up + right + down) -
-                            rv.get(C, k, dyw, lane);
+                            rv.get(C, k, g.dyw, g.lane);
             if (TRACK) {
-                const float dl = om4 * t;
+                const float dl = g.om4 * t;
                 ev[C][k] = v + dl;
                 if (dx >= H && dy >= H) mx = fmaxf(mx, fabsf(dl));
             } else {
-                ev[C][k] = __builtin_fmaf(om4, t, v);
+                ev[C][k] = __builtin_fmaf(g.om4, t, v);
             }
         }
     }
 }
 
-template <int C, int H, bool EDGE, bool TRACK, typename RV>
-__device__ __forceinline__ void blocked_step(float (&ev)[2][BK_R], const RV& rv,
-                                             float (*xch)[BK_NW][2][64], int lane, int wave, int s,
-                                             int dx0, int dx1, int dyw, bool l0, bool l1, bool r0,
-                                             bool r1, bool in0, bool in1, int Yw, int h,
-                                             float om2, float om3, float om4, float& mx) {
+template <int C, int H, bool EDGE, bool TRACK, int R, int NW, typename RV>
+__device__ __forceinline__ void blocked_step(float (&ev)[2][R], const RV& rv,
+                                             float (*xch)[NW][2][64], const SweepGeom g,
+                                             int wave, float& mx) {
+    const int lane = g.lane;
     // publish this wave's first/last row of the colour the update reads
     xch[1 - C][wave][0][lane] = ev[1 - C][0];
-    xch[1 - C][wave][1][lane] = ev[1 - C][BK_R - 1];
+    xch[1 - C][wave][1][lane] = ev[1 - C][R - 1];
+    // the inner rows need only this wave's registers: update them while the
+    // other waves reach the barrier (the order of the updates of one colour
+    // does not matter: they read only the other colour)
+    blocked_half<C, H, EDGE, TRACK, R, R * NW, 1, R - 1>(ev, rv, g, 0.f, 0.f, mx);
     __syncthreads();
     const float up_row = xch[1 - C][wave > 0 ? wave - 1 : 0][1][lane];
-    const float down_row = xch[1 - C][wave < BK_NW - 1 ? wave + 1 : BK_NW - 1][0][lane];
-    blocked_half<C, H, EDGE, TRACK>(ev, rv, lane, up_row, down_row, s, dx0, dx1, dyw, l0, l1, r0,
-                                    r1, in0, in1, Yw, h, om2, om3, om4, mx);
+    const float down_row = xch[1 - C][wave < NW - 1 ? wave + 1 : NW - 1][0][lane];
+    blocked_half<C, H, EDGE, TRACK, R, R * NW, 0, 1>(ev, rv, g, up_row, down_row, mx);
+    blocked_half<C, H, EDGE, TRACK, R, R * NW, R - 1, R>(ev, rv, g, up_row, down_row, mx);
 }
 
-template <int H, bool EDGE, bool TRACK, typename RV>
-__device__ __forceinline__ void blocked_run(float (&ev)[2][BK_R], const RV& rv,
-                                            float (*xch)[BK_NW][2][64], int lane, int wave, int nhalf,
-                                            int dx0, int dx1, int dyw, bool l0, bool l1, bool r0,
-                                            bool r1, bool in0, bool in1, int Yw, int h, float om2,
-                                            float om3, float om4, float (&mxs)[H / 2]) {
+// nhalf <= H half-sweeps, colours alternating from 0; mxs[j] receives the
+// largest |update| of sweep j over the cells the launch stores (TRACK)
+template <int H, bool EDGE, bool TRACK, int R, int NW, typename RV>
+__device__ __forceinline__ void blocked_run(float (&ev)[2][R], const RV& rv,
+                                            float (*xch)[NW][2][64], const SweepGeom g, int wave,
+                                            int nhalf, float (&mxs)[H / 2]) {
 #pragma unroll
     for (int j = 0; j < H / 2; ++j) {
         const int s = 2 * j + 1;
         if (s > nhalf) break;
         float mx = 0.f;
-        blocked_step<0, H, EDGE, TRACK>(ev, rv, xch, lane, wave, s, dx0, dx1, dyw, l0, l1, r0, r1,
-                                        in0, in1, Yw, h, om2, om3, om4, mx);
-        if (s + 1 <= nhalf)
-            blocked_step<1, H, EDGE, TRACK>(ev, rv, xch, lane, wave, s + 1, dx0, dx1, dyw, l0, l1,
-                                            r0, r1, in0, in1, Yw, h, om2, om3, om4, mx);
+        blocked_step<0, H, EDGE, TRACK, R, NW>(ev, rv, xch, g, wave, mx);
+        if (s + 1 <= nhalf) blocked_step<1, H, EDGE, TRACK, R, NW>(ev, rv, xch, g, wave, mx);
         mxs[j] = mx;
     }
 }
@@ -561,12 +613,11 @@ __global__ __launch_bounds__(BK_THREADS) void k_rb_blocked(const float* __restri
 #pragma unroll
     for (int j = 0; j < H / 2; ++j) mxs[j] = 0.f;
     const bool edge = X0 < 1 || X0 + BK_RX > w - 1 || Y0 < 1 || Y0 + BK_RY > h - 1;
+    const SweepGeom g = sweep_geom(X0, BK_RX, Yw, dyw, w, h, om2, om3, om4);
     if (edge)
-        blocked_run<H, true, TRACK>(ev, rv, xch, lane, wave, nhalf, dx0, dx1, dyw, l0, l1, r0, r1,
-                                    in0, in1, Yw, h, om2, om3, om4, mxs);
+        blocked_run<H, true, TRACK, BK_R, BK_NW>(ev, rv, xch, g, wave, nhalf, mxs);
     else
-        blocked_run<H, false, TRACK>(ev, rv, xch, lane, wave, nhalf, dx0, dx1, dyw, l0, l1, r0,
-                                     r1, in0, in1, Yw, h, om2, om3, om4, mxs);
+        blocked_run<H, false, TRACK, BK_R, BK_NW>(ev, rv, xch, g, wave, nhalf, mxs);
     // write the interior
 #pragma unroll
     for (int k = 0; k < BK_R; ++k) {
@@ -1288,13 +1339,22 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     int k = kmax;
     if (adaptive) k = std::min(kmax, std::max(8, kmax / 4));
     bool done = ctx->host_status->done != 0;
+    // the warm start, kept while learned iterations run: if they are abandoned
+    // with a residual above the starting one (degenerate right-hand sides of
+    // folded meshes can do that: a lower energy norm of the error does not
+    // bound the residual), plain SOR restarts from it
+    const double start_rel = ctx->host_status->rel;
     if (learned && !done) {
         CD_TRY(cd_fill_f32(ctx, ctx->fcg_p.as<float>(), npk, 0.f));
         CD_TRY(cd_fill_f32(ctx, ctx->fcg_q.as<float>(), npk, 0.f));
         CD_TRY(cd_fill_f64(ctx, ctx->fcg_dev.as<double>(), sizeof(FcgDev) / sizeof(double), 0.0));
+        CD_TRY(ctx->ensure(ctx->phi_start, sizeof(double) * n));
+        CD_HIP(ctx, hipMemcpyAsync(ctx->phi_start.ptr, phi, sizeof(double) * n,
+                                   hipMemcpyDeviceToDevice, s));
     }
     // learned FCG iterations; abandoned for plain SOR when the residual stops
-    // falling (the iterate is kept: every step lowered the error's energy norm)
+    // falling (the iterate is kept when its residual is below the starting
+    // one: every step lowered the error's energy norm)
     double best_rel = ctx->host_status->rel;
     int stall = 0;
     // one FCG iteration up to the next check: restriction, network, prolongation,
@@ -1391,8 +1451,9 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         (void)fd;
         const SolveDev& hs = *ctx->host_status;
         done = hs.done != 0;
-        if (!std::isfinite(hs.rel)) break;
-        if (capture && !done && check_id % std::max(ctx->cap_every, 1) == 0)
+        if (!std::isfinite(hs.rel)) stall = 7;  // abandoned below, back to the warm start
+        if (capture && !done && std::isfinite(hs.rel) &&
+            check_id % std::max(ctx->cap_every, 1) == 0)
             CD_TRY(capture_pair());
 #ifdef CD_TRACE_LEARNED
         {
@@ -1410,8 +1471,16 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         } else if (++stall >= 8) {
             ++learned_rejected;
             learned = false;
+            if (!(hs.rel <= start_rel) && !done) {
+                // back to the warm start; the next check recomputes its residual
+                // (and folds the zeroed corrections)
+                CD_HIP(ctx, hipMemcpyAsync(phi, ctx->phi_start.ptr, sizeof(double) * n,
+                                           hipMemcpyDeviceToDevice, s));
+                CD_TRY(enqueue_check(++check_id));
+                done = ctx->host_status->done != 0;
+            }
         }
-        prev_rel = hs.rel;
+        prev_rel = ctx->host_status->rel;
     }
     // Max-update rule in mixed precision: the sweeps see the right-hand side
     // rounded (BF16 / FP32) at the last check, which perturbs every update by

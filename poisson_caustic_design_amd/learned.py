@@ -200,11 +200,41 @@ def fit(ctx: capi.Context, arch: capi.NetArch, mesh_res: int, seeds, config: str
     return net, np.concatenate(losses), pool
 
 
+def design_errors(f, ref, width: float) -> dict:
+    """Final-design differences of f from ref: vertex positions / width,
+    vertex heights / max height, phi and h mean-free (relative max)."""
+    mf = lambda g: g - g.mean()
+    return {"xy": float(np.abs(f["tp"][:, :2] - ref["tp"][:, :2]).max() / width),
+            "z": float(np.abs(f["sp"][:, 2] - ref["sp"][:, 2]).max() /
+                       max(np.abs(ref["sp"][:, 2]).max(), 1e-300)),
+            "phi": float(np.abs(mf(f["phi"]) - mf(ref["phi"])).max() /
+                         max(np.abs(mf(ref["phi"])).max(), 1e-300)),
+            "h": float(np.abs(mf(f["h"]) - mf(ref["h"])).max() /
+                       max(np.abs(mf(ref["h"])).max(), 1e-300))}
+
+
+def run_design(ctx: capi.Context, img, prm, opts: capi.SolveOpts, net=None, max_ot: int = 50):
+    """One complete design (main.cpp:243-262 loop); its run statistics and
+    final fields."""
+    ctx.set_net(net)
+    try:
+        cd = design_for(ctx, img, prm, opts)
+        rs = cd.run(max_ot, float(np.float32(0.01)), 3,
+                    allow=(capi.OK, capi.RASTER_MISS, capi.NOT_CONVERGED))
+        out = {"tp": cd.get("target_points"), "sp": cd.get("source_points"),
+               "phi": cd.get("phi"), "h": cd.get("h")}
+        cd.close()
+    finally:
+        ctx.set_net(None)
+    return rs, out
+
+
 def evaluate(ctx: capi.Context, net: capi.Net, config: str, seeds, mesh_res: int,
              tol: float = 1e-6, log=print, control: bool = False) -> dict:
     """Full designs on held-out seeds, plain SOR and learned form on the same
     GPU at the same residual tolerance: per-design times, sweeps, iterations,
-    residuals and the final-design difference between the two forms."""
+    residuals, the inverted triangles each form's final mesh holds, and the
+    final-design difference between the two forms."""
     rows = []
     conv = float(np.float32(0.01))
     for i, seed in enumerate(seeds):
@@ -224,7 +254,11 @@ def evaluate(ctx: capi.Context, net: capi.Net, config: str, seeds, mesh_res: int
             rec[key] = {"ms": rs.elapsed_ms, "status": rs.status, "ot_steps": rs.transport_steps,
                         "sweeps": rs.phi_sweeps + rs.h_sweeps, "iterations": rs.learned_applied,
                         "fallbacks": rs.learned_rejected,
-                        "max_residual": max(rs.max_phi_residual, rs.max_h_residual)}
+                        "max_residual": max(rs.max_phi_residual, rs.max_h_residual),
+                        "unconverged_solves": int(rs.unconverged_solves),
+                        "folded_triangles": int(rs.folded_triangles),
+                        "folded_steps": int(rs.folded_steps),
+                        "first_folded_step": int(rs.first_folded_step)}
             final[use] = (cd.get("target_points"), cd.get("source_points"))
             cd.close()
         ctx.set_net(None)
@@ -250,14 +284,82 @@ def evaluate(ctx: capi.Context, net: capi.Net, config: str, seeds, mesh_res: int
         "speedup_median": float(np.median([r["plain"]["ms"] / r["learned"]["ms"] for r in ok])) if ok else None,
         "learned_iterations_mean": float(np.mean([r["learned"]["iterations"] for r in ok])) if ok else None,
         "fallbacks": int(sum(r["learned"]["fallbacks"] for r in ok)),
+        "unconverged_solves": {"plain": int(sum(r["plain"]["unconverged_solves"] for r in ok)),
+                               "learned": int(sum(r["learned"]["unconverged_solves"] for r in ok))},
         "worst_residual_plain": max((r["plain"]["max_residual"] for r in ok), default=None),
         "worst_residual_learned": max((r["learned"]["max_residual"] for r in ok), default=None),
         "max_xy_diff_over_width": max((r["max_xy_diff_over_width"] for r in ok), default=None),
         "max_height_diff_rel": max((r["max_height_diff_rel"] for r in ok), default=None),
         "median_xy_diff_over_width": float(np.median([r["max_xy_diff_over_width"] for r in ok]))
         if ok else None,
+        "outliers": outlier_summary(ok),
         "rows": rows,
     }
+
+
+def outlier_summary(rows, threshold: float = 1e-3) -> dict:
+    """Designs whose two forms end more than `threshold` widths apart, and
+    which of them hold inverted triangles in the plain form, the learned form
+    or both (a folded mesh is a degenerate design: the reference steps interior
+    vertices unclamped, mesh.cpp:485-532)."""
+    out = [r for r in rows if r["max_xy_diff_over_width"] > threshold]
+    fp = lambda r: r["plain"]["folded_triangles"] > 0 or r["plain"]["folded_steps"] > 0
+    fl = lambda r: r["learned"]["folded_triangles"] > 0 or r["learned"]["folded_steps"] > 0
+    res = {"threshold_width": threshold, "count": len(out),
+           "folded_both": sum(1 for r in out if fp(r) and fl(r)),
+           "folded_plain_only": sum(1 for r in out if fp(r) and not fl(r)),
+           "folded_learned_only": sum(1 for r in out if fl(r) and not fp(r)),
+           "unfolded": sum(1 for r in out if not fp(r) and not fl(r)),
+           "folded_designs_all": {"plain": sum(1 for r in rows if fp(r)),
+                                  "learned": sum(1 for r in rows if fl(r))},
+           "seeds": [r["seed"] for r in out]}
+    if "control_max_xy_diff_over_width" in (out[0] if out else {}):
+        res["within_10x_control"] = sum(
+            1 for r in out
+            if r["max_xy_diff_over_width"] <= 10 * r["control_max_xy_diff_over_width"])
+    return res
+
+
+def equal_error(ctx: capi.Context, net: capi.Net, config: str, seeds, mesh_res: int,
+                ref_tol: float, tols=(1e-6, 1e-7, 1e-8, 1e-9), max_ot: int = 50,
+                log=print) -> dict:
+    """Plain SOR at 1e-6 and the learned form at each of `tols`, against the
+    FP64 reference design (FP64 sweeps to relative residual ref_tol): per
+    design the errors, times and the tightest-needed learned tolerance at which
+    every error (positions, heights, phi, h) is at most plain's."""
+    rows = []
+    for seed in seeds:
+        img, prm = synth.target_for(config, int(seed), mesh_res)
+        W = prm["width"]
+        rr, ref = run_design(ctx, img, prm, capi.default_opts(tol=ref_tol,
+                                                              precision=capi.PREC_FP64),
+                             max_ot=max_ot)
+        rp, plain = run_design(ctx, img, prm, capi.default_opts(tol=1e-6), max_ot=max_ot)
+        ep = design_errors(plain, ref, W)
+        rec = {"seed": int(seed), "reference": {"ms": rr.elapsed_ms, "ot_steps": rr.transport_steps,
+                                                "status": rr.status,
+                                                "max_residual": max(rr.max_phi_residual,
+                                                                    rr.max_h_residual)},
+               "plain": {"ms": rp.elapsed_ms, "ot_steps": rp.transport_steps, "errors": ep},
+               "learned": {}}
+        chosen = None
+        for t in tols:
+            rl, lf = run_design(ctx, img, prm, capi.default_opts(tol=t, learned_init=1), net,
+                                max_ot=max_ot)
+            el = design_errors(lf, ref, W)
+            rec["learned"][f"{t:g}"] = {"ms": rl.elapsed_ms, "ot_steps": rl.transport_steps,
+                                        "iterations": rl.learned_applied,
+                                        "fallbacks": rl.learned_rejected, "errors": el}
+            if chosen is None and all(el[k] <= ep[k] for k in ep):
+                chosen = t
+        rec["equal_error_tol"] = chosen
+        if chosen is not None:
+            rec["speedup_at_equal_error"] = rp.elapsed_ms / rec["learned"][f"{chosen:g}"]["ms"]
+        rows.append(rec)
+        if log:
+            log(f"equal error seed {seed}: plain {ep}, chosen learned tol {chosen}")
+    return {"config": config, "mesh_res": mesh_res, "reference_tol": ref_tol,
+            "max_ot_steps": max_ot, "rows": rows}
 
 
 def save_params(net: capi.Net, path: str, meta: dict):
@@ -316,12 +418,29 @@ def main(argv=None):
     ap.add_argument("--eval-out", default=None)
     ap.add_argument("--control", action="store_true",
                     help="also run plain SOR at half the tolerance (design-loop sensitivity)")
+    ap.add_argument("--equal-error", default=None, metavar="FIRST:COUNT",
+                    help="with --weights: plain at 1e-6 and learned at 1e-6..1e-9 against the "
+                         "FP64 reference design (--ref-tol) on COUNT seeds")
+    ap.add_argument("--ref-tol", type=float, default=2e-9)
+    ap.add_argument("--max-ot", type=int, default=50,
+                    help="transport steps of the equal-error designs (the reference loop: 50)")
     a = ap.parse_args(argv)
 
     def log(*m):
         print(*m, file=sys.stderr, flush=True)
 
     ctx = capi.Context(0)
+    if a.weights and a.equal_error:
+        first, count = (int(v) for v in a.equal_error.split(":"))
+        net = load_net(ctx, a.weights)
+        res = equal_error(ctx, net, a.config, range(first, first + count), a.mesh_res,
+                          a.ref_tol, max_ot=a.max_ot, log=log)
+        res.update(weights=a.weights)
+        if a.eval_out:
+            with open(a.eval_out, "w") as f:
+                json.dump(res, f, indent=1)
+        print(json.dumps(res), flush=True)
+        return
     if a.weights and a.evaluate:
         first, count = (int(v) for v in a.evaluate.split(":"))
         net = load_net(ctx, a.weights)

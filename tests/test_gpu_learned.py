@@ -173,6 +173,30 @@ def test_learned_solve_with_untrained_net_still_converges(ctx):
     assert np.abs((phi - phi.mean()) - ex).max() <= 1e-4 * np.abs(ex).max()
 
 
+def test_learned_solve_that_blows_up_restarts_from_the_warm_start(ctx):
+    """A network whose output overflows makes the learned iterations' residual
+    non-finite: they are abandoned, the solve goes back to its warm start and
+    plain SOR finishes to the tolerance, as many sweeps as plain SOR alone
+    needs from there (plus the abandoned cycles' smoothing sweeps)."""
+    net = capi.Net(ctx)
+    net.set_params(NetRef(linear=True).random_params(11, head_scale=3.0) * 1e20)
+    b = smooth_rhs(np.random.default_rng(5), 256, 256)
+    p0 = dct_solve(b) * 0.9  # a warm start the blow-up would otherwise destroy
+    plain, sp = ctx.poisson_solve(b, p0, capi.default_opts(tol=1e-6))
+    ctx.set_net(net)
+    try:
+        phi, st = ctx.poisson_solve(b, p0, capi.default_opts(tol=1e-6, learned_init=1))
+    finally:
+        ctx.set_net(None)
+    print(f"plain {sp.sweeps} sweeps; learned {st.learned_applied} iterations, "
+          f"{st.learned_rejected} abandoned, {st.sweeps} sweeps, rel {st.rel_residual:.2e}")
+    assert st.status == capi.OK and st.rel_residual <= 1e-6
+    assert st.learned_rejected == 1
+    assert st.sweeps <= sp.sweeps + 16 * (st.learned_applied + 1)
+    ex = dct_solve(b)
+    assert np.abs((phi - phi.mean()) - ex).max() <= 1e-4 * np.abs(ex).max()
+
+
 def test_intermediate_grid_preconditioner_converges(ctx):
     """mid_sweeps > 0: coarse correction -> intermediate grid (coarsen/4 cells
     per cell) smoothed there -> fine grid.  A non-square grid, an untrained

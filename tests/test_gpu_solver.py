@@ -50,6 +50,25 @@ def test_fp32_blocked_iterates_match_oracle(ctx, oracle, rng, w, h, k, every):
     assert np.abs(g - ref).max() <= 1e-2 * scale, (w, h, np.abs(g - ref).max(), scale)
 
 
+@pytest.mark.parametrize("w,h,k,every", [(64, 64, 13, 5), (300, 130, 9, 4), (129, 77, 22, 7),
+                                         (1000, 50, 6, 6), (520, 390, 31, 31)])
+def test_fp32_residual_rule_iterates_match_oracle(ctx, oracle, rng, w, h, k, every):
+    """The residual rule's sweeps (BF16 right-hand side, non-tracking blocked
+    kernel, `every` sweeps between FP64 checks): after k sweeps the iterate
+    equals k red-black sweeps of the oracle to the BF16 rounding of the
+    right-hand side, across launch and tile boundaries (an indexing error is
+    O(1))."""
+    b = rng.standard_normal((h, w)); b -= b.mean()
+    p0 = 0.1 * rng.standard_normal((h, w))
+    o = capi.default_opts(max_sweeps=k, stop=capi.STOP_RESIDUAL, tol=0.0,
+                          precision=capi.PREC_MIXED, omega=-1.0, check_every=every)
+    g, st = ctx.poisson_solve(b, p0, o)
+    ref, _ = oracle.poisson_sor(b, p0, k, max_update_tol=1e-300, ordering=RED_BLACK)
+    assert st.sweeps == k
+    scale = np.abs(ref - p0).max()
+    assert np.abs(g - ref).max() <= 1e-2 * scale, (w, h, np.abs(g - ref).max(), scale)
+
+
 @pytest.mark.parametrize("precision", [capi.PREC_MIXED, capi.PREC_FP64])
 @pytest.mark.parametrize("w,h", [(256, 256), (129, 77), (64, 200)])
 def test_solve_to_tolerance_matches_dct(ctx, rng, precision, w, h):
