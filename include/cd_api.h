@@ -372,6 +372,11 @@ int cd_net_forward(cd_net* net, const float* d_x, float* d_y, int batch, int h, 
  * Both give bit-identical outputs; the switch exists to check that.  No
  * counterpart in the reference (the network is new). */
 int cd_net_set_fused_forward(cd_net* net, int on);
+/* Forward convolutions (training and inference) on 8x8 output tiles whose
+ * input and weights are staged in LDS (on != 0, the default) or with
+ * per-chunk global gathers (0).  Bit-identical outputs; the switch exists to
+ * check that.  No counterpart in the reference. */
+int cd_net_set_tiled_forward(cd_net* net, int on);
 /* Mean squared error of the forward pass against d_t and its gradient with
  * respect to every parameter (FP32, host order of cd_net_get_params). */
 int cd_net_loss_grad(cd_net* net, const float* d_x, const float* d_t, int batch, int h, int w,

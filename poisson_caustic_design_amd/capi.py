@@ -124,6 +124,7 @@ def load(path: str = LIB_PATH):
         "cd_design_run": ([_P, _I, _D, _I, C.POINTER(RunStats)], _I),
         "cd_designs_run_batch": ([_P, _P, _I, _I, _D, _I, C.POINTER(RunStats)], _I),
         "cd_net_set_fused_forward": ([_P, _I], _I),
+        "cd_net_set_tiled_forward": ([_P, _I], _I),
         "cd_design_field_size": ([_P, _I], _SZ),
         "cd_design_get": ([_P, _I, _P, _SZ], _I),
         "cd_design_set": ([_P, _I, _P, _SZ], _I),
@@ -190,7 +191,8 @@ def exported_symbols():
         "cd_json_write_heightmap", "cd_svg_write_grid", "cd_svg_write_triangles",
         "cd_design_save_solid_obj", "cd_design_save_heightmap_json", "cd_design_export_svg",
         "cd_design_laplacian_smoothing", "cd_design_set_miss_recovery",
-        "cd_designs_run_batch", "cd_net_set_fused_forward", "cd_design_set_fold_check",
+        "cd_designs_run_batch", "cd_net_set_fused_forward", "cd_net_set_tiled_forward",
+        "cd_design_set_fold_check",
         "cd_design_count_folds",
     ]
 
@@ -668,6 +670,11 @@ class Net:
         """Inference forwards fold the decoder upsampling into the convolutions
         (default) or materialise it as training does; outputs are identical."""
         self.ctx.check(self.L.cd_net_set_fused_forward(self.h, int(on)), "set_fused_forward")
+
+    def set_tiled_forward(self, on: bool):
+        """Forward convolutions on LDS-staged 8x8 tiles (default) or with
+        per-chunk global gathers; bit-identical outputs."""
+        self.ctx.check(self.L.cd_net_set_tiled_forward(self.h, int(on)), "set_tiled_forward")
 
     def loss_grad(self, d_x: int, d_t: int, batch: int, h: int, w: int):
         loss = _D()

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <fstream>
 #include <string>
+#include <new>
 #include <vector>
 
 #include "cd_api.h"
@@ -221,6 +222,13 @@ int decode_png(const char* path, std::vector<unsigned char>& rgba, uint32_t& w, 
         const uint32_t ph = h > sy ? (h - sy + dy - 1) / dy : 0;
         if (pw && ph) total += (size_t)ph * (((size_t)pw * bits + 7) / 8 + 1);
     }
+    // Refuse sizes before allocating: deflate expands at most ~1032:1, so a
+    // header claiming more filtered bytes than the IDAT data can inflate to is
+    // corrupt (or hostile); and the decoded image is capped at 2^28 pixels
+    // (1 GiB of RGBA), far above any target this path is used for.
+    if ((uint64_t)w * h > (1ull << 28) || total > (uint64_t)pi.idat.size() * 1032 + 1024 ||
+        total > 0xFFFFFFFFull)
+        return CD_ERR_BAD_ARG;
     std::vector<unsigned char> raw(total);
     z_stream zs;
     std::memset(&zs, 0, sizeof(zs));
@@ -301,7 +309,14 @@ int cd_png_read_rgba8(const char* path, int* width, int* height, unsigned char* 
     if (!path || !width || !height) return CD_ERR_BAD_ARG;
     std::vector<unsigned char> img;
     uint32_t w = 0, h = 0;
-    const int st = decode_png(path, img, w, h);
+    int st = CD_OK;
+    try {  // no C++ exception may leave the C ABI
+        st = decode_png(path, img, w, h);
+    } catch (const std::bad_alloc&) {
+        return CD_ERR_OOM;
+    } catch (...) {
+        return CD_ERR_BAD_ARG;
+    }
     if (st != CD_OK) return st;
     *width = (int)w;
     *height = (int)h;

@@ -169,6 +169,112 @@ __global__ __launch_bounds__(256) void k_conv3x3(const u16* __restrict__ x1,
 }
 
 // ---------------------------------------------------------------------------
+// Forward 3x3 convolution on one 8x8 output tile per workgroup (4 waves, two
+// tile rows each).  The tile's input (10x10 pixels with the zero-padded halo,
+// x1 read through nearest 2x upsampling when mode1, then x2 concatenated) and
+// the weights of this workgroup's NB*16 output channels are staged in LDS by
+// one round of coalesced loads; the MFMA loop then reads only LDS.  The A/B
+// fragments and the order of the K chunks are those of k_conv3x3<NB, 0, OUT>,
+// so every output is bit-identical to it; the tiles cut the latency chain of
+// the per-chunk global gathers, which sets the time of these small layers.
+//   OUT 0: BF16 output ybf [B][H][W][cy] (ReLU optional);
+//   OUT 1: FP32 output yf [B][H][W][cy].
+// grid: (ceil(W/8), ceil(H/8), B * cblocks), output channels
+// [16*NB*cblk, 16*NB*(cblk+1)) with cblk = blockIdx.z % cblocks.
+// LDS: 100 * (cin + 8) + NB*16 * (kpad + 8) u16 (row paddings spread banks).
+// ---------------------------------------------------------------------------
+constexpr int CT = 8;  // output tile side
+template <int NB, int OUT>
+__global__ __launch_bounds__(256) void k_conv3x3_tile(const u16* __restrict__ x1,
+                                                      const u16* __restrict__ x2, int c1, int c2,
+                                                      const u16* __restrict__ wt, int kpad, int H,
+                                                      int W, int cblocks, u16* __restrict__ ybf,
+                                                      float* __restrict__ yf, int cy, int relu,
+                                                      int mode1) {
+    extern __shared__ uint4 tile_lds[];
+    u16* xs = reinterpret_cast<u16*>(tile_lds);
+    const int cin = c1 + c2;
+    const int xstride = cin + 8;                      // u16 per tile pixel
+    u16* ws = xs + 100 * xstride;                     // 16-byte aligned: xstride % 8 == 0
+    const int wstride = kpad + 8;
+    const int cblk = blockIdx.z % cblocks, b = blockIdx.z / cblocks;
+    const int tx0 = blockIdx.x * CT, ty0 = blockIdx.y * CT;
+    // stage the input tile: 100 pixels x cin/8 vectors
+    const int ng = cin >> 3;
+    for (int it = threadIdx.x; it < 100 * ng; it += 256) {
+        const int px = it / ng, gi = it - px * ng;
+        const int yy = ty0 - 1 + px / 10, xx = tx0 - 1 + px % 10;
+        const int ci = gi * 8;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
+            const long long q = ((long long)b * H + yy) * W + xx;
+            if (ci >= c1) {
+                v = *reinterpret_cast<const uint4*>(x2 + q * c2 + (ci - c1));
+            } else if (mode1 == 0) {
+                v = *reinterpret_cast<const uint4*>(x1 + q * c1 + ci);
+            } else {
+                const long long q1 = ((long long)b * (H >> 1) + (yy >> 1)) * (W >> 1) + (xx >> 1);
+                v = *reinterpret_cast<const uint4*>(x1 + q1 * c1 + ci);
+            }
+        }
+        *reinterpret_cast<uint4*>(xs + px * xstride + ci) = v;
+    }
+    // stage this block's weights: NB*16 rows of kpad
+    const int kv = kpad >> 3;
+    const u16* wsrc = wt + (size_t)cblk * NB * 16 * kpad;
+    for (int it = threadIdx.x; it < NB * 16 * kv; it += 256) {
+        const int row = it / kv, col = (it - row * kv) * 8;
+        *reinterpret_cast<uint4*>(ws + row * wstride + col) =
+            *reinterpret_cast<const uint4*>(wsrc + (size_t)row * kpad + col);
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // this lane's A pixel: tile row 2*wave + (l%16)/8, column (l%16)%8
+    const int ar = 2 * wave + ((lane & 15) >> 3), ac = lane & 7;
+    const int K = 9 * cin;
+    const int kq = 8 * (lane >> 4);
+    floatx4 acc[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) acc[nb] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const u16* wrow = ws + (lane & 15) * wstride + kq;
+    for (int kc = 0; kc < kpad; kc += 32) {
+        V8 a;
+        a.u = make_uint4(0, 0, 0, 0);
+        const int k = kc + kq;
+        if (k < K) {
+            const int tap = k / cin, ci = k - tap * cin;
+            const int px = (ar + tap / 3) * 10 + ac + tap % 3;
+            a.u = *reinterpret_cast<const uint4*>(xs + px * xstride + ci);
+        }
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            V8 bw;
+            bw.u = *reinterpret_cast<const uint4*>(wrow + nb * 16 * wstride + kc);
+            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, bw.v, acc[nb], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            // D row 4*(l/16) + r is tile pixel (2*wave + row/8, row % 8)
+            const int row = 4 * (lane >> 4) + r;
+            const int yy = ty0 + 2 * wave + (row >> 3), xx = tx0 + (row & 7);
+            const int ch = (cblk * NB + nb) * 16 + (lane & 15);
+            if (yy >= H || xx >= W || ch >= cy) continue;
+            const long long p = ((long long)b * H + yy) * W + xx;
+            float v = acc[nb][r];
+            if (OUT == 0) {
+                if (relu) v = fmaxf(v, 0.f);
+                ybf[p * cy + ch] = to_bf16(v);
+            } else {
+                yf[p * cy + ch] = v;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Weight gradient dW[co][k] += sum_p gm[p][co] * xs[p][k], gm = g*(mask>0),
 // xs the 3x3-shifted (concatenated) input: MFMA with M = output channels,
 // N = k, reduction over pixels in steps of 32.  grid: (co blocks, k blocks,
@@ -565,9 +671,33 @@ struct cd_net {
     int batch = 0, bh = 0, bw = 0;
     bool weights_dirty = true;
     bool fused_forward = true;  // inference forwards fold the decoder upsampling into the convs
+    bool tiled_forward = true;  // forward convolutions on LDS-staged 8x8 tiles (bit-identical)
 };
 
 namespace {
+
+// Dynamic LDS available to k_conv3x3_tile on this context's device (the
+// kernels' limit raised to the device's per-workgroup maximum once).
+size_t tiled_lds_limit(cd_context* c) {
+    static std::atomic<long long> limit{-1};
+    long long v = limit.load();
+    if (v >= 0) return (size_t)v;
+    int bytes = 0;
+    if (hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) !=
+        hipSuccess)
+        bytes = 0;
+    const void* fns[] = {reinterpret_cast<const void*>(&k_conv3x3_tile<1, 0>),
+                         reinterpret_cast<const void*>(&k_conv3x3_tile<2, 0>),
+                         reinterpret_cast<const void*>(&k_conv3x3_tile<1, 1>)};
+    for (const void* f : fns)
+        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) !=
+            hipSuccess) {
+            (void)hipGetLastError();
+            bytes = std::min(bytes, 64 * 1024);
+        }
+    limit.store(bytes);
+    return (size_t)bytes;
+}
 
 // mode1: 0 reads in1 as stored; 1 reads `x1src` (the half-resolution tensor
 // in1 is upsampled from) through nearest 2x upsampling instead
@@ -577,7 +707,39 @@ int launch_conv(cd_net* n, const NetConv& cv, int B, int H, int W, bool grad_inp
     hipStream_t s = c->stream;
     const long long npix = (long long)B * H * W;
     const int blocks = div_up(npix, 64);
-    if (!grad_input) {
+    // LDS the tiled kernel needs for this convolution (8x8 tile + halo, the
+    // weights of up to 32 output channels)
+    const int nbw_t = (cv.coutpad / 16) % 2 == 0 ? 2 : 1;
+    const size_t lds_t = sizeof(u16) * (100 * (size_t)(cv.c1 + cv.c2 + 8) +
+                                        (size_t)nbw_t * 16 * (cv.kpad + 8));
+    if (!grad_input && n->tiled_forward && lds_t <= tiled_lds_limit(c)) {
+        const u16* x1 = mode1 ? x1src
+                              : (cv.in1 < 0 ? n->x8.as<u16>() : n->t[cv.in1].act.as<u16>());
+        const u16* x2 = cv.in2 < 0 ? nullptr : n->t[cv.in2].act.as<u16>();
+        const u16* wt = n->wbf.as<u16>() + cv.woff;
+        const int nb = cv.coutpad / 16;
+        // up to 32 output channels per workgroup (LDS), the rest in more blocks
+        const int nbw = nbw_t;
+        const int cblocks = nb / nbw;
+        const size_t lds = lds_t;
+        const dim3 grid(div_up(W, CT), div_up(H, CT), B * cblocks);
+        if (cv.out < 0) {  // head: FP32 output [npix][8]
+            if (nbw != 1 || cblocks != 1) return CD_ERR_BAD_ARG;
+            hipLaunchKernelGGL((k_conv3x3_tile<1, 1>), grid, dim3(256), lds, s, x1, x2, cv.c1,
+                               cv.c2, wt, cv.kpad, H, W, cblocks, (u16*)nullptr,
+                               n->head.as<float>(), 8, 0, mode1);
+        } else {
+            u16* y = n->t[cv.out].act.as<u16>();
+            if (nbw == 2)
+                hipLaunchKernelGGL((k_conv3x3_tile<2, 0>), grid, dim3(256), lds, s, x1, x2,
+                                   cv.c1, cv.c2, wt, cv.kpad, H, W, cblocks, y, (float*)nullptr,
+                                   cv.coutA, cv.relu, mode1);
+            else
+                hipLaunchKernelGGL((k_conv3x3_tile<1, 0>), grid, dim3(256), lds, s, x1, x2,
+                                   cv.c1, cv.c2, wt, cv.kpad, H, W, cblocks, y, (float*)nullptr,
+                                   cv.coutA, cv.relu, mode1);
+        }
+    } else if (!grad_input) {
         const u16* x1 = mode1 ? x1src
                               : (cv.in1 < 0 ? n->x8.as<u16>() : n->t[cv.in1].act.as<u16>());
         const u16* x2 = cv.in2 < 0 ? nullptr : n->t[cv.in2].act.as<u16>();
@@ -1152,6 +1314,14 @@ int cd_net_set_fused_forward(cd_net* n, int on) {
     if (!n) return CD_ERR_BAD_ARG;
     if (!n->ctx) return CD_ERR_STATE;
     n->fused_forward = on != 0;
+    return CD_OK;
+}
+
+int cd_net_set_tiled_forward(cd_net* n, int on) {
+    if (!n) return CD_ERR_BAD_ARG;
+    if (!n->ctx) return CD_ERR_STATE;
+    n->tiled_forward = on != 0;
+    ++n->gen;  // captured solver graphs hold the previous kernels
     return CD_OK;
 }
 

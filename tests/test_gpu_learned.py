@@ -58,6 +58,34 @@ def test_fused_forward_is_bit_identical(ctx, linear):
     assert np.array_equal(ys[0], ys[1])
 
 
+@pytest.mark.parametrize("channels,shape,linear", [((16, 32, 32, 32), (2, 32, 40), 1),
+                                                   ((16, 32, 64, 64), (1, 128, 128), 1),
+                                                   ((16, 32, 64, 64), (3, 24, 56), 0)])
+def test_tiled_forward_is_bit_identical(ctx, channels, shape, linear):
+    """Forward convolutions on LDS-staged 8x8 tiles (the default) give exactly
+    the output of the per-chunk gather kernel: the architectures of the
+    shipped weights (16/32/32/32 and 16/32/64/64, so 64+64-channel decoder
+    inputs and 64-channel outputs split over two workgroups), batches, and
+    image sides that leave partial tiles at the coarsest level."""
+    arch = capi.default_arch(linear=linear, channels=(*channels, 0, 0))
+    net = capi.Net(ctx, arch)
+    net.set_params(NetRef(4, channels, linear=bool(linear)).random_params(5))
+    b, h, w = shape
+    dx = dev(np.random.default_rng(6).standard_normal(shape))
+    ys = []
+    for tiled in (True, False):
+        net.set_tiled_forward(tiled)
+        for fused in (True, False):
+            net.set_fused_forward(fused)
+            dy = torch.zeros(shape, device="cuda:0")
+            torch.cuda.synchronize()
+            net.forward(dx.data_ptr(), dy.data_ptr(), b, h, w)
+            ys.append(dy.cpu().numpy())
+    assert np.abs(ys[0]).max() > 0
+    for y in ys[1:]:
+        assert np.array_equal(ys[0], y)
+
+
 @pytest.mark.parametrize("linear", [1, 0])
 def test_net_gradient_matches_netref(ctx, linear):
     net = capi.Net(ctx, small_arch(linear))

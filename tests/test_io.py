@@ -156,6 +156,23 @@ def test_corrupt_files_are_rejected(tmp_path):
         capi.png_read_rgba8(str(tmp_path / "missing.png"))
 
 
+def test_oversized_headers_are_rejected_before_allocating(tmp_path):
+    """An IHDR claiming 65536 x 65536 RGBA (16 GiB decoded) over a tiny IDAT,
+    and one whose filtered size exceeds what the IDAT can inflate to, fail
+    with a status instead of an allocation the process cannot survive."""
+    def png(w, h, ctype, idat):
+        return (b"\x89PNG\r\n\x1a\n"
+                + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, ctype, 0, 0, 0))
+                + chunk(b"IDAT", idat) + chunk(b"IEND", b""))
+    for data in (png(65536, 65536, 6, zlib.compress(b"\0" * 64)),
+                 png(4000, 4000, 2, zlib.compress(b"\0" * 64))):
+        p = tmp_path / "big.png"
+        p.write_bytes(data)
+        with pytest.raises(capi.CDError) as e:
+            capi.png_read_rgba8(str(p))
+        assert e.value.status == capi.BAD_ARG
+
+
 def test_ingest_restatement_closed_forms():
     px = np.array([[[255, 0, 0, 255], [0, 255, 0, 0], [0, 0, 255, 9], [255, 255, 255, 1]]],
                   np.uint8)
