@@ -149,6 +149,24 @@ __device__ __forceinline__ size_t brick_row(const Brick& b, int c, int Y) {
     return (size_t)c * b.plane + ((size_t)(Y >> 3) * b.nbx) * 512 + (size_t)(Y & 7) * 64;
 }
 __device__ __forceinline__ unsigned brick_col(int j) { return (unsigned)((j >> 6) * 512 + (j & 63)); }
+// Buffer access to a bricked plane pair of at most 2^31 bytes: scalar
+// resource + scalar (row) byte offset + the lane's vector byte offset, no
+// per-lane 64-bit address arithmetic.  Every access stays in range.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* p, size_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t rs, unsigned voff, unsigned soff);
+template <>
+__device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t rs, unsigned voff,
+                                                 unsigned soff) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ unsigned short buf_load<unsigned short>(__amdgpu_buffer_rsrc_t rs,
+                                                                   unsigned voff, unsigned soff) {
+    return __builtin_amdgcn_raw_buffer_load_b16(rs, voff, soff, 0);
+}
 
 // FP32 -> BF16 (round to nearest even) and back; finite inputs only
 __device__ __forceinline__ unsigned short f32_to_bf16(float x) {
@@ -333,25 +351,37 @@ __global__ __launch_bounds__(256) void k_rb_sweep(T* __restrict__ e, const T* __
 // A workgroup of W waves owns a region of W*R rows x 128 columns; the halo is
 // H cells on every side.  Lane l of wave v keeps packed column l (cells
 // 2l and 2l+1 of every row, one per colour) of rows v*R .. v*R+R-1 in
-// registers, the region's right-hand side sits in LDS.  Up/down neighbours of a cell are the other
-// colour in the rows above/below: registers of the same lane, except at a
-// wave's first/last row, which comes through a small LDS exchange.  Left/right
-// neighbours are the other colour in the same row at packed column l-1+p or
-// l+p (p = column parity of the cell): the same lane or the adjacent lane,
-// fetched with a DPP wave shift.  Half-sweep s (1-based) updates the cells at
-// distance >= s from the region border, whose neighbours are exact at that
-// point; after nhalf <= H half-sweeps the interior (distance >= H) equals
-// nhalf global half-sweeps and is written to the other ping-pong buffer.
+// registers, with the cells' scaled right-hand side.  Up/down neighbours of a
+// cell are the other colour in the rows above/below: registers of the same
+// lane, except at a wave's first/last row, which comes through a small LDS
+// exchange.  Left/right neighbours are the other colour in the same row at
+// packed column l-1+p or l+p (p = column parity of the cell): the same lane or
+// the adjacent lane, fetched with a DPP wave shift folded into the first add.
+// Half-sweep s (1-based) updates the cells at distance >= s from the region
+// border, whose neighbours are exact at that point; after nhalf <= H
+// half-sweeps the interior (distance >= H) equals nhalf global half-sweeps
+// and is written to the other ping-pong buffer.
 // ---------------------------------------------------------------------------
 // Geometry is fixed at compile time; CD_BK_W / CD_BK_R / CD_BK_H / CD_BK_RV_LDS
-// exist to compare variants on the hardware.  Measured at 2048^2 (4 sweeps per
+// exist to compare variants on the hardware.  The kernel is bound by vector
+// ALU issue and by the load phase that precedes the sweeps (a round of
+// resident workgroups loads, then sweeps): the interior update takes five
+// VALU operations per cell (three adds, the first carrying the DPP shift, and
+// two FMAs), border tiles six.  Measured at 2048^2, 5 sweeps per launch:
+// 14.3-14.8 us per launch with these updates against 17.6-18.0 us for the
+// earlier form (fma(-4, v, sum) - r, then fma(oc, t, v); selects on border
+// tiles; 64-bit per-lane addresses); right-hand side in LDS 15.0-15.4 us.
+// Halo (= half-sweeps per launch): H = 6 / 8 / 10 / 12 give 3.76 / 3.27 /
+// 2.94 / 2.99 us per sweep at 2048^2 and 1.96 / 1.56 / 1.36 / 1.45 at 1024^2;
+// 16 waves x 6 rows 3.14 and 1.69.  Earlier geometry comparisons (4 sweeps per
 // launch, same 128 x 64 region): 4 waves x 16 rows 22.1 us, 8 x 8 17.0 us,
-// 12 x 6 17.7 us, 16 x 4 16.8 us -- more, shorter waves overlap the loads and
-// the sweeps of the one round of resident workgroups better; 8 x 12 and
-// 8 x 16 (taller regions, fewer workgroups) 21.7 and 27.0 us.  Halo (= half-
-// sweeps per launch) with 16 x 4: H = 6 3.6 us per sweep inside a design,
-// H = 8 3.9, H = 10 3.7, H = 12 3.7 (plain designs 177 / 154 / 148 / 147 ms);
-// H = 10 keeps the learned iteration's 8 smoothing sweeps at two launches.
+// 12 x 6 17.7 us, 16 x 4 16.8 us; 8 x 12 and 8 x 16 (taller regions, fewer
+// workgroups) 21.7 and 27.0 us.  A persistent variant (one round of
+// workgroups walking the tiles, the next tile's loads issued before the
+// current tile's sweeps) measured 21.1-21.5 us against 17.6-18.0 us: the
+// loop-carried registers cost copies in every half-sweep and the static tile
+// split loses the dispatcher's load balance.  H = 10 also keeps the learned
+// iteration's 8 smoothing sweeps at two launches.
 #ifndef CD_BK_R
 #define CD_BK_R 4
 #endif
@@ -359,7 +389,7 @@ __global__ __launch_bounds__(256) void k_rb_sweep(T* __restrict__ e, const T* __
 #define CD_BK_H 10
 #endif
 #ifndef CD_BK_RV_LDS
-#define CD_BK_RV_LDS 1
+#define CD_BK_RV_LDS 0
 #endif
 #ifndef CD_BK_W
 #define CD_BK_W 16
@@ -377,12 +407,14 @@ constexpr bool BK_RV_LDS = CD_BK_RV_LDS != 0;  // right-hand side in LDS (else r
 template <int R, int RY>
 struct RhsRegsT {
     float v[2][R];
+    __device__ __forceinline__ void bind(float*) {}
     __device__ __forceinline__ float get(int c, int k, int, int) const { return v[c][k]; }
     __device__ __forceinline__ void set(int c, int k, int, int, float x) { v[c][k] = x; }
 };
 template <int RY>
 struct RhsLdsT {
     float (*lds)[RY][64];
+    __device__ __forceinline__ void bind(float* p) { lds = reinterpret_cast<float (*)[RY][64]>(p); }
     __device__ __forceinline__ float get(int c, int k, int dyw, int lane) const {
         return lds[c][dyw + k][lane];
     }
@@ -408,11 +440,14 @@ struct SweepGeom {
     int dyw, Yw, h;         // first region row and grid row of the wave, grid height
     bool l0, l1, r0, r1;    // the two cells have a left / right neighbour in the grid
     bool in0, in1;          // the two cells lie in the grid
+    float wl0, wl1, wr0, wr1;  // the same as weights 1 / 0
     float om2, om3, om4;    // omega / cnt
+    float c1;               // 1 - omega
 };
 
-__device__ __forceinline__ SweepGeom sweep_geom(int X0, int RX, int Yw, int dyw, int w, int h,
-                                                float om2, float om3, float om4) {
+__device__ __forceinline__ SweepGeom sweep_geom(int X0, int RX, int RY, int R, int Yw, int dyw,
+                                                int w, int h, float om2, float om3, float om4,
+                                                float c1) {
     SweepGeom g;
     g.lane = threadIdx.x & 63;
     const int x0 = X0 + 2 * g.lane;
@@ -427,72 +462,99 @@ __device__ __forceinline__ SweepGeom sweep_geom(int X0, int RX, int Yw, int dyw,
     g.l1 = x0 + 1 > 0;
     g.r0 = x0 < w - 1;
     g.r1 = x0 + 1 < w - 1;
+    g.wl0 = g.l0 ? 1.f : 0.f;
+    g.wl1 = g.l1 ? 1.f : 0.f;
+    g.wr0 = g.r0 ? 1.f : 0.f;
+    g.wr1 = g.r1 ? 1.f : 0.f;
     g.om2 = om2;
     g.om3 = om3;
     g.om4 = om4;
+    g.c1 = c1;
     return g;
 }
 
 // Rows K0 <= k < K1 of one half-sweep of colour C.  Only rows 0 and R-1 read
 // up_row / down_row (the neighbouring waves' rows).
+//
+// Residual rule (!TRACK): the right-hand side holds nq = -oc r per cell
+// (oc = omega / cnt, cnt = in-grid neighbours; 0 outside the grid), and
+//     v' = oc S + ((1 - omega) v + nq),   S = sum of the in-grid neighbours,
+// which is v + oc (S - cnt v - r), the reference operator's SOR update
+// (solver.cpp:29-47), in two FMAs after the neighbour sum.  On border tiles
+// (EDGE) S is a weighted sum with weight 0 for a neighbour outside the grid
+// and oc comes per cell from `occ`; weights 1 and oc = omega / 4 give the
+// interior path's operations, hence its bits.  Cells outside the grid have
+// oc = nq = 0 and value 0, and stay 0.
+// Max-update rule (TRACK): the right-hand side holds r; the update
+// oc (S - cnt v - r) is formed explicitly, as the stop test tracks it.
 template <int C, int H, bool EDGE, bool TRACK, int R, int RY, int K0, int K1, typename RV>
 __device__ __forceinline__ void blocked_half(float (&ev)[2][R], const RV& rv, const SweepGeom g,
-                                             float up_row, float down_row, float& mx) {
+                                             const float (&occ)[2][R], float up_row,
+                                             float down_row, float& mx) {
 #pragma unroll
     for (int k = K0; k < K1; ++k) {
         const int p = (k + C) & 1;  // column parity of colour-C cells in row k (R even)
         const float o = ev[1 - C][k];
-        const float left = p ? o : lane_prev(o);
-        const float right = p ? lane_next(o) : o;
         const float up = k > 0 ? ev[1 - C][k - 1] : up_row;
         const float down = k < R - 1 ? ev[1 - C][k + 1] : down_row;
         const int dy = min(g.dyw + k, RY - 1 - (g.dyw + k));
         const int dx = p ? g.dx1 : g.dx0;
         const float v = ev[C][k];
-        if (EDGE) {
-            // A neighbour outside the grid contributes the cell's own value:
-            // the sum is then the sum over the in-grid neighbours plus
-            // (4 - cnt) v, so fma(-4, v, sum) = sum_in - cnt v, the reference
-            // operator (solver.cpp:29-47), and omega/cnt (om4 / om3 / om2)
-            // scales the update.  Cells outside the grid keep their value (0);
-            // they are never read by cells inside it.  The grid border lies at
-            // least H cells inside the region, so, as in the interior path,
-            // every in-grid cell is updated: stale values near the region
-            // border never reach the stored interior.  A cell with four
-            // neighbours gets the same operations, hence the same bits, as in
-            // the interior path.  (Branch-free: uniform per-row branches here
-            // keep the compiler from interleaving the rows' dependent chains
-            // and measured 13% slower.)
-            const int Y = g.Yw + k;
-            const bool lu = Y > 0, ld = Y < g.h - 1;  // uniform over the wave
-            const bool ll = p ? g.l1 : g.l0, lr = p ? g.r1 : g.r0;
-            const bool in = (p ? g.in1 : g.in0) && Y >= 0 && Y < g.h;
-            const float sum = (ll ? left : v) + (lu ? up : v) + (lr ? right : v) + (ld ? down : v);
-            const int miss = (int)!ll + (int)!lr + (int)!lu + (int)!ld;
-            const float oc = miss == 0 ? g.om4 : (miss == 1 ? g.om3 : g.om2);
-            const float t = __builtin_fmaf(-4.f, v, sum) - rv.get(C, k, g.dyw, g.lane);
-            if (TRACK) {
+        if (!TRACK) {
+            const float nq = rv.get(C, k, g.dyw, g.lane);
+            if (EDGE) {
+                const int Y = g.Yw + k;
+                const float wu = Y > 0 ? 1.f : 0.f, wd = Y < g.h - 1 ? 1.f : 0.f;  // uniform
+                const float wl = p ? g.wl1 : g.wl0, wr = p ? g.wr1 : g.wr0;
+                float sum = (p ? o : lane_prev(o)) * wl;
+                sum = __builtin_fmaf(wu, up, sum);
+                sum = __builtin_fmaf(wr, p ? lane_next(o) : o, sum);
+                sum = __builtin_fmaf(wd, down, sum);
+                ev[C][k] = __builtin_fmaf(occ[C][k], sum, __builtin_fmaf(g.c1, v, nq));
+            } else {
+                // Interior region: every cell is updated.  A cell at distance
+                // d < s from the region border takes stale neighbours, but
+                // after half-sweep s only cells at distance >= s are exact, and
+                // those read cells at distance >= s-1 only: the stale values
+                // never reach the stored interior (distance >= H >= nhalf) and
+                // stay finite (each update is a bounded combination of finite
+                // values).
+                const float left = p ? o : lane_prev(o);
+                const float right = p ? lane_next(o) : o;
+                const float sum = left + up + right + down;
+                ev[C][k] = __builtin_fmaf(g.om4, sum, __builtin_fmaf(g.c1, v, nq));
+            }
+        } else {
+            const float left = p ? o : lane_prev(o);
+            const float right = p ? lane_next(o) : o;
+            if (EDGE) {
+                // A neighbour outside the grid contributes the cell's own
+                // value: the sum is then the sum over the in-grid neighbours
+                // plus (4 - cnt) v, so fma(-4, v, sum) = sum_in - cnt v, and
+                // omega/cnt (om4 / om3 / om2) scales the update.  Cells outside
+                // the grid keep their value (0); they are never read by cells
+                // inside it.  The grid border lies at least H cells inside the
+                // region, so every in-grid cell is updated: stale values near
+                // the region border never reach the stored interior.
+                // (Branch-free: uniform per-row branches here keep the
+                // compiler from interleaving the rows' dependent chains.)
+                const int Y = g.Yw + k;
+                const bool lu = Y > 0, ld = Y < g.h - 1;  // uniform over the wave
+                const bool ll = p ? g.l1 : g.l0, lr = p ? g.r1 : g.r0;
+                const bool in = (p ? g.in1 : g.in0) && Y >= 0 && Y < g.h;
+                const float sum = (ll ? left : v) + (lu ? up : v) + (lr ? right : v) + (ld ? down : v);
+                const int miss = (int)!ll + (int)!lr + (int)!lu + (int)!ld;
+                const float oc = miss == 0 ? g.om4 : (miss == 1 ? g.om3 : g.om2);
+                const float t = __builtin_fmaf(-4.f, v, sum) - rv.get(C, k, g.dyw, g.lane);
                 const float dl = oc * t;
                 ev[C][k] = in ? v + dl : v;
                 if (in && dx >= H && dy >= H) mx = fmaxf(mx, fabsf(dl));
             } else {
-                ev[C][k] = in ? __builtin_fmaf(oc, t, v) : v;
-            }
-        } else {
-            // Interior region: every cell is updated.  A cell at distance d < s
-            // from the region border takes stale neighbours, but after
-            // half-sweep s only cells at distance >= s are exact, and those
-            // read cells at distance >= s-1 only: the stale values never
-            // reach the stored interior (distance >= H >= nhalf) and stay
-            // finite (each update is a bounded combination of finite values).
-            const float t = __builtin_fmaf(-4.f, v, left + up + right + down) -
-                            rv.get(C, k, g.dyw, g.lane);
-            if (TRACK) {
+                const float t = __builtin_fmaf(-4.f, v, left + up + right + down) -
+                                rv.get(C, k, g.dyw, g.lane);
                 const float dl = g.om4 * t;
                 ev[C][k] = v + dl;
                 if (dx >= H && dy >= H) mx = fmaxf(mx, fabsf(dl));
-            } else {
-                ev[C][k] = __builtin_fmaf(g.om4, t, v);
             }
         }
     }
@@ -501,7 +563,7 @@ __device__ __forceinline__ void blocked_half(float (&ev)[2][R], const RV& rv, co
 template <int C, int H, bool EDGE, bool TRACK, int R, int NW, typename RV>
 __device__ __forceinline__ void blocked_step(float (&ev)[2][R], const RV& rv,
                                              float (*xch)[NW][2][64], const SweepGeom g,
-                                             int wave, float& mx) {
+                                             const float (&occ)[2][R], int wave, float& mx) {
     const int lane = g.lane;
     // publish this wave's first/last row of the colour the update reads
     xch[1 - C][wave][0][lane] = ev[1 - C][0];
@@ -509,28 +571,62 @@ __device__ __forceinline__ void blocked_step(float (&ev)[2][R], const RV& rv,
     // the inner rows need only this wave's registers: update them while the
     // other waves reach the barrier (the order of the updates of one colour
     // does not matter: they read only the other colour)
-    blocked_half<C, H, EDGE, TRACK, R, R * NW, 1, R - 1>(ev, rv, g, 0.f, 0.f, mx);
+    blocked_half<C, H, EDGE, TRACK, R, R * NW, 1, R - 1>(ev, rv, g, occ, 0.f, 0.f, mx);
     __syncthreads();
     const float up_row = xch[1 - C][wave > 0 ? wave - 1 : 0][1][lane];
     const float down_row = xch[1 - C][wave < NW - 1 ? wave + 1 : NW - 1][0][lane];
-    blocked_half<C, H, EDGE, TRACK, R, R * NW, 0, 1>(ev, rv, g, up_row, down_row, mx);
-    blocked_half<C, H, EDGE, TRACK, R, R * NW, R - 1, R>(ev, rv, g, up_row, down_row, mx);
+    blocked_half<C, H, EDGE, TRACK, R, R * NW, 0, 1>(ev, rv, g, occ, up_row, down_row, mx);
+    blocked_half<C, H, EDGE, TRACK, R, R * NW, R - 1, R>(ev, rv, g, occ, up_row, down_row, mx);
 }
 
 // nhalf <= H half-sweeps, colours alternating from 0; mxs[j] receives the
 // largest |update| of sweep j over the cells the launch stores (TRACK)
 template <int H, bool EDGE, bool TRACK, int R, int NW, typename RV>
 __device__ __forceinline__ void blocked_run(float (&ev)[2][R], const RV& rv,
-                                            float (*xch)[NW][2][64], const SweepGeom g, int wave,
-                                            int nhalf, float (&mxs)[H / 2]) {
+                                            float (*xch)[NW][2][64], const SweepGeom g,
+                                            const float (&occ)[2][R], int wave, int nhalf,
+                                            float (&mxs)[H / 2]) {
 #pragma unroll
     for (int j = 0; j < H / 2; ++j) {
         const int s = 2 * j + 1;
         if (s > nhalf) break;
         float mx = 0.f;
-        blocked_step<0, H, EDGE, TRACK, R, NW>(ev, rv, xch, g, wave, mx);
-        if (s + 1 <= nhalf) blocked_step<1, H, EDGE, TRACK, R, NW>(ev, rv, xch, g, wave, mx);
+        blocked_step<0, H, EDGE, TRACK, R, NW>(ev, rv, xch, g, occ, wave, mx);
+        if (s + 1 <= nhalf) blocked_step<1, H, EDGE, TRACK, R, NW>(ev, rv, xch, g, occ, wave, mx);
         mxs[j] = mx;
+    }
+}
+
+// Right-hand side of the sweeps, per cell: r - mean (TRACK) or
+// nq = (mean - r) oc (residual rule, see blocked_half).  On interior tiles
+// every cell lies in the grid with four neighbours (oc = omega / 4); on border
+// tiles cells outside the grid get value 0 and nq = oc = 0.
+template <bool EDGE, bool TRACK, typename RT, typename RV>
+__device__ __forceinline__ void region_setup(float (&ev)[2][BK_R], const RT (&rr)[2][BK_R], RV& rv,
+                                             float (&occ)[2][BK_R], const SweepGeom g, float rm) {
+#pragma unroll
+    for (int k = 0; k < BK_R; ++k) {
+        const int Y = g.Yw + k;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int p = (k + c) & 1;
+            const float rc = rhs_to_f32(rr[c][k]);
+            if (!EDGE) {
+                rv.set(c, k, g.dyw, g.lane, TRACK ? rc - rm : (rm - rc) * g.om4);
+                continue;
+            }
+            const bool in = (p ? g.in1 : g.in0) && Y >= 0 && Y < g.h;
+            ev[c][k] = in ? ev[c][k] : 0.f;
+            if (TRACK) {
+                rv.set(c, k, g.dyw, g.lane, in ? rc - rm : 0.f);
+            } else {
+                const int miss = (int)!(p ? g.l1 : g.l0) + (int)!(p ? g.r1 : g.r0) +
+                                 (int)!(Y > 0) + (int)!(Y < g.h - 1);
+                const float oc = in ? (miss == 0 ? g.om4 : (miss == 1 ? g.om3 : g.om2)) : 0.f;
+                occ[c][k] = oc;
+                rv.set(c, k, g.dyw, g.lane, (rm - rc) * oc);
+            }
+        }
     }
 }
 
@@ -538,7 +634,9 @@ __device__ __forceinline__ void blocked_run(float (&ev)[2][R], const RV& rv,
 // re-rounds the current residual, so the rounding never reaches the result)
 // and FP32 under the max-update rule (TRACK), which cannot see the smooth
 // error a rounded right-hand side leaves behind.
-template <int H, bool TRACK>
+// FULL: nhalf == H, known at compile time (every launch but a batch's last),
+// which leaves the half-sweeps free of branches.
+template <int H, bool TRACK, bool FULL>
 __global__ __launch_bounds__(BK_THREADS) void k_rb_blocked(const float* __restrict__ ein,
                                                     float* __restrict__ eout,
                                                     const void* __restrict__ rhs,
@@ -575,7 +673,9 @@ __global__ __launch_bounds__(BK_THREADS) void k_rb_blocked(const float* __restri
     // a uniform row offset plus this one 32-bit lane offset
     const unsigned col = brick_col(min(max(X0 / 2 + lane, 0), wp - 1));
     using RT = typename std::conditional<TRACK, float, unsigned short>::type;
-    const RT* __restrict__ r = static_cast<const RT*>(rhs);
+    const size_t npk = 2 * bk.plane;
+    const __amdgpu_buffer_rsrc_t ein_rs = plane_rsrc(ein, npk * sizeof(float));
+    const __amdgpu_buffer_rsrc_t r_rs = plane_rsrc(rhs, npk * sizeof(RT));
     float ev[2][BK_R];
     RT rr[2][BK_R];
 #pragma unroll
@@ -583,9 +683,9 @@ __global__ __launch_bounds__(BK_THREADS) void k_rb_blocked(const float* __restri
         const int Yc = min(max(Yw + k, 0), h - 1);
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
-            const size_t row = brick_row(bk, c, Yc);
-            ev[c][k] = ein[row + col];
-            rr[c][k] = r[row + col];
+            const unsigned row = (unsigned)brick_row(bk, c, Yc);
+            ev[c][k] = buf_load<float>(ein_rs, col * 4u, row * 4u);
+            rr[c][k] = buf_load<RT>(r_rs, col * (unsigned)sizeof(RT), row * (unsigned)sizeof(RT));
         }
     }
     // Max-update rule: the stop flag can be set between the launches of one
@@ -598,27 +698,24 @@ __global__ __launch_bounds__(BK_THREADS) void k_rb_blocked(const float* __restri
     const float rm = *rmean;
     __shared__ float rlds[BK_RV_LDS ? 2 : 1][BK_RV_LDS ? BK_RY : 1][64];
     typename std::conditional<BK_RV_LDS, RhsLds, RhsRegs>::type rv;
-    if constexpr (BK_RV_LDS) rv.lds = reinterpret_cast<float (*)[BK_RY][64]>(&rlds[0][0][0]);
-#pragma unroll
-    for (int k = 0; k < BK_R; ++k) {
-        const bool iny = (unsigned)(Yw + k) < (unsigned)h;
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            const bool in = iny && (((k + c) & 1) ? in1 : in0);
-            ev[c][k] = in ? ev[c][k] : 0.f;
-            rv.set(c, k, dyw, lane, in ? rhs_to_f32(rr[c][k]) - rm : 0.f);
-        }
-    }
+    rv.bind(&rlds[0][0][0]);
+    const bool edge = X0 < 1 || X0 + BK_RX > w - 1 || Y0 < 1 || Y0 + BK_RY > h - 1;
+    const SweepGeom g = sweep_geom(X0, BK_RX, BK_RY, BK_R, Yw, dyw, w, h, om2, om3, om4,
+                                   1.f - 4.f * om4);
+    float occ[2][BK_R];  // border tiles: omega / cnt per cell, 0 outside the grid
     float mxs[H / 2];
 #pragma unroll
     for (int j = 0; j < H / 2; ++j) mxs[j] = 0.f;
-    const bool edge = X0 < 1 || X0 + BK_RX > w - 1 || Y0 < 1 || Y0 + BK_RY > h - 1;
-    const SweepGeom g = sweep_geom(X0, BK_RX, Yw, dyw, w, h, om2, om3, om4);
-    if (edge)
-        blocked_run<H, true, TRACK, BK_R, BK_NW>(ev, rv, xch, g, wave, nhalf, mxs);
-    else
-        blocked_run<H, false, TRACK, BK_R, BK_NW>(ev, rv, xch, g, wave, nhalf, mxs);
+    if (edge) {
+        region_setup<true, TRACK>(ev, rr, rv, occ, g, rm);
+        blocked_run<H, true, TRACK, BK_R, BK_NW>(ev, rv, xch, g, occ, wave, FULL ? H : nhalf, mxs);
+    } else {
+        region_setup<false, TRACK>(ev, rr, rv, occ, g, rm);
+        blocked_run<H, false, TRACK, BK_R, BK_NW>(ev, rv, xch, g, occ, wave, FULL ? H : nhalf,
+                                                  mxs);
+    }
     // write the interior
+    const __amdgpu_buffer_rsrc_t eout_rs = plane_rsrc(eout, npk * sizeof(float));
 #pragma unroll
     for (int k = 0; k < BK_R; ++k) {
         const int Y = Yw + k;
@@ -628,7 +725,8 @@ __global__ __launch_bounds__(BK_THREADS) void k_rb_blocked(const float* __restri
             const int p = (k + c) & 1;
             const int dx = p ? dx1 : dx0;
             if (dx >= H && dy >= H && (unsigned)Y < (unsigned)h && (p ? in1 : in0))
-                eout[brick_row(bk, c, Y) + col] = ev[c][k];
+                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(ev[c][k]), eout_rs, col * 4u,
+                                                      (unsigned)brick_row(bk, c, Y) * 4u, 0);
         }
     }
     if (TRACK) {
@@ -1065,8 +1163,9 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     const int nred = rgrid.x * rgrid.y;
     // FP32 sweeps without holes take the temporally blocked kernel on the
     // packed layout; FP64 sweeps and hole masks take the per-half-sweep kernel.
-    const bool blocked = mixed && mask == nullptr;
+    // (the blocked kernel addresses a plane pair through 32-bit byte offsets)
     const Brick bk = make_brick(w, h);
+    const bool blocked = mixed && mask == nullptr && 2 * bk.plane * sizeof(float) < (1ull << 31);
     const size_t npk = 2 * bk.plane;
     SolveWork ws;
     if (work) {
@@ -1169,19 +1268,22 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         int launches = 0;
         for (int done_sw = 0; done_sw < count_sweeps;) {
             const int nsw = std::min(BH / 2, count_sweeps - done_sw);
+            const bool full = 2 * nsw == BH;
             const int in = launches & 1;
             float* ein = in ? e32b : e32;
             float* eout = in ? e32 : e32b;
             if (track) {
-                hipLaunchKernelGGL((k_rb_blocked<BH, true>), dim3(bgrid), dim3(BK_THREADS), 0, s, ein,
-                                   eout, r32, bk, w, h, tiles_x, ntiles, 2 * nsw, om2, om3, om4,
-                                   &sd->rmean32, sd, 1 - in);
+                hipLaunchKernelGGL(full ? (k_rb_blocked<BH, true, true>) : (k_rb_blocked<BH, true, false>),
+                                   dim3(bgrid), dim3(BK_THREADS), 0, s, ein, eout, r32, bk, w, h,
+                                   tiles_x, ntiles, 2 * nsw, om2, om3, om4, &sd->rmean32, sd,
+                                   1 - in);
                 hipLaunchKernelGGL(k_maxcheck_blocked, dim3(1), dim3(64), 0, s, sd,
                                    first + done_sw, nsw, o->tol);
             } else {
-                hipLaunchKernelGGL((k_rb_blocked<BH, false>), dim3(bgrid), dim3(BK_THREADS), 0, s, ein,
-                                   eout, r16, bk, w, h, tiles_x, ntiles, 2 * nsw, om2, om3, om4,
-                                   &sd->rmean32, sd, 1 - in);
+                hipLaunchKernelGGL(full ? (k_rb_blocked<BH, false, true>) : (k_rb_blocked<BH, false, false>),
+                                   dim3(bgrid), dim3(BK_THREADS), 0, s, ein, eout, r16, bk, w, h,
+                                   tiles_x, ntiles, 2 * nsw, om2, om3, om4, &sd->rmean32, sd,
+                                   1 - in);
             }
             ++launches;
             done_sw += nsw;
@@ -1383,7 +1485,8 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
             int cur_m = 0;
             for (int done_sw = 0; done_sw < mid_sweeps;) {
                 const int nsw = std::min(BH / 2, mid_sweeps - done_sw);
-                hipLaunchKernelGGL((k_rb_blocked<BH, false>), dim3(gm), dim3(BK_THREADS), 0, s, em[cur_m],
+                hipLaunchKernelGGL(2 * nsw == BH ? (k_rb_blocked<BH, false, true>) : (k_rb_blocked<BH, false, false>),
+                                   dim3(gm), dim3(BK_THREADS), 0, s, em[cur_m],
                                    em[1 - cur_m], rm, bkm, wm, hm, tx, ntm, 2 * nsw, oms / 2.f,
                                    oms / 3.f, oms / 4.f, ctx->fcg_zero.as<float>(), sd, 0);
                 cur_m = 1 - cur_m;
