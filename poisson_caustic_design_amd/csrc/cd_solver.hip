@@ -362,8 +362,8 @@ __global__ __launch_bounds__(256) void k_rb_sweep(T* __restrict__ e, const T* __
 // half-sweeps the interior (distance >= H) equals nhalf global half-sweeps
 // and is written to the other ping-pong buffer.
 // ---------------------------------------------------------------------------
-// Geometry is fixed at compile time; CD_BK_W / CD_BK_R / CD_BK_H / CD_BK_RV_LDS
-// exist to compare variants on the hardware.  The kernel is bound by vector
+// Geometry is fixed at compile time; CD_BK_W / CD_BK_R / CD_BK_H exist to
+// compare variants on the hardware.  The kernel is bound by vector
 // ALU issue and by the load phase that precedes the sweeps (a round of
 // resident workgroups loads, then sweeps): the interior update takes five
 // VALU operations per cell (three adds, the first carrying the DPP shift, and
@@ -380,16 +380,17 @@ __global__ __launch_bounds__(256) void k_rb_sweep(T* __restrict__ e, const T* __
 // workgroups walking the tiles, the next tile's loads issued before the
 // current tile's sweeps) measured 21.1-21.5 us against 17.6-18.0 us: the
 // loop-carried registers cost copies in every half-sweep and the static tile
-// split loses the dispatcher's load balance.  H = 10 also keeps the learned
-// iteration's 8 smoothing sweeps at two launches.
+// split loses the dispatcher's load balance; two tiles per workgroup (the
+// second's loads issued before the first's sweeps) 17.2 us against 14.8 us;
+// delaying the start of a quarter of the first round's workgroups by 0.9-2.6
+// us (to stagger the load and sweep phases) 15.2-16.6 us.
+// H = 10 also keeps the learned iteration's 8 smoothing sweeps at two
+// launches.
 #ifndef CD_BK_R
 #define CD_BK_R 4
 #endif
 #ifndef CD_BK_H
 #define CD_BK_H 10
-#endif
-#ifndef CD_BK_RV_LDS
-#define CD_BK_RV_LDS 0
 #endif
 #ifndef CD_BK_W
 #define CD_BK_W 16
@@ -399,31 +400,16 @@ constexpr int BK_R = CD_BK_R;   // rows per wave
 constexpr int BK_NW = CD_BK_W;  // waves per workgroup
 constexpr int BK_THREADS = 64 * BK_NW;
 constexpr int BK_RY = BK_NW * BK_R;
-constexpr bool BK_RV_LDS = CD_BK_RV_LDS != 0;  // right-hand side in LDS (else registers)
 
-// Right-hand side of the region: registers of the owning thread, or LDS
-// (frees VGPRs for occupancy).  Element (colour c, region row y, lane);
-// R rows per wave, RY rows per region.
-template <int R, int RY>
+// Right-hand side of the region in the registers of the owning thread:
+// element (colour c, row k of the wave).  (In LDS it measured 4% slower.)
+template <int R>
 struct RhsRegsT {
     float v[2][R];
-    __device__ __forceinline__ void bind(float*) {}
     __device__ __forceinline__ float get(int c, int k, int, int) const { return v[c][k]; }
     __device__ __forceinline__ void set(int c, int k, int, int, float x) { v[c][k] = x; }
 };
-template <int RY>
-struct RhsLdsT {
-    float (*lds)[RY][64];
-    __device__ __forceinline__ void bind(float* p) { lds = reinterpret_cast<float (*)[RY][64]>(p); }
-    __device__ __forceinline__ float get(int c, int k, int dyw, int lane) const {
-        return lds[c][dyw + k][lane];
-    }
-    __device__ __forceinline__ void set(int c, int k, int dyw, int lane, float x) {
-        lds[c][dyw + k][lane] = x;
-    }
-};
-using RhsRegs = RhsRegsT<BK_R, BK_RY>;
-using RhsLds = RhsLdsT<BK_RY>;
+using RhsRegs = RhsRegsT<BK_R>;
 
 // value of lane l-1 (0 in lane 0) / lane l+1 (0 in lane 63): DPP wave shifts
 __device__ __forceinline__ float lane_prev(float v) {
@@ -593,7 +579,7 @@ __device__ __forceinline__ void blocked_run(float (&ev)[2][R], const RV& rv,
         float mx = 0.f;
         blocked_step<0, H, EDGE, TRACK, R, NW>(ev, rv, xch, g, occ, wave, mx);
         if (s + 1 <= nhalf) blocked_step<1, H, EDGE, TRACK, R, NW>(ev, rv, xch, g, occ, wave, mx);
-        mxs[j] = mx;
+        mxs[j] = fmaxf(mxs[j], mx);
     }
 }
 
@@ -634,6 +620,78 @@ __device__ __forceinline__ void region_setup(float (&ev)[2][BK_R], const RT (&rr
 // re-rounds the current residual, so the rounding never reaches the result)
 // and FP32 under the max-update rule (TRACK), which cannot see the smooth
 // error a rounded right-hand side leaves behind.
+// Loads of one region: packed column `lane` of the wave's BK_R rows, both
+// colours, from the correction and the right-hand side.  Rows and columns are
+// clamped into the grid (cells outside it are masked by region_setup).
+template <int H, typename RT>
+__device__ __forceinline__ void region_load(float (&ev)[2][BK_R], RT (&rr)[2][BK_R],
+                                            __amdgpu_buffer_rsrc_t ein_rs,
+                                            __amdgpu_buffer_rsrc_t r_rs, const Brick& bk,
+                                            int tile, int tiles_x, int w, int h, int wave,
+                                            int lane) {
+    constexpr int IX = BK_RX - 2 * H, IY = BK_RY - 2 * H;
+    const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
+    const int X0 = tx * IX - H, Yw = ty * IY - H + BK_R * wave;
+    const unsigned col = brick_col(min(max(X0 / 2 + lane, 0), (w + 1) / 2 - 1));
+#pragma unroll
+    for (int k = 0; k < BK_R; ++k) {
+        const int Yc = min(max(Yw + k, 0), h - 1);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const unsigned row = (unsigned)brick_row(bk, c, Yc);
+            ev[c][k] = buf_load<float>(ein_rs, col * 4u, row * 4u);
+            rr[c][k] = buf_load<RT>(r_rs, col * (unsigned)sizeof(RT), row * (unsigned)sizeof(RT));
+        }
+    }
+}
+
+// Setup, nhalf half-sweeps and the store of one loaded region; mxs[j] takes
+// the max with the largest |update| of sweep j (TRACK).
+template <int H, bool TRACK, typename RT>
+__device__ __forceinline__ void blocked_tile(float (&ev)[2][BK_R], const RT (&rr)[2][BK_R],
+                                             float (*xch)[BK_NW][2][64], float (&mxs)[H / 2],
+                                             __amdgpu_buffer_rsrc_t eout_rs, const Brick& bk,
+                                             int tile, int tiles_x, int w, int h, int nhalf,
+                                             float om2, float om3, float om4, float rm, int wave,
+                                             int lane) {
+    constexpr int IX = BK_RX - 2 * H, IY = BK_RY - 2 * H;
+    const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
+    const int X0 = tx * IX - H, Y0 = ty * IY - H;
+    const int wp = (w + 1) / 2;
+    const int Yw = Y0 + BK_R * wave;  // first global row of this wave
+    const int dyw = BK_R * wave;      // first region row of this wave
+    const unsigned col = brick_col(min(max(X0 / 2 + lane, 0), wp - 1));
+    RhsRegs rv;
+    const SweepGeom g = sweep_geom(X0, BK_RX, BK_RY, BK_R, Yw, dyw, w, h, om2, om3, om4,
+                                   1.f - 4.f * om4);
+    const bool edge = X0 < 1 || X0 + BK_RX > w - 1 || Y0 < 1 || Y0 + BK_RY > h - 1;
+    float occ[2][BK_R];  // border tiles: omega / cnt per cell, 0 outside the grid
+    if (edge) {
+        region_setup<true, TRACK>(ev, rr, rv, occ, g, rm);
+        blocked_run<H, true, TRACK, BK_R, BK_NW>(ev, rv, xch, g, occ, wave, nhalf, mxs);
+    } else {
+        region_setup<false, TRACK>(ev, rr, rv, occ, g, rm);
+        blocked_run<H, false, TRACK, BK_R, BK_NW>(ev, rv, xch, g, occ, wave, nhalf, mxs);
+    }
+    // write the interior
+#pragma unroll
+    for (int k = 0; k < BK_R; ++k) {
+        const int Y = Yw + k;
+        const int dy = min(dyw + k, BK_RY - 1 - (dyw + k));
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int p = (k + c) & 1;
+            const int dx = p ? g.dx1 : g.dx0;
+            if (dx >= H && dy >= H && (unsigned)Y < (unsigned)h && (p ? g.in1 : g.in0))
+                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(ev[c][k]), eout_rs, col * 4u,
+                                                      (unsigned)brick_row(bk, c, Y) * 4u, 0);
+        }
+    }
+}
+
+// workgroups of a k_rb_blocked launch over ntiles tiles (a multiple of 8)
+inline int blocked_grid(int ntiles) { return (ntiles + 7) / 8 * 8; }
+
 // FULL: nhalf == H, known at compile time (every launch but a batch's last),
 // which leaves the half-sweeps free of branches.
 template <int H, bool TRACK, bool FULL>
@@ -654,40 +712,16 @@ __global__ __launch_bounds__(BK_THREADS) void k_rb_blocked(const float* __restri
     const int per = gridDim.x >> 3;
     const int tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (tile >= ntiles) return;
-    constexpr int IX = BK_RX - 2 * H, IY = BK_RY - 2 * H;
-    const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
-    const int X0 = tx * IX - H, Y0 = ty * IY - H;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wp = (w + 1) / 2;
-    const int Yw = Y0 + BK_R * wave;  // first global row of this wave
-    const int dyw = BK_R * wave;      // first region row of this wave
-    // per-lane column facts for the two cells of packed column `lane`
-    const int X0l = X0 + 2 * lane;
-    const int dx0 = min(2 * lane, BK_RX - 1 - 2 * lane);
-    const int dx1 = min(2 * lane + 1, BK_RX - 2 - 2 * lane);
-    const bool in0 = (unsigned)X0l < (unsigned)w, in1 = (unsigned)(X0l + 1) < (unsigned)w;
-    const bool l0 = X0l > 0, l1 = X0l + 1 > 0;
-    const bool r0 = X0l < w - 1, r1 = X0l + 1 < w - 1;
-    // packed column of both cells, clamped into the row: every load below is
-    // a uniform row offset plus this one 32-bit lane offset
-    const unsigned col = brick_col(min(max(X0 / 2 + lane, 0), wp - 1));
     using RT = typename std::conditional<TRACK, float, unsigned short>::type;
     const size_t npk = 2 * bk.plane;
     const __amdgpu_buffer_rsrc_t ein_rs = plane_rsrc(ein, npk * sizeof(float));
     const __amdgpu_buffer_rsrc_t r_rs = plane_rsrc(rhs, npk * sizeof(RT));
+    const __amdgpu_buffer_rsrc_t eout_rs = plane_rsrc(eout, npk * sizeof(float));
     float ev[2][BK_R];
     RT rr[2][BK_R];
-#pragma unroll
-    for (int k = 0; k < BK_R; ++k) {
-        const int Yc = min(max(Yw + k, 0), h - 1);
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            const unsigned row = (unsigned)brick_row(bk, c, Yc);
-            ev[c][k] = buf_load<float>(ein_rs, col * 4u, row * 4u);
-            rr[c][k] = buf_load<RT>(r_rs, col * (unsigned)sizeof(RT), row * (unsigned)sizeof(RT));
-        }
-    }
+    region_load<H>(ev, rr, ein_rs, r_rs, bk, tile, tiles_x, w, h, wave, lane);
     // Max-update rule: the stop flag can be set between the launches of one
     // batch (k_maxcheck_blocked), and a finished solve leaves the buffers
     // untouched.  Residual rule: only a check sets it, and no sweeps are
@@ -696,39 +730,11 @@ __global__ __launch_bounds__(BK_THREADS) void k_rb_blocked(const float* __restri
     // loads).
     if (TRACK && st->done) return;
     const float rm = *rmean;
-    __shared__ float rlds[BK_RV_LDS ? 2 : 1][BK_RV_LDS ? BK_RY : 1][64];
-    typename std::conditional<BK_RV_LDS, RhsLds, RhsRegs>::type rv;
-    rv.bind(&rlds[0][0][0]);
-    const bool edge = X0 < 1 || X0 + BK_RX > w - 1 || Y0 < 1 || Y0 + BK_RY > h - 1;
-    const SweepGeom g = sweep_geom(X0, BK_RX, BK_RY, BK_R, Yw, dyw, w, h, om2, om3, om4,
-                                   1.f - 4.f * om4);
-    float occ[2][BK_R];  // border tiles: omega / cnt per cell, 0 outside the grid
     float mxs[H / 2];
 #pragma unroll
     for (int j = 0; j < H / 2; ++j) mxs[j] = 0.f;
-    if (edge) {
-        region_setup<true, TRACK>(ev, rr, rv, occ, g, rm);
-        blocked_run<H, true, TRACK, BK_R, BK_NW>(ev, rv, xch, g, occ, wave, FULL ? H : nhalf, mxs);
-    } else {
-        region_setup<false, TRACK>(ev, rr, rv, occ, g, rm);
-        blocked_run<H, false, TRACK, BK_R, BK_NW>(ev, rv, xch, g, occ, wave, FULL ? H : nhalf,
-                                                  mxs);
-    }
-    // write the interior
-    const __amdgpu_buffer_rsrc_t eout_rs = plane_rsrc(eout, npk * sizeof(float));
-#pragma unroll
-    for (int k = 0; k < BK_R; ++k) {
-        const int Y = Yw + k;
-        const int dy = min(dyw + k, BK_RY - 1 - (dyw + k));
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            const int p = (k + c) & 1;
-            const int dx = p ? dx1 : dx0;
-            if (dx >= H && dy >= H && (unsigned)Y < (unsigned)h && (p ? in1 : in0))
-                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(ev[c][k]), eout_rs, col * 4u,
-                                                      (unsigned)brick_row(bk, c, Y) * 4u, 0);
-        }
-    }
+    blocked_tile<H, TRACK>(ev, rr, xch, mxs, eout_rs, bk, tile, tiles_x, w, h, FULL ? H : nhalf,
+                           om2, om3, om4, rm, wave, lane);
     if (TRACK) {
 #pragma unroll
         for (int j = 0; j < H / 2; ++j) {
@@ -1259,7 +1265,7 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     const int bix = BK_RX - 2 * BH, biy = BK_RY - 2 * BH;
     const int tiles_x = div_up(w, bix), tiles_y = div_up(h, biy);
     const int ntiles = tiles_x * tiles_y;
-    const int bgrid = div_up(ntiles, 8) * 8;
+    const int bgrid = blocked_grid(ntiles);
     const float om = (float)omega;
     const float om2 = om / 2.f, om3 = om / 3.f, om4 = om / 4.f;
     auto enqueue_blocked_om = [&](int first, int count_sweeps, float om2, float om3,
@@ -1481,7 +1487,7 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
                                ctx->lc_y.as<float>(), stats + 2, wm, hm, lf / fm, lwc, lhc, nscale,
                                bkm, em[0]);
             const int tx = div_up(wm, bix), ntm = tx * div_up(hm, biy);
-            const int gm = div_up(ntm, 8) * 8;
+            const int gm = blocked_grid(ntm);
             int cur_m = 0;
             for (int done_sw = 0; done_sw < mid_sweeps;) {
                 const int nsw = std::min(BH / 2, mid_sweeps - done_sw);
