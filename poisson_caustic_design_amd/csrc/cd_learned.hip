@@ -212,9 +212,25 @@ __global__ __launch_bounds__(256) void k_conv3x3_tile(const u16* __restrict__ x1
                 v = *reinterpret_cast<const uint4*>(x2 + q * c2 + (ci - c1));
             } else if (mode1 == 0) {
                 v = *reinterpret_cast<const uint4*>(x1 + q * c1 + ci);
-            } else {
+            } else if (mode1 == 1) {
                 const long long q1 = ((long long)b * (H >> 1) + (yy >> 1)) * (W >> 1) + (xx >> 1);
                 v = *reinterpret_cast<const uint4*>(x1 + q1 * c1 + ci);
+            } else {
+                // x1 at twice the resolution, 2x2 average pooled as k_avgpool2 does
+                const long long r0 = ((long long)b * (2 * H) + 2 * yy) * (2 * W) + 2 * xx;
+                const long long r1 = r0 + 2 * W;
+                V8 p00, p01, p10, p11, o;
+                p00.u = *reinterpret_cast<const uint4*>(x1 + r0 * c1 + ci);
+                p01.u = *reinterpret_cast<const uint4*>(x1 + (r0 + 1) * c1 + ci);
+                p10.u = *reinterpret_cast<const uint4*>(x1 + r1 * c1 + ci);
+                p11.u = *reinterpret_cast<const uint4*>(x1 + (r1 + 1) * c1 + ci);
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const float sm = from_bf16(p00.h[e]) + from_bf16(p01.h[e]) +
+                                     from_bf16(p10.h[e]) + from_bf16(p11.h[e]);
+                    o.h[e] = to_bf16(0.25f * sm);
+                }
+                v = o.u;
             }
         }
         *reinterpret_cast<uint4*>(xs + px * xstride + ci) = v;
@@ -670,7 +686,8 @@ struct cd_net {
     std::vector<NetTensor> t;
     int batch = 0, bh = 0, bw = 0;
     bool weights_dirty = true;
-    bool fused_forward = true;  // inference forwards fold the decoder upsampling into the convs
+    bool fused_forward = true;  // inference forwards fold the decoder upsampling and the
+                                // encoder pooling into the convs
     bool tiled_forward = true;  // forward convolutions on LDS-staged 8x8 tiles (bit-identical)
 };
 
@@ -700,7 +717,10 @@ size_t tiled_lds_limit(cd_context* c) {
 }
 
 // mode1: 0 reads in1 as stored; 1 reads `x1src` (the half-resolution tensor
-// in1 is upsampled from) through nearest 2x upsampling instead
+// in1 is upsampled from) through nearest 2x upsampling instead; 2 reads
+// `x1src` (the double-resolution tensor in1 is pooled from) through 2x2
+// average pooling (the tiled kernel; otherwise the pooled tensor is written
+// first)
 int launch_conv(cd_net* n, const NetConv& cv, int B, int H, int W, bool grad_input,
                 int mode1 = 0, const u16* x1src = nullptr) {
     cd_context* c = n->ctx;
@@ -712,7 +732,15 @@ int launch_conv(cd_net* n, const NetConv& cv, int B, int H, int W, bool grad_inp
     const int nbw_t = (cv.coutpad / 16) % 2 == 0 ? 2 : 1;
     const size_t lds_t = sizeof(u16) * (100 * (size_t)(cv.c1 + cv.c2 + 8) +
                                         (size_t)nbw_t * 16 * (cv.kpad + 8));
-    if (!grad_input && n->tiled_forward && lds_t <= tiled_lds_limit(c)) {
+    const bool tiled = !grad_input && n->tiled_forward && lds_t <= tiled_lds_limit(c);
+    if (mode1 == 2 && !tiled) {
+        const long long cnt = npix * cv.c1;
+        hipLaunchKernelGGL(k_avgpool2, dim3(div_up(cnt, 256)), dim3(256), 0, s, x1src,
+                           n->t[cv.in1].act.as<u16>(), B, 2 * H, 2 * W, cv.c1);
+        mode1 = 0;
+        x1src = nullptr;
+    }
+    if (tiled) {
         const u16* x1 = mode1 ? x1src
                               : (cv.in1 < 0 ? n->x8.as<u16>() : n->t[cv.in1].act.as<u16>());
         const u16* x2 = cv.in2 < 0 ? nullptr : n->t[cv.in2].act.as<u16>();
@@ -932,21 +960,29 @@ int forward(cd_net* n, int B, int H, int W, bool fused = false) {
     cd_context* c = n->ctx;
     CD_TRY(refresh_weights(n));
     const int L = n->arch.levels;
+    const u16* pooled_from = nullptr;  // fused: the next conv pools this tensor as it reads
     for (const NetConv& cv : n->convs) {
         const int Hl = H >> cv.level, Wl = W >> cv.level;
         const int idx = (int)(&cv - &n->convs[0]);
         if (fused && cv.in2 >= 0) {
             CD_TRY(launch_conv(n, cv, B, Hl, Wl, false, 1, n->t[n->convs[idx - 1].out].act.as<u16>()));
+        } else if (pooled_from) {
+            CD_TRY(launch_conv(n, cv, B, Hl, Wl, false, 2, pooled_from));
         } else {
             CD_TRY(launch_conv(n, cv, B, Hl, Wl, false));
         }
+        pooled_from = nullptr;
         // after the second conv of an encoder level below the bottleneck: pool
         if (idx < 2 * L && idx % 2 == 1 && cv.level < L - 1) {
             const NetConv& nx = n->convs[idx + 1];  // next level's first conv reads the pooled tensor
-            const long long cnt = (long long)B * (Hl / 2) * (Wl / 2) * n->t[nx.in1].C;
-            hipLaunchKernelGGL(k_avgpool2, dim3(div_up(cnt, 256)), dim3(256), 0, c->stream,
-                               n->t[cv.out].act.as<u16>(), n->t[nx.in1].act.as<u16>(), B, Hl, Wl,
-                               n->t[nx.in1].C);
+            if (fused) {
+                pooled_from = n->t[cv.out].act.as<u16>();
+            } else {
+                const long long cnt = (long long)B * (Hl / 2) * (Wl / 2) * n->t[nx.in1].C;
+                hipLaunchKernelGGL(k_avgpool2, dim3(div_up(cnt, 256)), dim3(256), 0, c->stream,
+                                   n->t[cv.out].act.as<u16>(), n->t[nx.in1].act.as<u16>(), B, Hl,
+                                   Wl, n->t[nx.in1].C);
+            }
         }
         // before a decoder concat conv: upsample the previous decoder output
         if (!fused && idx + 1 < (int)n->convs.size()) {

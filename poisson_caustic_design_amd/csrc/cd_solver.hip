@@ -258,9 +258,12 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ phi
     }
 }
 
-// single block: finish a convergence check
+// single block: finish a convergence check; the status also goes to the
+// host's pinned mirror (read after the stream has been synchronised), which
+// saves a copy per check
 __global__ void k_check(const double* __restrict__ partials, int nblocks, SolveDev* st,
-                        double count, int check_id, int stop_rule, double tol) {
+                        SolveDev* host_st, double count, int check_id, int stop_rule,
+                        double tol) {
     __shared__ double smem[16];
     double a = 0.0, q = 0.0;
     for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
@@ -286,6 +289,9 @@ __global__ void k_check(const double* __restrict__ partials, int nblocks, SolveD
             st->done = 1;
             st->done_check = check_id;
         }
+        const unsigned* src = reinterpret_cast<const unsigned*>(st);
+        unsigned* dst = reinterpret_cast<unsigned*>(host_st);
+        for (int k = 0; k < (int)(sizeof(SolveDev) / 4); ++k) dst[k] = src[k];
     }
 }
 
@@ -394,6 +400,10 @@ __global__ __launch_bounds__(256) void k_rb_sweep(T* __restrict__ e, const T* __
 #endif
 #ifndef CD_BK_W
 #define CD_BK_W 16
+#endif
+// cache policy bits of the blocked kernel's stores
+#ifndef CD_BK_STORE_POLICY
+#define CD_BK_STORE_POLICY 16  // sc1: written through, not left dirty in L2
 #endif
 constexpr int BK_RX = 128;      // region columns (64 packed columns, one per lane)
 constexpr int BK_R = CD_BK_R;   // rows per wave
@@ -684,7 +694,8 @@ __device__ __forceinline__ void blocked_tile(float (&ev)[2][BK_R], const RT (&rr
             const int dx = p ? g.dx1 : g.dx0;
             if (dx >= H && dy >= H && (unsigned)Y < (unsigned)h && (p ? g.in1 : g.in0))
                 __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(ev[c][k]), eout_rs, col * 4u,
-                                                      (unsigned)brick_row(bk, c, Y) * 4u, 0);
+                                                      (unsigned)brick_row(bk, c, Y) * 4u,
+                                                      CD_BK_STORE_POLICY);
         }
     }
 }
@@ -946,11 +957,10 @@ __device__ __forceinline__ bool packed_thread(const Brick& bk, int w, int Y, int
 // value for the whole launch), written in the
 // bricked layout of the blocked sweeps (the start of the smoothing); one
 // thread per packed element, coalesced writes
-__global__ __launch_bounds__(256) void k_prolong_bricked(const float* __restrict__ y,
-                                                         const double* __restrict__ derived,
-                                                         int w, int h, int f, int wc, int hc,
-                                                         double nscale, Brick bk,
-                                                         float* __restrict__ e) {
+__device__ __forceinline__ void prolong_bricked_at(const float* __restrict__ y,
+                                                   const double* __restrict__ derived, int w,
+                                                   int h, int f, int wc, int hc, double nscale,
+                                                   const Brick& bk, float* __restrict__ e) {
     const int yy = blockIdx.y;
     int c, j, xx;
     if (!packed_thread(bk, w, yy, c, j, xx)) return;
@@ -966,16 +976,30 @@ __global__ __launch_bounds__(256) void k_prolong_bricked(const float* __restrict
                      (y[j1 * wc + i0] * (1.f - tu) + y[j1 * wc + i1] * tu) * tv;
     e[brick_row(bk, c, yy) + brick_col(j)] = (float)(sig * nscale * (double)ev);
 }
+__global__ __launch_bounds__(256) void k_prolong_bricked(const float* __restrict__ y,
+                                                         const double* __restrict__ derived,
+                                                         int w, int h, int f, int wc, int hc,
+                                                         double nscale, Brick bk,
+                                                         float* __restrict__ e) {
+    prolong_bricked_at(y, derived, w, h, f, wc, hc, nscale, bk, e);
+}
 
 // Intermediate grid of the learned preconditioner (fm x fm fine cells per
 // cell): rm = the block sum of the projected fine residual (r16 - rmean32),
 // i.e. fm^2 times its block mean, the right-hand side of the unit-spacing
 // operator on that grid; BF16, bricked like the fine right-hand side.  One
 // thread per packed element of the intermediate grid.
+// With the start of the intermediate-grid correction in the same launch (the
+// two share the thread layout): em = the coarse output interpolated there.
 __global__ __launch_bounds__(256) void k_restrict_to_mid(const unsigned short* __restrict__ r16,
                                                          Brick bk, const SolveDev* __restrict__ st,
-                                                         int fm, int wm, Brick bkm,
-                                                         unsigned short* __restrict__ rm) {
+                                                         int fm, int wm, int hm, Brick bkm,
+                                                         unsigned short* __restrict__ rm,
+                                                         const float* __restrict__ y,
+                                                         const double* __restrict__ derived,
+                                                         int fc, int wc, int hc, double nscale,
+                                                         float* __restrict__ em) {
+    prolong_bricked_at(y, derived, wm, hm, fc, wc, hc, nscale, bkm, em);
     const int Ym = blockIdx.y;
     int c, j, xm;
     if (!packed_thread(bkm, wm, Ym, c, j, xm)) return;
@@ -1023,7 +1047,21 @@ struct FcgDev {
     double pq;     // p . Lp of the current direction
     int have;      // a previous direction exists
     int bad;       // the last step had no descent direction
+    double* phi;          // FP64 iterate of the current solve (the captured
+                          // iteration body reads it here: the graph serves
+                          // every solve of the grid size)
 };
+
+// start of the learned iterations of a solve
+__global__ void k_fcg_init(FcgDev* fd, double* phi) {
+    if (threadIdx.x != 0) return;
+    fd->beta = 0.0;
+    fd->alpha = 0.0;
+    fd->pq = 0.0;
+    fd->have = 0;
+    fd->bad = 0;
+    fd->phi = phi;
+}
 
 // Lz at packed element (c, Y, j) (cell x): the four neighbours lie in the
 // other colour plane, up/down at column j, left/right at (x -+ 1) >> 1
@@ -1104,25 +1142,32 @@ __global__ void k_fcg_coef(const double* __restrict__ partials, int nblocks, Fcg
     }
 }
 
-// p = z + beta p; q = Lz + beta q; eo = alpha p (the correction the next check
-// folds into phi64); st->cur names eo's buffer.  One thread per packed element.
+// p = z + beta p; q = Lz + beta q; phi64 += alpha p (the step, folded into
+// the FP64 iterate at once: the FP32 corrections need no flush).  One thread
+// per packed column j of row Y, both colours: cells 2j and 2j+1, so that the
+// FP64 read-modify-write of phi is contiguous across the wave.
 __global__ __launch_bounds__(256) void k_fcg_update(const float* __restrict__ z,
                                                     float* __restrict__ p, float* __restrict__ q,
-                                                    float* __restrict__ eo,
                                                     const FcgDev* __restrict__ fd, Brick bk,
-                                                    int w, int h, SolveDev* st, int eo_index) {
+                                                    int w, int h) {
     const int Y = blockIdx.y;
-    if (blockIdx.x == 0 && Y == 0 && blockIdx.z == 0 && threadIdx.x == 0) st->cur = eo_index;
-    int c, j, x;
-    if (!packed_thread(bk, w, Y, c, j, x)) return;
-    const size_t i = brick_row(bk, c, Y) + brick_col(j);
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= bk.wpb) return;
+    double* __restrict__ phi = fd->phi;
     const float beta = (float)fd->beta, alpha = (float)fd->alpha;
-    const float zc = z[i];
-    const float lz = lap_packed(z, bk, c, Y, j, x, w, h, zc);
-    const float pn = zc + beta * p[i];
-    p[i] = pn;
-    q[i] = lz + beta * q[i];
-    eo[i] = alpha * pn;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int x = 2 * j + t;
+        if (x >= w) break;
+        const int c = (x + Y) & 1;
+        const size_t i = brick_row(bk, c, Y) + brick_col(j);
+        const float zc = z[i];
+        const float lz = lap_packed(z, bk, c, Y, j, x, w, h, zc);
+        const float pn = zc + beta * p[i];
+        p[i] = pn;
+        q[i] = lz + beta * q[i];
+        phi[(size_t)Y * w + x] += (double)(alpha * pn);
+    }
 }
 
 // coarse FP64 block mean of phi
@@ -1229,10 +1274,13 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     constexpr int kResidualRows = 8;
     const dim3 cgrid(div_up(w, RED_BX), div_up(h, RED_BY * kResidualRows));
     const int ncheck = cgrid.x * cgrid.y;  // <= nred
-    auto enqueue_check = [&](int check_id) -> int {
+    // flush: fold the current FP32 correction into phi64 first (the learned
+    // iterations add theirs to phi64 themselves)
+    auto enqueue_check = [&](int check_id, bool flush = true) -> int {
         CD_TRY(timer_begin());
         if (blocked) {
-            hipLaunchKernelGGL(k_flush_packed, rgrid, rblk, 0, s, phi, e32, e32b, sd, w, h, bk);
+            if (flush)
+                hipLaunchKernelGGL(k_flush_packed, rgrid, rblk, 0, s, phi, e32, e32b, sd, w, h, bk);
             if (track)
                 hipLaunchKernelGGL((k_residual<3>), cgrid, rblk, 0, s, phi, b, mask, sd, r32, r16,
                                    bk, w, h, kResidualRows, partials);
@@ -1247,12 +1295,11 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
             hipLaunchKernelGGL((k_residual<0>), cgrid, rblk, 0, s, phi, b, mask, sd, r32, r16, bk,
                                w, h, kResidualRows, partials);
         }
-        hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, partials, ncheck, sd, count,
-                           check_id, o->stop, o->tol);
+        hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, partials, ncheck, sd,
+                           ctx->host_status, count, check_id, o->stop, o->tol);
         CD_CHECK_LAUNCH(ctx);
-        CD_TRY(timer_end(CD_KERNEL_RESIDUAL, mixed ? 3 : 2,
-                              (blocked ? 48.0 : (mixed ? 44.0 : 16.0)) * (double)n));
-        CD_HIP(ctx, hipMemcpyAsync(ctx->host_status, sd, sizeof(SolveDev), hipMemcpyDeviceToHost, s));
+        CD_TRY(timer_end(CD_KERNEL_RESIDUAL, (mixed && flush) ? 3 : 2,
+                              (blocked ? (flush ? 48.0 : 18.0) : (mixed ? 44.0 : 16.0)) * (double)n));
         CD_HIP(ctx, hipStreamSynchronize(s));
         CD_TRY(ctx->resolve_timers());
         return CD_OK;
@@ -1435,8 +1482,6 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     const int dot_blocks = 2 * pbx * dot_rows;
     if (dot_blocks > nred) learned = false;  // tiny grids: partials hold nred blocks
     float* zbuf = (smooth_launches & 1) ? e32b : e32;
-    float* obuf = (smooth_launches & 1) ? e32 : e32b;
-    const int obuf_index = (smooth_launches & 1) ? 0 : 1;
     int learned_applied = 0, learned_rejected = 0;
 
     int check_id = 0;
@@ -1455,7 +1500,8 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     if (learned && !done) {
         CD_TRY(cd_fill_f32(ctx, ctx->fcg_p.as<float>(), npk, 0.f));
         CD_TRY(cd_fill_f32(ctx, ctx->fcg_q.as<float>(), npk, 0.f));
-        CD_TRY(cd_fill_f64(ctx, ctx->fcg_dev.as<double>(), sizeof(FcgDev) / sizeof(double), 0.0));
+        hipLaunchKernelGGL(k_fcg_init, dim3(1), dim3(64), 0, s, ctx->fcg_dev.as<FcgDev>(), phi);
+        CD_CHECK_LAUNCH(ctx);
         CD_TRY(ctx->ensure(ctx->phi_start, sizeof(double) * n));
         CD_HIP(ctx, hipMemcpyAsync(ctx->phi_start.ptr, phi, sizeof(double) * n,
                                    hipMemcpyDeviceToDevice, s));
@@ -1482,10 +1528,8 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
             unsigned short* rm = ctx->fcg_rm.as<unsigned short>();
             float* em[2] = {ctx->fcg_em.as<float>(), ctx->fcg_emb.as<float>()};
             hipLaunchKernelGGL(k_restrict_to_mid, dim3(pbm, hm, 2), dim3(256), 0, s, r16, bk, sd,
-                               fm, wm, bkm, rm);
-            hipLaunchKernelGGL(k_prolong_bricked, dim3(pbm, hm, 2), dim3(256), 0, s,
-                               ctx->lc_y.as<float>(), stats + 2, wm, hm, lf / fm, lwc, lhc, nscale,
-                               bkm, em[0]);
+                               fm, wm, hm, bkm, rm, ctx->lc_y.as<float>(), stats + 2, lf / fm,
+                               lwc, lhc, nscale, em[0]);
             const int tx = div_up(wm, bix), ntm = tx * div_up(hm, biy);
             const int gm = blocked_grid(ntm);
             int cur_m = 0;
@@ -1513,11 +1557,10 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
                            ctx->fcg_p.as<float>(), ctx->fcg_q.as<float>(), r16, sd, bk, w, h,
                            partials);
         hipLaunchKernelGGL(k_fcg_coef, dim3(1), dim3(1024), 0, s, partials, dot_blocks, fd);
-        hipLaunchKernelGGL(k_fcg_update, packed_grid, dim3(256), 0, s, zbuf,
-                           ctx->fcg_p.as<float>(), ctx->fcg_q.as<float>(), obuf, fd, bk, w, h,
-                           sd, obuf_index);
+        hipLaunchKernelGGL(k_fcg_update, dim3(pbx, h), dim3(256), 0, s, zbuf,
+                           ctx->fcg_p.as<float>(), ctx->fcg_q.as<float>(), fd, bk, w, h);
         CD_CHECK_LAUNCH(ctx);
-        CD_TRY(timer_end(CD_KERNEL_LEARNED, 3, 26.0 * (double)n));
+        CD_TRY(timer_end(CD_KERNEL_LEARNED, 2, 38.0 * (double)n));
         return CD_OK;
     };
     int fcg_iter = 0;
@@ -1554,7 +1597,7 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         }
         ++fcg_iter;
         cum += nsmooth;
-        CD_TRY(enqueue_check(++check_id));
+        CD_TRY(enqueue_check(++check_id, false));
         ++learned_applied;
         FcgDev* fd = ctx->fcg_dev.as<FcgDev>();
         (void)fd;
@@ -1582,14 +1625,19 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
             learned = false;
             if (!(hs.rel <= start_rel) && !done) {
                 // back to the warm start; the next check recomputes its residual
-                // (and folds the zeroed corrections)
                 CD_HIP(ctx, hipMemcpyAsync(phi, ctx->phi_start.ptr, sizeof(double) * n,
                                            hipMemcpyDeviceToDevice, s));
-                CD_TRY(enqueue_check(++check_id));
+                CD_TRY(enqueue_check(++check_id, false));
                 done = ctx->host_status->done != 0;
             }
         }
         prev_rel = ctx->host_status->rel;
+    }
+    // plain sweeps after learned iterations start from a zero correction (the
+    // iterations left their FP32 work in both ping-pong buffers)
+    if (fcg_iter > 0 && !done) {
+        CD_TRY(cd_fill_f32(ctx, e32, npk, 0.f));
+        CD_TRY(cd_fill_f32(ctx, e32b, npk, 0.f));
     }
     // Max-update rule in mixed precision: the sweeps see the right-hand side
     // rounded (BF16 / FP32) at the last check, which perturbs every update by

@@ -40,8 +40,9 @@ def test_net_forward_matches_netref(ctx, linear):
 
 @pytest.mark.parametrize("linear", [1, 0])
 def test_fused_forward_is_bit_identical(ctx, linear):
-    """Upsampled decoder inputs read through the convolutions (inference)
-    give exactly the output of the materialised tensors (training)."""
+    """Upsampled decoder inputs and pooled encoder inputs read through the
+    convolutions (inference) give exactly the output of the materialised
+    tensors (training)."""
     arch = capi.default_arch(linear=linear, channels=(16, 32, 32, 32, 0, 0))
     net = capi.Net(ctx, arch)
     net.set_params(NetRef(4, (16, 32, 32, 32), linear=bool(linear)).random_params(3))
