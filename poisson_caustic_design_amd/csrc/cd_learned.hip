@@ -716,6 +716,11 @@ size_t tiled_lds_limit(cd_context* c) {
     return (size_t)bytes;
 }
 
+#ifndef CD_CONV_WIDE_MIN_BLOCKS
+#define CD_CONV_WIDE_MIN_BLOCKS 512
+#endif
+constexpr long long kConvWideMinBlocks = CD_CONV_WIDE_MIN_BLOCKS;
+
 // mode1: 0 reads in1 as stored; 1 reads `x1src` (the half-resolution tensor
 // in1 is upsampled from) through nearest 2x upsampling instead; 2 reads
 // `x1src` (the double-resolution tensor in1 is pooled from) through 2x2
@@ -729,7 +734,11 @@ int launch_conv(cd_net* n, const NetConv& cv, int B, int H, int W, bool grad_inp
     const int blocks = div_up(npix, 64);
     // LDS the tiled kernel needs for this convolution (8x8 tile + halo, the
     // weights of up to 32 output channels)
-    const int nbw_t = (cv.coutpad / 16) % 2 == 0 ? 2 : 1;
+    // 32 output channels per workgroup where the image has enough 8x8 tiles to
+    // fill the GPU, else 16 (twice the workgroups, half the weights staged by
+    // each: the deep levels' convolutions are bound by that staging)
+    const long long tiles_t = (long long)B * div_up(H, CT) * div_up(W, CT);
+    const int nbw_t = ((cv.coutpad / 16) % 2 == 0 && tiles_t * (cv.coutpad / 32) >= kConvWideMinBlocks) ? 2 : 1;
     const size_t lds_t = sizeof(u16) * (100 * (size_t)(cv.c1 + cv.c2 + 8) +
                                         (size_t)nbw_t * 16 * (cv.kpad + 8));
     const bool tiled = !grad_input && n->tiled_forward && lds_t <= tiled_lds_limit(c);

@@ -264,7 +264,9 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ phi
 __global__ void k_check(const double* __restrict__ partials, int nblocks, SolveDev* st,
                         SolveDev* host_st, double count, int check_id, int stop_rule,
                         double tol) {
+    static_assert(sizeof(SolveDev) % 4 == 0 && sizeof(SolveDev) / 4 <= 1024, "status words");
     __shared__ double smem[16];
+    __shared__ unsigned words[sizeof(SolveDev) / 4];
     double a = 0.0, q = 0.0;
     for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
         a += partials[2 * b];
@@ -290,9 +292,12 @@ __global__ void k_check(const double* __restrict__ partials, int nblocks, SolveD
             st->done_check = check_id;
         }
         const unsigned* src = reinterpret_cast<const unsigned*>(st);
-        unsigned* dst = reinterpret_cast<unsigned*>(host_st);
-        for (int k = 0; k < (int)(sizeof(SolveDev) / 4); ++k) dst[k] = src[k];
+        for (int k = 0; k < (int)(sizeof(SolveDev) / 4); ++k) words[k] = src[k];
     }
+    __syncthreads();
+    // one word per thread: the stores to host memory go out together
+    if (threadIdx.x < sizeof(SolveDev) / 4)
+        reinterpret_cast<unsigned*>(host_st)[threadIdx.x] = words[threadIdx.x];
 }
 
 // ---------------------------------------------------------------------------
@@ -848,7 +853,7 @@ __device__ __forceinline__ size_t brick_at(const Brick& bk, int x, int y) {
 }
 
 // coarse residual rc = block mean of the projected BF16 residual the blocked
-// sweeps see (r16 - rmean32).  f <= 8: one thread per coarse cell; larger f:
+// sweeps see (r16 - rmean32).  f < 8: one thread per coarse cell; larger f:
 // one wave per coarse cell (lanes stride over its f^2 fine cells, fixed-order
 // shuffle tree).  Statistics of rc follow in k_coarse_stats.
 __global__ __launch_bounds__(256) void k_restrict_r16(const unsigned short* __restrict__ r16,
@@ -857,7 +862,7 @@ __global__ __launch_bounds__(256) void k_restrict_r16(const unsigned short* __re
                                                       float* __restrict__ rc) {
     const float rm = st->rmean32;
     const int ff = f * f;
-    if (f <= 8) {
+    if (f < 8) {
         const int i = blockIdx.x * blockDim.x + threadIdx.x;
         if (i >= wc * hc) return;
         const int J = i / wc, I = i - J * wc;
@@ -1270,8 +1275,9 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     if (blocked) CD_TRY(cd_fill_f32(ctx, e32b, npk, 0.f));
 
     const int fblocks = std::min(div_up((long long)n, CD_BLOCK), 65536);
-    // residual launch of a check: kResidualRows rows per thread
-    constexpr int kResidualRows = 8;
+    // residual launch of a check: kResidualRows rows per thread, fewer on
+    // smaller grids (more workgroups in flight; the partials stay <= nred)
+    const int kResidualRows = std::max(1, std::min(8, nred / 2048));
     const dim3 cgrid(div_up(w, RED_BX), div_up(h, RED_BY * kResidualRows));
     const int ncheck = cgrid.x * cgrid.y;  // <= nred
     // flush: fold the current FP32 correction into phi64 first (the learned
@@ -1425,7 +1431,7 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     // slot receives (sum, sum^2) of the coarse residual
     auto restrict_stats = [&](int f, int wc, int hc, double* stats, double* derived) -> int {
         const int nc = wc * hc;
-        const int rb = f <= 8 ? div_up(nc, 256) : div_up(nc, 4);  // thread / wave per cell
+        const int rb = f < 8 ? div_up(nc, 256) : div_up(nc, 4);  // thread / wave per cell
         hipLaunchKernelGGL(k_restrict_r16, dim3(rb), dim3(256), 0, s, r16, bk, sd, f, wc, hc,
                            ctx->lc_r.as<float>());
         if (nc <= 65536) {
@@ -1476,7 +1482,7 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     const int pbx = div_up(bk.wpb, 256);
     const dim3 packed_grid(pbx, h, 2);
     // (at most 256 row groups: each thread accumulates several rows before
-    // the block reduction)
+    // the block reduction; 1024 measured slower at 1024^2)
     const int dot_rows = std::max(1, std::min(std::min(h, 256), nred / (2 * pbx)));
     const dim3 dot_grid(pbx, dot_rows, 2);
     const int dot_blocks = 2 * pbx * dot_rows;

@@ -102,14 +102,15 @@ def random_fields(count: int, h: int, w: int, seed: int = 0) -> np.ndarray:
 
 def capture_inputs(ctx: capi.Context, seeds, mesh_res: int, config: str = "mixture",
                    capacity: int = 4096, coarsen: int = 4, net: capi.Net = None,
-                   tol: float = 1e-6, max_ot: int = 50, log=print):
+                   tol: float = 1e-6, max_ot: int = 50, log=print, every: int = 1):
     """Coarse residuals met by solves of seeded designs (with `net` in use when
-    given); returns a torch tensor [count][h][w] on the device."""
+    given), one per `every` residual checks; returns a torch tensor
+    [count][h][w] on the device."""
     if net is not None:
         ctx.set_net(net)
     try:
-        x, _ = generate_pairs(ctx, seeds, mesh_res, config, capacity, coarsen, 1, tol, max_ot,
-                              log=log, learned_init=1 if net is not None else 0)
+        x, _ = generate_pairs(ctx, seeds, mesh_res, config, capacity, coarsen, every, tol,
+                              max_ot, log=log, learned_init=1 if net is not None else 0)
     finally:
         if net is not None:
             ctx.set_net(None)
@@ -174,7 +175,7 @@ def train(net: capi.Net, x, t, steps: int = 2000, batch: int = 8, lr: float = 1e
 
 def fit(ctx: capi.Context, arch: capi.NetArch, mesh_res: int, seeds, config: str = "mixture",
         random_count: int = 1000, capacity: int = 4096, steps: int = 3000, batch: int = 16,
-        lr: float = 2e-3, rounds: int = 1, seed: int = 0, log=print):
+        lr: float = 2e-3, rounds: int = 1, seed: int = 0, log=print, every: int = 1):
     """Train a network for solves of (4*mesh_res)^2 grids: solver residuals of
     plain-SOR designs plus random fields, then `rounds` more passes that add
     the residuals met with the network in use.  Returns (net, losses, inputs)."""
@@ -182,7 +183,8 @@ def fit(ctx: capi.Context, arch: capi.NetArch, mesh_res: int, seeds, config: str
 
     n = 4 * mesh_res
     hc = wc = n // arch.coarsen
-    xs = [capture_inputs(ctx, seeds, mesh_res, config, capacity, arch.coarsen, log=log)]
+    xs = [capture_inputs(ctx, seeds, mesh_res, config, capacity, arch.coarsen, log=log,
+                         every=every)]
     xs.append(torch.from_numpy(random_fields(random_count, hc, wc, seed)).to(xs[0].device))
     pool = torch.cat(xs)
     if log:
@@ -192,7 +194,7 @@ def fit(ctx: capi.Context, arch: capi.NetArch, mesh_res: int, seeds, config: str
     losses = [train_operator(net, pool, steps, batch, lr, seed, log=log)]
     for r in range(rounds):
         more = capture_inputs(ctx, seeds, mesh_res, config, capacity, arch.coarsen, net=net,
-                              log=log)
+                              log=log, every=every)
         pool = torch.cat([pool, more])
         if log:
             log(f"round {r + 1}: +{more.shape[0]} inputs met with the network in use")
@@ -407,6 +409,8 @@ def main(argv=None):
                     help="smoothing sweeps on the intermediate grid (coarsen/4 cells per cell)")
     ap.add_argument("--random", type=int, default=1200)
     ap.add_argument("--capacity", type=int, default=4000)
+    ap.add_argument("--capture-every", type=int, default=1,
+                    help="keep the coarse residual of one in this many checks")
     ap.add_argument("--steps", type=int, default=8000)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=5e-4)
@@ -461,11 +465,12 @@ def main(argv=None):
     t0 = time.time()
     net, losses, pool = fit(ctx, arch, a.mesh_res, range(a.seed0, a.seed0 + a.designs), a.config,
                             a.random, a.capacity, a.steps, a.batch, a.lr, a.rounds, a.seed0,
-                            log=log)
+                            log=log, every=a.capture_every)
     out = a.out or os.path.join(WEIGHTS_DIR, f"learned_grid{4 * a.mesh_res}.npz")
     os.makedirs(os.path.dirname(out), exist_ok=True)
     meta = {"mesh_res": a.mesh_res, "designs": a.designs, "seed0": a.seed0, "config": a.config,
-            "inputs": int(pool.shape[0]), "steps": a.steps, "rounds": a.rounds,
+            "inputs": int(pool.shape[0]), "capture_every": a.capture_every,
+            "steps": a.steps, "rounds": a.rounds,
             "loss": a.loss, "final_loss": float(np.mean(losses[-100:])),
             "train_seconds": round(time.time() - t0, 1)}
     save_params(net, out, meta)
