@@ -160,7 +160,7 @@ int cd_context_create(int device, cd_context** out) {
     ctx->device = device;
     // default-priority, non-blocking stream
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipHostMalloc((void**)&ctx->host_status, sizeof(SolveDev), hipHostMallocDefault) !=
+        hipHostMalloc((void**)&ctx->host_status, sizeof(SolveDev), hipHostMallocCoherent) !=
             hipSuccess) {
         cd_context_destroy(ctx);
         return CD_ERR_HIP;

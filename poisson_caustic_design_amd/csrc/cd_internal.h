@@ -73,6 +73,8 @@ struct SolveDev {
     float rmean32;       // mean of the last FP32 residual (projected out in sweeps)
     int cur;             // blocked path: ping-pong buffer holding the current correction
     unsigned int maxupd_bits[16];  // per-sweep max |delta| (float bits) for the max-update rule
+    unsigned int seq;    // host mirror: written last by a check (the host waits for it)
+    unsigned int pad_;
 };
 
 // An event pair recorded around a group of launches; resolved (elapsed time
@@ -121,7 +123,8 @@ struct cd_context {
 
     // solver workspace (grow-only)
     DevBuf e32, e32b, r32, e64w, phi64w, b64w, mask, partials, solve_dev, fill;
-    SolveDev* host_status = nullptr;  // pinned
+    SolveDev* host_status = nullptr;  // pinned, coherent: checks write it directly
+    unsigned int status_seq = 0;      // sequence number of the last check enqueued
     int ws_w = 0, ws_h = 0;
 
     // learned-init network (owned by the caller) and its solver workspace
@@ -240,5 +243,9 @@ const cd_net_arch* cd_net_arch_of(const cd_net* n);
 // forward on a coarse residual normalised on the fly (stats = sum, sum^2)
 int cd_net_forward_coarse(cd_net* n, const float* rc, const double* stats, float* d_y, int h,
                           int w);
+// the net's input buffer for an h x w coarse field (BF16, 8 channels per
+// cell: the normalised residual, then zeros), and the forward on it
+int cd_net_coarse_input(cd_net* n, int h, int w, unsigned short** x8);
+int cd_net_forward_prepared(cd_net* n, float* d_y, int h, int w);
 // bumped whenever the net's activation buffers move (batch shape changes)
 unsigned long long cd_net_gen(const cd_net* n);

@@ -1205,6 +1205,20 @@ int cd_net_forward_coarse(cd_net* n, const float* rc, const double* stats, float
     return finish_output(n, 1, h, w, d_y);
 }
 
+// The network's BF16 input buffer for one h x w coarse field (sized here),
+// for a producer that writes the normalised input itself (channel 0, the
+// other 7 zero), and the forward on that input
+int cd_net_coarse_input(cd_net* n, int h, int w, unsigned short** x8) {
+    CD_TRY(ensure_batch(n, 1, h, w));
+    *x8 = n->x8.as<u16>();
+    return CD_OK;
+}
+int cd_net_forward_prepared(cd_net* n, float* d_y, int h, int w) {
+    CD_TRY(ensure_batch(n, 1, h, w));
+    CD_TRY(forward(n, 1, h, w, n->fused_forward));
+    return finish_output(n, 1, h, w, d_y);
+}
+
 // solver-side entry: forward on one coarse field (batch 1), output FP32 [h][w]
 int cd_net_forward_internal(cd_net* n, const float* d_x, float* d_y, int h, int w) {
     CD_TRY(prep(n, d_x, 1, h, w));

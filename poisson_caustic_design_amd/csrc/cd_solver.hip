@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <cstddef>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -149,6 +151,10 @@ __device__ __forceinline__ size_t brick_row(const Brick& b, int c, int Y) {
     return (size_t)c * b.plane + ((size_t)(Y >> 3) * b.nbx) * 512 + (size_t)(Y & 7) * 64;
 }
 __device__ __forceinline__ unsigned brick_col(int j) { return (unsigned)((j >> 6) * 512 + (j & 63)); }
+// bricked index of cell (x, y)
+__device__ __forceinline__ size_t brick_at(const Brick& bk, int x, int y) {
+    return brick_row(bk, (x + y) & 1, y) + brick_col(x >> 1);
+}
 // Buffer access to a bricked plane pair of at most 2^31 bytes: scalar
 // resource + scalar (row) byte offset + the lane's vector byte offset, no
 // per-lane 64-bit address arithmetic.  Every access stays in range.
@@ -258,15 +264,18 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ phi
     }
 }
 
-// single block: finish a convergence check; the status also goes to the
-// host's pinned mirror (read after the stream has been synchronised), which
-// saves a copy per check
+// single block: finish a convergence check.  The status also goes to the
+// host's pinned coherent mirror, its sequence word `seq` last, after every
+// other word has been written: the host waits for that word instead of
+// synchronising the stream (a round trip of ~30 us per check).
 __global__ void k_check(const double* __restrict__ partials, int nblocks, SolveDev* st,
-                        SolveDev* host_st, double count, int check_id, int stop_rule,
-                        double tol) {
-    static_assert(sizeof(SolveDev) % 4 == 0 && sizeof(SolveDev) / 4 <= 1024, "status words");
+                        SolveDev* host_st, unsigned seq, double count, int check_id,
+                        int stop_rule, double tol) {
+    constexpr int kWords = (int)(sizeof(SolveDev) / 4);
+    constexpr int kSeq = (int)(offsetof(SolveDev, seq) / 4);
+    static_assert(sizeof(SolveDev) % 4 == 0 && kWords <= 1024, "status words");
     __shared__ double smem[16];
-    __shared__ unsigned words[sizeof(SolveDev) / 4];
+    __shared__ unsigned words[kWords];
     double a = 0.0, q = 0.0;
     for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
         a += partials[2 * b];
@@ -291,13 +300,19 @@ __global__ void k_check(const double* __restrict__ partials, int nblocks, SolveD
             st->done = 1;
             st->done_check = check_id;
         }
+        st->seq = seq;
         const unsigned* src = reinterpret_cast<const unsigned*>(st);
-        for (int k = 0; k < (int)(sizeof(SolveDev) / 4); ++k) words[k] = src[k];
+        for (int k = 0; k < kWords; ++k) words[k] = src[k];
     }
     __syncthreads();
-    // one word per thread: the stores to host memory go out together
-    if (threadIdx.x < sizeof(SolveDev) / 4)
-        reinterpret_cast<unsigned*>(host_st)[threadIdx.x] = words[threadIdx.x];
+    // one word per thread, then the sequence word as a system-scope release
+    // (it makes the other threads' words, ordered before it by the barrier,
+    // visible to the host first)
+    unsigned* dst = reinterpret_cast<unsigned*>(host_st);
+    if (threadIdx.x < kWords && threadIdx.x != kSeq) dst[threadIdx.x] = words[threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        __hip_atomic_store(dst + kSeq, words[kSeq], __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // ---------------------------------------------------------------------------
@@ -635,26 +650,52 @@ __device__ __forceinline__ void region_setup(float (&ev)[2][BK_R], const RT (&rr
 // re-rounds the current residual, so the rounding never reaches the result)
 // and FP32 under the max-update rule (TRACK), which cannot see the smooth
 // error a rounded right-hand side leaves behind.
+// The intermediate-grid correction em (bricked, wm x hm, fm x fm fine cells
+// per cell) interpolated bilinearly at the centre of fine cell (xx, yy).
+struct MidGrid {
+    const float* em;
+    Brick bkm;
+    int wm, hm, fm;
+};
+__device__ __forceinline__ float prolong_mid_at(const MidGrid& m, int xx, int yy) {
+    const float inv = 1.f / m.fm;
+    const float u = (xx + 0.5f) * inv - 0.5f, v = (yy + 0.5f) * inv - 0.5f;
+    int i0 = (int)floorf(u), j0 = (int)floorf(v);
+    const float tu = u - i0, tv = v - j0;
+    const int i1 = min(max(i0 + 1, 0), m.wm - 1), j1 = min(max(j0 + 1, 0), m.hm - 1);
+    i0 = min(max(i0, 0), m.wm - 1);
+    j0 = min(max(j0, 0), m.hm - 1);
+    const float a = m.em[brick_at(m.bkm, i0, j0)], b = m.em[brick_at(m.bkm, i1, j0)];
+    const float cc = m.em[brick_at(m.bkm, i0, j1)], d = m.em[brick_at(m.bkm, i1, j1)];
+    return (a * (1.f - tu) + b * tu) * (1.f - tv) + (cc * (1.f - tu) + d * tu) * tv;
+}
+
 // Loads of one region: packed column `lane` of the wave's BK_R rows, both
 // colours, from the correction and the right-hand side.  Rows and columns are
 // clamped into the grid (cells outside it are masked by region_setup).
-template <int H, typename RT>
+// PROLONG: the correction is the intermediate-grid correction interpolated at
+// the cells (the start of the learned iteration's smoothing), not ein.
+template <int H, bool PROLONG, typename RT>
 __device__ __forceinline__ void region_load(float (&ev)[2][BK_R], RT (&rr)[2][BK_R],
                                             __amdgpu_buffer_rsrc_t ein_rs,
                                             __amdgpu_buffer_rsrc_t r_rs, const Brick& bk,
-                                            int tile, int tiles_x, int w, int h, int wave,
-                                            int lane) {
+                                            const MidGrid& mid, int tile, int tiles_x, int w,
+                                            int h, int wave, int lane) {
     constexpr int IX = BK_RX - 2 * H, IY = BK_RY - 2 * H;
     const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
     const int X0 = tx * IX - H, Yw = ty * IY - H + BK_R * wave;
-    const unsigned col = brick_col(min(max(X0 / 2 + lane, 0), (w + 1) / 2 - 1));
+    const int j = min(max(X0 / 2 + lane, 0), (w + 1) / 2 - 1);
+    const unsigned col = brick_col(j);
 #pragma unroll
     for (int k = 0; k < BK_R; ++k) {
         const int Yc = min(max(Yw + k, 0), h - 1);
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
             const unsigned row = (unsigned)brick_row(bk, c, Yc);
-            ev[c][k] = buf_load<float>(ein_rs, col * 4u, row * 4u);
+            if (PROLONG)
+                ev[c][k] = prolong_mid_at(mid, min(2 * j + ((c + Yc) & 1), w - 1), Yc);
+            else
+                ev[c][k] = buf_load<float>(ein_rs, col * 4u, row * 4u);
             rr[c][k] = buf_load<RT>(r_rs, col * (unsigned)sizeof(RT), row * (unsigned)sizeof(RT));
         }
     }
@@ -709,8 +750,9 @@ __device__ __forceinline__ void blocked_tile(float (&ev)[2][BK_R], const RT (&rr
 inline int blocked_grid(int ntiles) { return (ntiles + 7) / 8 * 8; }
 
 // FULL: nhalf == H, known at compile time (every launch but a batch's last),
-// which leaves the half-sweeps free of branches.
-template <int H, bool TRACK, bool FULL>
+// which leaves the half-sweeps free of branches.  PROLONG: the sweeps start
+// from `mid` interpolated at the cells instead of from ein.
+template <int H, bool TRACK, bool FULL, bool PROLONG = false>
 __global__ __launch_bounds__(BK_THREADS) void k_rb_blocked(const float* __restrict__ ein,
                                                     float* __restrict__ eout,
                                                     const void* __restrict__ rhs,
@@ -718,7 +760,7 @@ __global__ __launch_bounds__(BK_THREADS) void k_rb_blocked(const float* __restri
                                                     int tiles_x, int ntiles, int nhalf,
                                                     float om2, float om3, float om4,
                                                     const float* __restrict__ rmean,
-                                                    SolveDev* st, int out_index) {
+                                                    SolveDev* st, int out_index, MidGrid mid) {
     static_assert(H % 2 == 0 && H >= 2 && 2 * H < BK_RY && BK_R % 2 == 0, "geometry");
     __shared__ float xch[2][BK_NW][2][64];
     __shared__ float smax[BK_NW][H / 2];
@@ -737,7 +779,7 @@ __global__ __launch_bounds__(BK_THREADS) void k_rb_blocked(const float* __restri
     const __amdgpu_buffer_rsrc_t eout_rs = plane_rsrc(eout, npk * sizeof(float));
     float ev[2][BK_R];
     RT rr[2][BK_R];
-    region_load<H>(ev, rr, ein_rs, r_rs, bk, tile, tiles_x, w, h, wave, lane);
+    region_load<H, PROLONG>(ev, rr, ein_rs, r_rs, bk, mid, tile, tiles_x, w, h, wave, lane);
     // Max-update rule: the stop flag can be set between the launches of one
     // batch (k_maxcheck_blocked), and a finished solve leaves the buffers
     // untouched.  Residual rule: only a check sets it, and no sweeps are
@@ -847,11 +889,6 @@ int auto_check_every(int w, int h) {
 // Neumann counts).  Coarse cell (I,J) averages the f x f fine cells
 // [f*I, f*I+f) x [f*J, f*J+f).
 // ---------------------------------------------------------------------------
-// bricked index of cell (x, y)
-__device__ __forceinline__ size_t brick_at(const Brick& bk, int x, int y) {
-    return brick_row(bk, (x + y) & 1, y) + brick_col(x >> 1);
-}
-
 // coarse residual rc = block mean of the projected BF16 residual the blocked
 // sweeps see (r16 - rmean32).  f < 8: one thread per coarse cell; larger f:
 // one wave per coarse cell (lanes stride over its f^2 fine cells, fixed-order
@@ -897,11 +934,15 @@ __device__ __forceinline__ void coarse_derived(const double* stats, int nc, doub
 }
 
 // (sum, sum^2) of the coarse residual in one workgroup (fixed order), for
-// coarse grids of up to 64Ki cells; derived (may be NULL) receives (mean, rms)
+// coarse grids of up to 64Ki cells; derived (may be NULL) receives (mean, rms);
+// x8 (may be NULL) receives the network input: the residual normalised to
+// zero mean and unit rms in BF16, channel 0 of 8
 __global__ __launch_bounds__(1024) void k_coarse_stats1(const float* __restrict__ rc, int nc,
                                                         double* __restrict__ stats,
-                                                        double* __restrict__ derived) {
+                                                        double* __restrict__ derived,
+                                                        unsigned short* __restrict__ x8) {
     __shared__ double smem[16];
+    __shared__ double ms[2];
     double a = 0.0, q = 0.0;
     for (int i = threadIdx.x; i < nc; i += blockDim.x) {
         const double v = rc[i];
@@ -914,6 +955,18 @@ __global__ __launch_bounds__(1024) void k_coarse_stats1(const float* __restrict_
         stats[0] = a;
         stats[1] = q;
         if (derived) coarse_derived(stats, nc, derived);
+        const double mean = a / nc;
+        const double var = q / nc - mean * mean;
+        ms[0] = mean;
+        ms[1] = sqrt(var > 1e-300 ? var : 1e-300);
+    }
+    if (!x8) return;
+    __syncthreads();
+    const double mean = ms[0], sig = ms[1];
+    for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+        uint4 v = make_uint4(0, 0, 0, 0);
+        v.x = f32_to_bf16((float)(((double)rc[i] - mean) / sig));
+        *reinterpret_cast<uint4*>(x8 + (size_t)i * 8) = v;
     }
 }
 
@@ -1014,27 +1067,6 @@ __global__ __launch_bounds__(256) void k_restrict_to_mid(const unsigned short* _
         for (int dx = 0; dx < fm; ++dx)
             acc += bf16_to_f32(r16[brick_at(bk, xm * fm + dx, Ym * fm + dy)]) - rmean;
     rm[brick_row(bkm, c, Ym) + brick_col(j)] = f32_to_bf16(acc);
-}
-
-// fine e = bilinear interpolation of the intermediate-grid correction em
-// (bricked) at fine cell centres; one thread per fine packed element
-__global__ __launch_bounds__(256) void k_prolong_mid(const float* __restrict__ em, Brick bkm,
-                                                     int wm, int hm, int fm, int w, int h,
-                                                     Brick bk, float* __restrict__ e) {
-    const int yy = blockIdx.y;
-    int c, j, xx;
-    if (!packed_thread(bk, w, yy, c, j, xx)) return;
-    const float inv = 1.f / fm;
-    const float u = (xx + 0.5f) * inv - 0.5f, v = (yy + 0.5f) * inv - 0.5f;
-    int i0 = (int)floorf(u), j0 = (int)floorf(v);
-    const float tu = u - i0, tv = v - j0;
-    const int i1 = min(max(i0 + 1, 0), wm - 1), j1 = min(max(j0 + 1, 0), hm - 1);
-    i0 = min(max(i0, 0), wm - 1);
-    j0 = min(max(j0, 0), hm - 1);
-    const float a = em[brick_at(bkm, i0, j0)], b = em[brick_at(bkm, i1, j0)];
-    const float cc = em[brick_at(bkm, i0, j1)], d = em[brick_at(bkm, i1, j1)];
-    e[brick_row(bk, c, yy) + brick_col(j)] = (a * (1.f - tu) + b * tu) * (1.f - tv) +
-                                             (cc * (1.f - tu) + d * tu) * tv;
 }
 
 // ---------------------------------------------------------------------------
@@ -1202,6 +1234,30 @@ __global__ void k_capture_finish(const double* __restrict__ final_c,
 
 }  // namespace
 
+// Wait until the check with sequence number seq has written the host's status
+// mirror: spin on its sequence word (the status arrives within microseconds of
+// the check), then, past a few milliseconds (long sweep batches, or a failed
+// launch), synchronise the stream, which also reports errors.
+int cd_wait_status(cd_context* ctx, unsigned seq) {
+    const volatile unsigned* w = &ctx->host_status->seq;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0;; ++i) {
+        if (*w == seq) {
+            std::atomic_thread_fence(std::memory_order_acquire);
+            return CD_OK;
+        }
+        if ((i & 255) == 255 &&
+            std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(4))
+            break;
+    }
+    CD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->host_status->seq != seq) {
+        ctx->set_error("a check finished without writing the host status");
+        return CD_ERR_HIP;
+    }
+    return CD_OK;
+}
+
 int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, const uint8_t* mask,
                              long long valid_cells, int w, int h, const cd_solve_opts* o,
                              cd_solve_stats* st_out, const SolveWork* work) {
@@ -1301,13 +1357,18 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
             hipLaunchKernelGGL((k_residual<0>), cgrid, rblk, 0, s, phi, b, mask, sd, r32, r16, bk,
                                w, h, kResidualRows, partials);
         }
+        const unsigned seq = ++ctx->status_seq;
         hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, partials, ncheck, sd,
-                           ctx->host_status, count, check_id, o->stop, o->tol);
+                           ctx->host_status, seq, count, check_id, o->stop, o->tol);
         CD_CHECK_LAUNCH(ctx);
         CD_TRY(timer_end(CD_KERNEL_RESIDUAL, (mixed && flush) ? 3 : 2,
                               (blocked ? (flush ? 48.0 : 18.0) : (mixed ? 44.0 : 16.0)) * (double)n));
-        CD_HIP(ctx, hipStreamSynchronize(s));
-        CD_TRY(ctx->resolve_timers());
+        if (ctx->profiling) {
+            CD_HIP(ctx, hipStreamSynchronize(s));
+            CD_TRY(ctx->resolve_timers());
+        } else {
+            CD_TRY(cd_wait_status(ctx, seq));
+        }
         return CD_OK;
     };
 
@@ -1321,10 +1382,13 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     const int bgrid = blocked_grid(ntiles);
     const float om = (float)omega;
     const float om2 = om / 2.f, om3 = om / 3.f, om4 = om / 4.f;
-    auto enqueue_blocked_om = [&](int first, int count_sweeps, float om2, float om3,
-                                  float om4) -> int {
+    // pro: the first launch starts from this intermediate-grid correction,
+    // interpolated at the cells, instead of from e32
+    auto enqueue_blocked_om = [&](int first, int count_sweeps, float om2, float om3, float om4,
+                                  const MidGrid* pro = nullptr) -> int {
         CD_TRY(timer_begin());
         int launches = 0;
+        const MidGrid none{};
         for (int done_sw = 0; done_sw < count_sweeps;) {
             const int nsw = std::min(BH / 2, count_sweeps - done_sw);
             const bool full = 2 * nsw == BH;
@@ -1335,14 +1399,20 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
                 hipLaunchKernelGGL(full ? (k_rb_blocked<BH, true, true>) : (k_rb_blocked<BH, true, false>),
                                    dim3(bgrid), dim3(BK_THREADS), 0, s, ein, eout, r32, bk, w, h,
                                    tiles_x, ntiles, 2 * nsw, om2, om3, om4, &sd->rmean32, sd,
-                                   1 - in);
+                                   1 - in, none);
                 hipLaunchKernelGGL(k_maxcheck_blocked, dim3(1), dim3(64), 0, s, sd,
                                    first + done_sw, nsw, o->tol);
+            } else if (pro && launches == 0) {
+                hipLaunchKernelGGL(full ? (k_rb_blocked<BH, false, true, true>)
+                                        : (k_rb_blocked<BH, false, false, true>),
+                                   dim3(bgrid), dim3(BK_THREADS), 0, s, ein, eout, r16, bk, w, h,
+                                   tiles_x, ntiles, 2 * nsw, om2, om3, om4, &sd->rmean32, sd,
+                                   1 - in, *pro);
             } else {
                 hipLaunchKernelGGL(full ? (k_rb_blocked<BH, false, true>) : (k_rb_blocked<BH, false, false>),
                                    dim3(bgrid), dim3(BK_THREADS), 0, s, ein, eout, r16, bk, w, h,
                                    tiles_x, ntiles, 2 * nsw, om2, om3, om4, &sd->rmean32, sd,
-                                   1 - in);
+                                   1 - in, none);
             }
             ++launches;
             done_sw += nsw;
@@ -1429,14 +1499,19 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     }
     // restrict the residual of the last check to (wc x hc) into lc_r; stats
     // slot receives (sum, sum^2) of the coarse residual
-    auto restrict_stats = [&](int f, int wc, int hc, double* stats, double* derived) -> int {
+    // x8 (may be NULL): also write the network input there (coarse grids of up
+    // to 64Ki cells; *x8_done tells whether it was)
+    auto restrict_stats = [&](int f, int wc, int hc, double* stats, double* derived,
+                              unsigned short* x8 = nullptr, bool* x8_done = nullptr) -> int {
         const int nc = wc * hc;
         const int rb = f < 8 ? div_up(nc, 256) : div_up(nc, 4);  // thread / wave per cell
         hipLaunchKernelGGL(k_restrict_r16, dim3(rb), dim3(256), 0, s, r16, bk, sd, f, wc, hc,
                            ctx->lc_r.as<float>());
+        if (x8_done) *x8_done = false;
         if (nc <= 65536) {
             hipLaunchKernelGGL(k_coarse_stats1, dim3(1), dim3(1024), 0, s, ctx->lc_r.as<float>(),
-                               nc, stats, derived);
+                               nc, stats, derived, x8);
+            if (x8_done) *x8_done = x8 != nullptr;
         } else {
             const int nb = div_up(nc, 256);  // <= nred: the partials buffer is large enough
             hipLaunchKernelGGL(k_coarse_stats, dim3(nb), dim3(256), 0, s, ctx->lc_r.as<float>(),
@@ -1525,9 +1600,15 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         CD_TRY(timer_begin());
         // slot 0: (sum, sum^2) of the coarse residual; slot 1: (mean, rms)
         double* stats = ctx->lc_stats.as<double>();
-        CD_TRY(restrict_stats(lf, lwc, lhc, stats, stats + 2));
-        CD_TRY(cd_net_forward_coarse(ctx->net, ctx->lc_r.as<float>(), stats,
-                                     ctx->lc_y.as<float>(), lhc, lwc));
+        unsigned short* x8 = nullptr;
+        bool prepared = false;
+        CD_TRY(cd_net_coarse_input(ctx->net, lhc, lwc, &x8));
+        CD_TRY(restrict_stats(lf, lwc, lhc, stats, stats + 2, x8, &prepared));
+        if (prepared)
+            CD_TRY(cd_net_forward_prepared(ctx->net, ctx->lc_y.as<float>(), lhc, lwc));
+        else
+            CD_TRY(cd_net_forward_coarse(ctx->net, ctx->lc_r.as<float>(), stats,
+                                         ctx->lc_y.as<float>(), lhc, lwc));
         if (mid) {
             // coarse -> intermediate grid, smoothing there, -> fine grid
             const int pbm = div_up(bkm.wpb, 256);
@@ -1544,20 +1625,24 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
                 hipLaunchKernelGGL(2 * nsw == BH ? (k_rb_blocked<BH, false, true>) : (k_rb_blocked<BH, false, false>),
                                    dim3(gm), dim3(BK_THREADS), 0, s, em[cur_m],
                                    em[1 - cur_m], rm, bkm, wm, hm, tx, ntm, 2 * nsw, oms / 2.f,
-                                   oms / 3.f, oms / 4.f, ctx->fcg_zero.as<float>(), sd, 0);
+                                   oms / 3.f, oms / 4.f, ctx->fcg_zero.as<float>(), sd, 0,
+                                   MidGrid{});
                 cur_m = 1 - cur_m;
                 done_sw += nsw;
             }
-            hipLaunchKernelGGL(k_prolong_mid, packed_grid, dim3(256), 0, s, em[cur_m], bkm, wm,
-                               hm, fm, w, h, bk, e32);
+            CD_CHECK_LAUNCH(ctx);
+            CD_TRY(timer_end(CD_KERNEL_LEARNED, 3, 0.0));
+            // the fine smoothing starts from em interpolated at the fine cells
+            const MidGrid pro{em[cur_m], bkm, wm, hm, fm};
+            CD_TRY(enqueue_blocked_om(first_sweep, nsmooth, oms / 2.f, oms / 3.f, oms / 4.f, &pro));
         } else {
             hipLaunchKernelGGL(k_prolong_bricked, packed_grid, dim3(256), 0, s,
                                ctx->lc_y.as<float>(), stats + 2, w, h, lf, lwc, lhc, nscale, bk,
                                e32);
+            CD_CHECK_LAUNCH(ctx);
+            CD_TRY(timer_end(CD_KERNEL_LEARNED, 3, 0.0));
+            CD_TRY(enqueue_blocked_om(first_sweep, nsmooth, oms / 2.f, oms / 3.f, oms / 4.f));
         }
-        CD_CHECK_LAUNCH(ctx);
-        CD_TRY(timer_end(CD_KERNEL_LEARNED, 4, 0.0));
-        CD_TRY(enqueue_blocked_om(first_sweep, nsmooth, oms / 2.f, oms / 3.f, oms / 4.f));
         CD_TRY(timer_begin());
         hipLaunchKernelGGL(k_fcg_dots, dot_grid, dim3(256), 0, s, zbuf,
                            ctx->fcg_p.as<float>(), ctx->fcg_q.as<float>(), r16, sd, bk, w, h,
