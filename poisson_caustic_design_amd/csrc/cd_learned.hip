@@ -199,49 +199,62 @@ __global__ __launch_bounds__(256) void k_conv3x3_tile(const u16* __restrict__ x1
     const int wstride = kpad + 8;
     const int cblk = blockIdx.z % cblocks, b = blockIdx.z / cblocks;
     const int tx0 = blockIdx.x * CT, ty0 = blockIdx.y * CT;
-    // stage the input tile: 100 pixels x cin/8 vectors
+    // Stage the input tile, 100 pixels x cin/8 vectors: thread t takes vector
+    // t % ng of pixels t / ng, t / ng + 256 / ng, ... (one division per thread,
+    // none in the loop)
     const int ng = cin >> 3;
-    for (int it = threadIdx.x; it < 100 * ng; it += 256) {
-        const int px = it / ng, gi = it - px * ng;
-        const int yy = ty0 - 1 + px / 10, xx = tx0 - 1 + px % 10;
-        const int ci = gi * 8;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
-            const long long q = ((long long)b * H + yy) * W + xx;
-            if (ci >= c1) {
-                v = *reinterpret_cast<const uint4*>(x2 + q * c2 + (ci - c1));
-            } else if (mode1 == 0) {
-                v = *reinterpret_cast<const uint4*>(x1 + q * c1 + ci);
-            } else if (mode1 == 1) {
-                const long long q1 = ((long long)b * (H >> 1) + (yy >> 1)) * (W >> 1) + (xx >> 1);
-                v = *reinterpret_cast<const uint4*>(x1 + q1 * c1 + ci);
-            } else {
-                // x1 at twice the resolution, 2x2 average pooled as k_avgpool2 does
-                const long long r0 = ((long long)b * (2 * H) + 2 * yy) * (2 * W) + 2 * xx;
-                const long long r1 = r0 + 2 * W;
-                V8 p00, p01, p10, p11, o;
-                p00.u = *reinterpret_cast<const uint4*>(x1 + r0 * c1 + ci);
-                p01.u = *reinterpret_cast<const uint4*>(x1 + (r0 + 1) * c1 + ci);
-                p10.u = *reinterpret_cast<const uint4*>(x1 + r1 * c1 + ci);
-                p11.u = *reinterpret_cast<const uint4*>(x1 + (r1 + 1) * c1 + ci);
+    const int pstride = 256 / ng;
+    const int gi = threadIdx.x % ng;
+    const int ci = gi * 8;
+    if (threadIdx.x < pstride * ng) {
+        for (int px = threadIdx.x / ng; px < 100; px += pstride) {
+            const int yy = ty0 - 1 + px / 10, xx = tx0 - 1 + px % 10;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
+                const long long q = ((long long)b * H + yy) * W + xx;
+                if (ci >= c1) {
+                    v = *reinterpret_cast<const uint4*>(x2 + q * c2 + (ci - c1));
+                } else if (mode1 == 0) {
+                    v = *reinterpret_cast<const uint4*>(x1 + q * c1 + ci);
+                } else if (mode1 == 1) {
+                    const long long q1 =
+                        ((long long)b * (H >> 1) + (yy >> 1)) * (W >> 1) + (xx >> 1);
+                    v = *reinterpret_cast<const uint4*>(x1 + q1 * c1 + ci);
+                } else {
+                    // x1 at twice the resolution, 2x2 average pooled as k_avgpool2 does
+                    const long long r0 = ((long long)b * (2 * H) + 2 * yy) * (2 * W) + 2 * xx;
+                    const long long r1 = r0 + 2 * W;
+                    V8 p00, p01, p10, p11, o;
+                    p00.u = *reinterpret_cast<const uint4*>(x1 + r0 * c1 + ci);
+                    p01.u = *reinterpret_cast<const uint4*>(x1 + (r0 + 1) * c1 + ci);
+                    p10.u = *reinterpret_cast<const uint4*>(x1 + r1 * c1 + ci);
+                    p11.u = *reinterpret_cast<const uint4*>(x1 + (r1 + 1) * c1 + ci);
 #pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const float sm = from_bf16(p00.h[e]) + from_bf16(p01.h[e]) +
-                                     from_bf16(p10.h[e]) + from_bf16(p11.h[e]);
-                    o.h[e] = to_bf16(0.25f * sm);
+                    for (int e = 0; e < 8; ++e) {
+                        const float sm = from_bf16(p00.h[e]) + from_bf16(p01.h[e]) +
+                                         from_bf16(p10.h[e]) + from_bf16(p11.h[e]);
+                        o.h[e] = to_bf16(0.25f * sm);
+                    }
+                    v = o.u;
                 }
-                v = o.u;
             }
+            *reinterpret_cast<uint4*>(xs + px * xstride + ci) = v;
         }
-        *reinterpret_cast<uint4*>(xs + px * xstride + ci) = v;
     }
-    // stage this block's weights: NB*16 rows of kpad
+    // stage this block's weights: NB*16 rows of kpad, vector it = row * kv + cv
+    // for it = threadIdx.x, +256, ... with (row, cv) advanced incrementally
     const int kv = kpad >> 3;
     const u16* wsrc = wt + (size_t)cblk * NB * 16 * kpad;
-    for (int it = threadIdx.x; it < NB * 16 * kv; it += 256) {
-        const int row = it / kv, col = (it - row * kv) * 8;
-        *reinterpret_cast<uint4*>(ws + row * wstride + col) =
-            *reinterpret_cast<const uint4*>(wsrc + (size_t)row * kpad + col);
+    {
+        const int dq = 256 / kv, dr = 256 - dq * kv;
+        int row = threadIdx.x / kv, cv = threadIdx.x - row * kv;
+        while (row < NB * 16) {
+            *reinterpret_cast<uint4*>(ws + row * wstride + cv * 8) =
+                *reinterpret_cast<const uint4*>(wsrc + (size_t)row * kpad + cv * 8);
+            row += dq;
+            cv += dr;
+            if (cv >= kv) { cv -= kv; ++row; }
+        }
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -253,20 +266,30 @@ __global__ __launch_bounds__(256) void k_conv3x3_tile(const u16* __restrict__ x1
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) acc[nb] = floatx4{0.f, 0.f, 0.f, 0.f};
     const u16* wrow = ws + (lane & 15) * wstride + kq;
+    // k = kc + kq = tap * cin + ci, tracked incrementally: cin is a multiple
+    // of 8, so the lane's 8 consecutive k lie in one tap
+    int tap = 0, cc = kq, tdy = 0, tdx = 0;
+    while (cc >= cin) {
+        cc -= cin;
+        ++tap;
+        if (++tdx == 3) { tdx = 0; ++tdy; }
+    }
+    const int pbase = ar * 10 + ac;
     for (int kc = 0; kc < kpad; kc += 32) {
         V8 a;
         a.u = make_uint4(0, 0, 0, 0);
-        const int k = kc + kq;
-        if (k < K) {
-            const int tap = k / cin, ci = k - tap * cin;
-            const int px = (ar + tap / 3) * 10 + ac + tap % 3;
-            a.u = *reinterpret_cast<const uint4*>(xs + px * xstride + ci);
-        }
+        if (kc + kq < K)
+            a.u = *reinterpret_cast<const uint4*>(xs + (pbase + tdy * 10 + tdx) * xstride + cc);
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
             V8 bw;
             bw.u = *reinterpret_cast<const uint4*>(wrow + nb * 16 * wstride + kc);
             acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, bw.v, acc[nb], 0, 0, 0);
+        }
+        cc += 32;
+        while (cc >= cin) {
+            cc -= cin;
+            if (++tdx == 3) { tdx = 0; ++tdy; }
         }
     }
 #pragma unroll
