@@ -415,6 +415,9 @@ def learned_only(a) -> int:
     wpath = a.weights or os.path.join(ROOT, "poisson_caustic_design_amd", "weights",
                                       f"learned_grid{N}.npz")
     ctx = capi.Context(0)
+    # graphs off: each launch of the learned iteration goes to the stream on
+    # its own (a kernel tracer then times every dispatch)
+    ctx.set_graphs(a.graphs == "on")
     cd = capi.CausticDesign(ctx)
     cd.set_mesh_resolution(M, M)
     cd.set_domain_resolution(N, N)
@@ -519,6 +522,8 @@ def main(argv=None):
                          "(calibrates rocprofv3 byte counters); learned: only the learned-"
                          "init designs (profiling); precision: the mixed-precision design "
                          "against FP64 sweeps to 1e-9")
+    ap.add_argument("--graphs", choices=["on", "off"], default="on",
+                    help="--mode learned: replay the learned iteration as a HIP graph")
     ap.add_argument("--learned", choices=["auto", "off"], default="auto",
                     help="auto: when weights for the grid exist, also time the same designs "
                          "with the learned coarse solve and report them beside the plain ones")
