@@ -651,51 +651,86 @@ __device__ __forceinline__ void region_setup(float (&ev)[2][BK_R], const RT (&rr
 // and FP32 under the max-update rule (TRACK), which cannot see the smooth
 // error a rounded right-hand side leaves behind.
 // The intermediate-grid correction em (bricked, wm x hm, fm x fm fine cells
-// per cell) interpolated bilinearly at the centre of fine cell (xx, yy).
+// per cell), which the learned iteration's first fine smoothing launch
+// interpolates bilinearly at the fine cell centres as it loads its region.
 struct MidGrid {
     const float* em;
     Brick bkm;
     int wm, hm, fm;
 };
-__device__ __forceinline__ float prolong_mid_at(const MidGrid& m, int xx, int yy) {
-    const float inv = 1.f / m.fm;
-    const float u = (xx + 0.5f) * inv - 0.5f, v = (yy + 0.5f) * inv - 0.5f;
-    int i0 = (int)floorf(u), j0 = (int)floorf(v);
-    const float tu = u - i0, tv = v - j0;
-    const int i1 = min(max(i0 + 1, 0), m.wm - 1), j1 = min(max(j0 + 1, 0), m.hm - 1);
-    i0 = min(max(i0, 0), m.wm - 1);
-    j0 = min(max(j0, 0), m.hm - 1);
-    const float a = m.em[brick_at(m.bkm, i0, j0)], b = m.em[brick_at(m.bkm, i1, j0)];
-    const float cc = m.em[brick_at(m.bkm, i0, j1)], d = m.em[brick_at(m.bkm, i1, j1)];
-    return (a * (1.f - tu) + b * tu) * (1.f - tv) + (cc * (1.f - tu) + d * tu) * tv;
-}
 
 // Loads of one region: packed column `lane` of the wave's BK_R rows, both
 // colours, from the correction and the right-hand side.  Rows and columns are
 // clamped into the grid (cells outside it are masked by region_setup).
-// PROLONG: the correction is the intermediate-grid correction interpolated at
-// the cells (the start of the learned iteration's smoothing), not ein.
+// PROLONG: the correction is the intermediate-grid correction interpolated
+// bilinearly at the cell centres (u, v) = ((x, y) + 0.5) / fm - 0.5, not ein;
+// the workgroup first stages the intermediate cells its region needs in
+// `patch` (each is read by up to 4 fm^2 fine cells), with the intermediate
+// grid's border clamping.  All waves of the workgroup must call it.
+constexpr int kPatchMax = (BK_RX / 2 + 2) * (BK_RY / 2 + 2);  // fm >= 2
 template <int H, bool PROLONG, typename RT>
 __device__ __forceinline__ void region_load(float (&ev)[2][BK_R], RT (&rr)[2][BK_R],
                                             __amdgpu_buffer_rsrc_t ein_rs,
                                             __amdgpu_buffer_rsrc_t r_rs, const Brick& bk,
-                                            const MidGrid& mid, int tile, int tiles_x, int w,
-                                            int h, int wave, int lane) {
+                                            const MidGrid& mid, float* patch, int tile,
+                                            int tiles_x, int w, int h, int wave, int lane) {
     constexpr int IX = BK_RX - 2 * H, IY = BK_RY - 2 * H;
     const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
-    const int X0 = tx * IX - H, Yw = ty * IY - H + BK_R * wave;
+    const int X0 = tx * IX - H, Y0 = ty * IY - H, Yw = Y0 + BK_R * wave;
     const int j = min(max(X0 / 2 + lane, 0), (w + 1) / 2 - 1);
     const unsigned col = brick_col(j);
+    int pi0[2] = {0, 0}, pj0 = 0, pw = 0;
+    float ptu[2] = {0.f, 0.f};
+    const float inv = PROLONG ? 1.f / mid.fm : 0.f;
+    if (PROLONG) {
+        // intermediate cells I0..I1 x J0..J1 cover the region's (clamped) cells
+        const int xa = min(max(X0, 0), w - 1), xb = min(max(X0 + BK_RX - 1, 0), w - 1);
+        const int ya = min(max(Y0, 0), h - 1), yb = min(max(Y0 + BK_RY - 1, 0), h - 1);
+        const int I0 = (int)floorf((xa + 0.5f) * inv - 0.5f);
+        const int I1 = (int)floorf((xb + 0.5f) * inv - 0.5f) + 1;
+        pj0 = (int)floorf((ya + 0.5f) * inv - 0.5f);
+        const int J1 = (int)floorf((yb + 0.5f) * inv - 0.5f) + 1;
+        pw = I1 - I0 + 1;
+        const int n = pw * (J1 - pj0 + 1);
+        for (int q = threadIdx.x; q < n; q += BK_THREADS) {
+            const int pr = q / pw, pc = q - pr * pw;
+            patch[q] = mid.em[brick_at(mid.bkm, min(max(I0 + pc, 0), mid.wm - 1),
+                                       min(max(pj0 + pr, 0), mid.hm - 1))];
+        }
+        // the lane's two cells 2j + t (t = 0, 1): the same columns in every row
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int xx = min(2 * j + t, w - 1);
+            const float u = (xx + 0.5f) * inv - 0.5f;
+            const int i0 = (int)floorf(u);
+            ptu[t] = u - i0;
+            pi0[t] = i0 - I0;
+        }
+        __syncthreads();
+    }
 #pragma unroll
     for (int k = 0; k < BK_R; ++k) {
         const int Yc = min(max(Yw + k, 0), h - 1);
+        int r0 = 0;
+        float tv = 0.f;
+        if (PROLONG) {
+            const float v = (Yc + 0.5f) * inv - 0.5f;
+            const int j0 = (int)floorf(v);
+            tv = v - j0;
+            r0 = (j0 - pj0) * pw;
+        }
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
             const unsigned row = (unsigned)brick_row(bk, c, Yc);
-            if (PROLONG)
-                ev[c][k] = prolong_mid_at(mid, min(2 * j + ((c + Yc) & 1), w - 1), Yc);
-            else
+            if (PROLONG) {
+                const int t = (c + Yc) & 1;
+                const float tu = ptu[t];
+                const float* pp = patch + r0 + pi0[t];
+                const float a = pp[0], b = pp[1], cc = pp[pw], d = pp[pw + 1];
+                ev[c][k] = (a * (1.f - tu) + b * tu) * (1.f - tv) + (cc * (1.f - tu) + d * tu) * tv;
+            } else {
                 ev[c][k] = buf_load<float>(ein_rs, col * 4u, row * 4u);
+            }
             rr[c][k] = buf_load<RT>(r_rs, col * (unsigned)sizeof(RT), row * (unsigned)sizeof(RT));
         }
     }
@@ -779,7 +814,9 @@ __global__ __launch_bounds__(BK_THREADS) void k_rb_blocked(const float* __restri
     const __amdgpu_buffer_rsrc_t eout_rs = plane_rsrc(eout, npk * sizeof(float));
     float ev[2][BK_R];
     RT rr[2][BK_R];
-    region_load<H, PROLONG>(ev, rr, ein_rs, r_rs, bk, mid, tile, tiles_x, w, h, wave, lane);
+    __shared__ float patch[PROLONG ? kPatchMax : 1];
+    region_load<H, PROLONG>(ev, rr, ein_rs, r_rs, bk, mid, patch, tile, tiles_x, w, h, wave,
+                            lane);
     // Max-update rule: the stop flag can be set between the launches of one
     // batch (k_maxcheck_blocked), and a finished solve leaves the buffers
     // untouched.  Residual rule: only a check sets it, and no sweeps are
