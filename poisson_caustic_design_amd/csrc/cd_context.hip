@@ -186,6 +186,7 @@ void cd_context_destroy(cd_context* ctx) {
     ctx->e32b.release();
     ctx->r32.release();
     ctx->phi_start.release();
+    ctx->fcg_phi.release();
     ctx->e64w.release();
     ctx->phi64w.release();
     ctx->b64w.release();

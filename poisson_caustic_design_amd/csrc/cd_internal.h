@@ -130,8 +130,9 @@ struct cd_context {
     // learned-init network (owned by the caller) and its solver workspace
     cd_net* net = nullptr;
     DevBuf lc_r, lc_x, lc_y, lc_stats;
-    DevBuf fcg_p, fcg_q, fcg_dev;  // learned FCG: direction, L direction, coefficients
+    DevBuf fcg_p, fcg_q, fcg_dev;  // learned FCG: direction (ping-pong pair), coefficients
     DevBuf phi_start;              // learned FCG: the warm start (FP64)
+    DevBuf fcg_phi;                // learned FCG: second buffer of the FP64 iterate
     // learned FCG intermediate grid: correction ping-pong, BF16 residual, a
     // device 0.0f (the mean the intermediate sweeps subtract)
     DevBuf fcg_em, fcg_emb, fcg_rm, fcg_zero;

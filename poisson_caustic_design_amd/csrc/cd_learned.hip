@@ -516,28 +516,46 @@ __global__ __launch_bounds__(256) void k_smooth_step(const float* __restrict__ i
 // has element stride `istride` (8: head channel 0).
 constexpr int kSmoothTile = 16;  // small tiles: enough workgroups at 128x128
 constexpr int kSmoothMaxK = 12;  // halo <= 12: 2 (16 + 2k)^2 floats of LDS
+// Each thread keeps the region coordinates of its (at most kSmoothCells)
+// cells for every step, and issues the loads of all of them before the first
+// LDS store.
+constexpr int kSmoothCells =
+    ((kSmoothTile + 2 * kSmoothMaxK) * (kSmoothTile + 2 * kSmoothMaxK) + 255) / 256;
 __global__ __launch_bounds__(256) void k_smooth_tiled(const float* __restrict__ in, int istride,
                                                       float* __restrict__ out, int B, int H,
                                                       int W, int k) {
     extern __shared__ float lds[];
-    const int R = kSmoothTile + 2 * k;
+    const int R = kSmoothTile + 2 * k, RR = R * R;
     float* a = lds;
-    float* b = lds + R * R;
+    float* b = lds + RR;
     const int bz = blockIdx.z;
     const int X0 = blockIdx.x * kSmoothTile - k, Y0 = blockIdx.y * kSmoothTile - k;
     const long long base = (long long)bz * H * W;
-    for (int idx = threadIdx.x; idx < R * R; idx += blockDim.x) {
-        const int ry = idx / R, rx = idx - ry * R;
-        const int gx = X0 + rx, gy = Y0 + ry;
-        a[idx] = (gx >= 0 && gx < W && gy >= 0 && gy < H) ? in[(base + (long long)gy * W + gx) * istride]
-                                                          : 0.f;
+    int rxs[kSmoothCells], rys[kSmoothCells];
+    float v[kSmoothCells];
+#pragma unroll
+    for (int u = 0; u < kSmoothCells; ++u) {
+        const int idx = threadIdx.x + 256 * u;
+        rys[u] = idx / R;
+        rxs[u] = idx - rys[u] * R;
+        const int gx = X0 + rxs[u], gy = Y0 + rys[u];
+        v[u] = (idx < RR && gx >= 0 && gx < W && gy >= 0 && gy < H)
+                   ? in[(base + (long long)gy * W + gx) * istride]
+                   : 0.f;
     }
+#pragma unroll
+    for (int u = 0; u < kSmoothCells; ++u)
+        if (threadIdx.x + 256 * u < RR) a[threadIdx.x + 256 * u] = v[u];
     __syncthreads();
     for (int st = 1; st <= k; ++st) {
-        for (int idx = threadIdx.x; idx < R * R; idx += blockDim.x) {
-            const int ry = idx / R, rx = idx - ry * R;
+#pragma unroll
+        for (int u = 0; u < kSmoothCells; ++u) {
+            const int idx = threadIdx.x + 256 * u;
+            if (256 * u >= RR) break;  // no thread has cells from here on
+            if (idx >= RR) continue;
+            const int rx = rxs[u], ry = rys[u];
             const int gx = X0 + rx, gy = Y0 + ry;
-            float v = a[idx];
+            float c = a[idx];
             if (rx >= st && rx < R - st && ry >= st && ry < R - st && gx >= 0 && gx < W &&
                 gy >= 0 && gy < H) {
                 float sum = 0.f, cnt = 0.f;
@@ -545,9 +563,9 @@ __global__ __launch_bounds__(256) void k_smooth_tiled(const float* __restrict__ 
                 if (gx < W - 1) { sum += a[idx + 1]; cnt += 1.f; }
                 if (gy > 0) { sum += a[idx - R]; cnt += 1.f; }
                 if (gy < H - 1) { sum += a[idx + R]; cnt += 1.f; }
-                v = v + 0.125f * (sum - cnt * v);
+                c = c + 0.125f * (sum - cnt * c);
             }
-            b[idx] = v;
+            b[idx] = c;
         }
         __syncthreads();
         float* t = a;

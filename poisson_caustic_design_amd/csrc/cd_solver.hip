@@ -208,59 +208,105 @@ __global__ __launch_bounds__(256) void k_flush_packed(double* __restrict__ phi,
 // ---------------------------------------------------------------------------
 // OUT: 0 norms only, 1 FP32 copy of r (natural layout), 2 BF16 copy of r
 // (bricked layout, blocked sweeps), 3 FP32 copy of r in the bricked layout
-// (blocked sweeps under the max-update rule).  A thread takes `rows` rows (stride
+// (blocked sweeps under the max-update rule).  A thread takes ROWS rows (stride
 // blockDim.y) of its column before the two block reductions, so a launch
-// reduces over few workgroups.
-template <int OUT>
+// reduces over few workgroups.  The loads of 4 rows at a time are issued
+// before the first is used (clamped, always in-range addresses; cells outside
+// the grid contribute nothing): a thread keeps 24 loads in flight instead of
+// waiting for each row in turn.
+template <int OUT, int ROWS, bool MASK>
+__device__ __forceinline__ void residual_rows(const double* __restrict__ phi,
+                                              const double* __restrict__ b,
+                                              const uint8_t* __restrict__ mask, double bmean,
+                                              float* __restrict__ r32,
+                                              unsigned short* __restrict__ r16, const Brick& bk,
+                                              int w, int h, double& rs, double& rq) {
+    constexpr int G = ROWS < 4 ? ROWS : 4;  // rows whose loads are in flight together
+    const int x0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y0 = blockIdx.y * blockDim.y * ROWS + threadIdx.y;
+    const int x = min(x0, w - 1);
+    const int xl = max(x - 1, 0), xr = min(x + 1, w - 1);
+#pragma unroll
+    for (int g = 0; g < ROWS; g += G) {
+        double c[G], l[G], r[G], u[G], d[G], bb[G];
+        uint8_t m[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const int y = min(y0 + (g + k) * (int)blockDim.y, h - 1);
+            const size_t row = (size_t)y * w;
+            c[k] = phi[row + x];
+            l[k] = phi[row + xl];
+            r[k] = phi[row + xr];
+            u[k] = phi[(size_t)max(y - 1, 0) * w + x];
+            d[k] = phi[(size_t)min(y + 1, h - 1) * w + x];
+            bb[k] = b[row + x];
+            if (MASK) m[k] = mask[row + x];
+        }
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const int y = y0 + (g + k) * (int)blockDim.y;
+            if (x0 >= w || y >= h) break;
+            double s = 0.0, cnt = 0.0;
+            bool hole = false;
+            if (MASK) {
+                hole = (m[k] & M_HOLE) != 0;
+                if (m[k] & M_L) { cnt += 1.0; s += l[k]; }
+                if (m[k] & M_U) { cnt += 1.0; s += u[k]; }
+                if (m[k] & M_R) { cnt += 1.0; s += r[k]; }
+                if (m[k] & M_D) { cnt += 1.0; s += d[k]; }
+            } else {
+                if (x > 0) { cnt += 1.0; s += l[k]; }
+                if (y > 0) { cnt += 1.0; s += u[k]; }
+                if (x < w - 1) { cnt += 1.0; s += r[k]; }
+                if (y < h - 1) { cnt += 1.0; s += d[k]; }
+            }
+            double rv = 0.0;
+            if (!hole) {
+                rv = (bb[k] - bmean) - (s - cnt * c[k]);
+                rs += rv;
+                rq += rv * rv;
+            }
+            const size_t i = (size_t)y * w + x;
+            if (OUT == 1) r32[i] = (float)rv;
+            if (OUT == 2)
+                r16[brick_row(bk, (x + y) & 1, y) + brick_col(x >> 1)] = f32_to_bf16((float)rv);
+            if (OUT == 3) r32[brick_row(bk, (x + y) & 1, y) + brick_col(x >> 1)] = (float)rv;
+        }
+    }
+}
+template <int OUT, int ROWS>
 __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ phi,
                                                   const double* __restrict__ b,
                                                   const uint8_t* __restrict__ mask,
                                                   const SolveDev* __restrict__ st,
                                                   float* __restrict__ r32,
                                                   unsigned short* __restrict__ r16, Brick bk,
-                                                  int w, int h, int rows,
-                                                  double* __restrict__ partials) {
+                                                  int w, int h, double* __restrict__ partials) {
     __shared__ double smem[16];
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y0 = blockIdx.y * blockDim.y * rows + threadIdx.y;
     const double bmean = st->bmean;
     double rs = 0.0, rq = 0.0;
-    for (int k = 0; k < rows; ++k) {
-        const int y = y0 + k * blockDim.y;
-        if (x >= w || y >= h) break;
-        size_t i = (size_t)y * w + x;
-        double c = phi[i];
-        double s = 0.0, cnt = 0.0;
-        bool hole = false;
-        if (mask) {
-            uint8_t m = mask[i];
-            hole = (m & M_HOLE) != 0;
-            if (m & M_L) { cnt += 1.0; s += phi[i - 1]; }
-            if (m & M_U) { cnt += 1.0; s += phi[i - w]; }
-            if (m & M_R) { cnt += 1.0; s += phi[i + 1]; }
-            if (m & M_D) { cnt += 1.0; s += phi[i + w]; }
-        } else {
-            if (x > 0) { cnt += 1.0; s += phi[i - 1]; }
-            if (y > 0) { cnt += 1.0; s += phi[i - w]; }
-            if (x < w - 1) { cnt += 1.0; s += phi[i + 1]; }
-            if (y < h - 1) { cnt += 1.0; s += phi[i + w]; }
-        }
-        double r = 0.0;
-        if (!hole) {
-            r = (b[i] - bmean) - (s - cnt * c);
-            rs += r;
-            rq += r * r;
-        }
-        if (OUT == 1) r32[i] = (float)r;
-        if (OUT == 2) r16[brick_row(bk, (x + y) & 1, y) + brick_col(x >> 1)] = f32_to_bf16((float)r);
-        if (OUT == 3) r32[brick_row(bk, (x + y) & 1, y) + brick_col(x >> 1)] = (float)r;
-    }
+    if (mask)
+        residual_rows<OUT, ROWS, true>(phi, b, mask, bmean, r32, r16, bk, w, h, rs, rq);
+    else
+        residual_rows<OUT, ROWS, false>(phi, b, mask, bmean, r32, r16, bk, w, h, rs, rq);
     double a = cd_block_sum(rs, smem);
     double q = cd_block_sum(rq, smem);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
         size_t bid = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
         partials[2 * bid] = a;
         partials[2 * bid + 1] = q;
+    }
+}
+// launch of k_residual<OUT, rows> for rows in {1, 2, 4, 8}
+template <int OUT>
+void launch_residual(int rows, dim3 grid, dim3 blk, hipStream_t s, const double* phi,
+                     const double* b, const uint8_t* mask, const SolveDev* st, float* r32,
+                     unsigned short* r16, const Brick& bk, int w, int h, double* partials) {
+    switch (rows) {
+    case 8: hipLaunchKernelGGL((k_residual<OUT, 8>), grid, blk, 0, s, phi, b, mask, st, r32, r16, bk, w, h, partials); break;
+    case 4: hipLaunchKernelGGL((k_residual<OUT, 4>), grid, blk, 0, s, phi, b, mask, st, r32, r16, bk, w, h, partials); break;
+    case 2: hipLaunchKernelGGL((k_residual<OUT, 2>), grid, blk, 0, s, phi, b, mask, st, r32, r16, bk, w, h, partials); break;
+    default: hipLaunchKernelGGL((k_residual<OUT, 1>), grid, blk, 0, s, phi, b, mask, st, r32, r16, bk, w, h, partials); break;
     }
 }
 
@@ -930,10 +976,14 @@ int auto_check_every(int w, int h) {
 // sweeps see (r16 - rmean32).  f < 8: one thread per coarse cell; larger f:
 // one wave per coarse cell (lanes stride over its f^2 fine cells, fixed-order
 // shuffle tree).  Statistics of rc follow in k_coarse_stats.
+// x8 (may be NULL): rc also written as the input of a linear network, BF16
+// channel 0 of 8 (a linear bias-free network commutes with the scaling that
+// normalises its input, so that input needs no statistics)
 __global__ __launch_bounds__(256) void k_restrict_r16(const unsigned short* __restrict__ r16,
                                                       Brick bk, const SolveDev* __restrict__ st,
                                                       int f, int wc, int hc,
-                                                      float* __restrict__ rc) {
+                                                      float* __restrict__ rc,
+                                                      unsigned short* __restrict__ x8) {
     const float rm = st->rmean32;
     const int ff = f * f;
     if (f < 8) {
@@ -945,6 +995,8 @@ __global__ __launch_bounds__(256) void k_restrict_r16(const unsigned short* __re
             for (int xx = I * f; xx < I * f + f; ++xx)
                 acc += bf16_to_f32(r16[brick_at(bk, xx, yy)]) - rm;
         rc[i] = acc / (float)ff;
+        if (x8) *reinterpret_cast<uint4*>(x8 + (size_t)i * 8) =
+            make_uint4(f32_to_bf16(acc / (float)ff), 0, 0, 0);
         return;
     }
     const int lane = threadIdx.x & 63;
@@ -958,7 +1010,11 @@ __global__ __launch_bounds__(256) void k_restrict_r16(const unsigned short* __re
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-    if (lane == 0) rc[i] = acc / (float)ff;
+    if (lane == 0) {
+        rc[i] = acc / (float)ff;
+        if (x8) *reinterpret_cast<uint4*>(x8 + (size_t)i * 8) =
+            make_uint4(f32_to_bf16(acc / (float)ff), 0, 0, 0);
+    }
 }
 
 // (mean, rms) of nc coarse values from stats = (sum, sum^2), as every consumer
@@ -1121,20 +1177,19 @@ struct FcgDev {
     double pq;     // p . Lp of the current direction
     int have;      // a previous direction exists
     int bad;       // the last step had no descent direction
-    double* phi;          // FP64 iterate of the current solve (the captured
-                          // iteration body reads it here: the graph serves
-                          // every solve of the grid size)
+    const float* pprev;  // the previous direction (one of two ping-pong
+                         // buffers; the captured iteration body reads it here)
 };
 
-// start of the learned iterations of a solve
-__global__ void k_fcg_init(FcgDev* fd, double* phi) {
+// start of the learned iterations of a solve; p0: the zeroed first direction
+__global__ void k_fcg_init(FcgDev* fd, const float* p0) {
     if (threadIdx.x != 0) return;
     fd->beta = 0.0;
     fd->alpha = 0.0;
     fd->pq = 0.0;
     fd->have = 0;
     fd->bad = 0;
-    fd->phi = phi;
+    fd->pprev = p0;
 }
 
 // Lz at packed element (c, Y, j) (cell x): the four neighbours lie in the
@@ -1150,28 +1205,60 @@ __device__ __forceinline__ float lap_packed(const float* __restrict__ z, const B
     return s - cnt * zc;
 }
 
-// partials[4 * block + k]: z.q_old, z.r, p_old.r, z.Lz over the packed
-// elements of the rows the workgroup strides through (fixed order)
+// partials[4 * block + k]: p_old.Lz (= z.Lp_old: L is symmetric, so Lp_old
+// needs neither a stencil pass nor storage), z.r, p_old.r, z.Lz over the
+// packed elements of the rows the workgroup strides through (fixed order).
+// Rows go in groups of 4 whose loads are all issued before the first is used
+// (clamped, in-range addresses; elements outside the grid contribute nothing).
 __global__ __launch_bounds__(256) void k_fcg_dots(const float* __restrict__ z,
-                                                  const float* __restrict__ p,
-                                                  const float* __restrict__ q,
+                                                  const FcgDev* __restrict__ fd,
                                                   const unsigned short* __restrict__ r16,
                                                   const SolveDev* __restrict__ st, Brick bk,
                                                   int w, int h, double* __restrict__ partials) {
+    constexpr int G = 4;
     __shared__ double smem[16];
     const float rm = st->rmean32;
+    const float* __restrict__ p = fd->pprev;
+    const int c = blockIdx.z, o = 1 - c;
+    const int j0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = min(j0, bk.wpb - 1);
+    const unsigned cj = brick_col(j), cjm = brick_col(max(j - 1, 0)),
+                   cjp = brick_col(min(j + 1, bk.wpb - 1));
     double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
-    for (int Y = blockIdx.y; Y < h; Y += gridDim.y) {
-        int c, j, x;
-        if (!packed_thread(bk, w, Y, c, j, x)) continue;
-        const size_t i = brick_row(bk, c, Y) + brick_col(j);
-        const float zc = z[i];
-        const float lz = lap_packed(z, bk, c, Y, j, x, w, h, zc);
-        const float rr = bf16_to_f32(r16[i]) - rm;
-        d0 += (double)(zc * q[i]);
-        d1 += (double)(zc * rr);
-        d2 += (double)(p[i] * rr);
-        d3 += (double)(zc * lz);
+    for (int Yb = blockIdx.y; Yb < h; Yb += G * gridDim.y) {
+        float zc[G], zl[G], zr[G], zu[G], zd[G], pv[G];
+        unsigned short rv[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const int Y = min(Yb + k * (int)gridDim.y, h - 1);
+            const int par = (c + Y) & 1;  // x = 2j + par
+            const size_t ro = brick_row(bk, o, Y);
+            const size_t i = brick_row(bk, c, Y) + cj;
+            zc[k] = z[i];
+            zl[k] = z[ro + (par ? cj : cjm)];   // (x - 1) >> 1
+            zr[k] = z[ro + (par ? cjp : cj)];   // (x + 1) >> 1
+            zu[k] = z[brick_row(bk, o, max(Y - 1, 0)) + cj];
+            zd[k] = z[brick_row(bk, o, min(Y + 1, h - 1)) + cj];
+            pv[k] = p[i];
+            rv[k] = r16[i];
+        }
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const int Y = Yb + k * (int)gridDim.y;
+            const int x = 2 * j0 + ((c + Y) & 1);
+            if (Y >= h || j0 >= bk.wpb || x >= w) continue;
+            float sn = 0.f, cnt = 0.f;
+            if (x > 0) { sn += zl[k]; cnt += 1.f; }
+            if (x < w - 1) { sn += zr[k]; cnt += 1.f; }
+            if (Y > 0) { sn += zu[k]; cnt += 1.f; }
+            if (Y < h - 1) { sn += zd[k]; cnt += 1.f; }
+            const float lz = sn - cnt * zc[k];
+            const float rr = bf16_to_f32(rv[k]) - rm;
+            d0 += (double)(pv[k] * lz);
+            d1 += (double)(zc[k] * rr);
+            d2 += (double)(pv[k] * rr);
+            d3 += (double)(zc[k] * lz);
+        }
     }
     d0 = cd_block_sum(d0, smem);
     d1 = cd_block_sum(d1, smem);
@@ -1194,7 +1281,7 @@ __global__ void k_fcg_coef(const double* __restrict__ partials, int nblocks, Fcg
         for (int k = 0; k < 4; ++k) d[k] += partials[4 * b + k];
     for (int k = 0; k < 4; ++k) d[k] = cd_block_sum(d[k], smem);
     if (threadIdx.x == 0) {
-        const double zq = d[0], zr = d[1], pr = d[2], zlz = d[3];
+        const double zq = d[0], zr = d[1], pr = d[2], zlz = d[3];  // zq = z.Lp_old
         double beta = 0.0;
         if (fd->have && fd->pq < 0.0) beta = -zq / fd->pq;
         double pq = zlz + 2.0 * beta * zq + beta * beta * (fd->have ? fd->pq : 0.0);
@@ -1216,31 +1303,103 @@ __global__ void k_fcg_coef(const double* __restrict__ partials, int nblocks, Fcg
     }
 }
 
-// p = z + beta p; q = Lz + beta q; phi64 += alpha p (the step, folded into
-// the FP64 iterate at once: the FP32 corrections need no flush).  One thread
-// per packed column j of row Y, both colours: cells 2j and 2j+1, so that the
-// FP64 read-modify-write of phi is contiguous across the wave.
-__global__ __launch_bounds__(256) void k_fcg_update(const float* __restrict__ z,
-                                                    float* __restrict__ p, float* __restrict__ q,
-                                                    const FcgDev* __restrict__ fd, Brick bk,
-                                                    int w, int h) {
-    const int Y = blockIdx.y;
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= bk.wpb) return;
-    double* __restrict__ phi = fd->phi;
-    const float beta = (float)fd->beta, alpha = (float)fd->alpha;
+// The learned iteration's FCG step folded into its convergence check:
+// p = z + beta p_old (z: the smoothed correction, packed), phi_out = phi_in +
+// alpha p, and the FP64 residual of phi_out with its BF16 copy and partial
+// norms (as k_residual<2>).  phi and p are ping-pong pairs: the stencil reads
+// phi_in and p_old at the four neighbours and forms their new values itself
+// (the same FP32 and FP64 operations as at the centre, so every cell's new
+// value is one number).  fd->pprev is pointed at p_new for the next iteration.
+template <int ROWS>
+__global__ __launch_bounds__(256) void k_residual_step(const double* __restrict__ phi_in,
+                                                       double* __restrict__ phi_out,
+                                                       const double* __restrict__ b,
+                                                       const SolveDev* __restrict__ st,
+                                                       const float* __restrict__ z,
+                                                       const float* __restrict__ p_old,
+                                                       float* __restrict__ p_new,
+                                                       FcgDev* __restrict__ fd,
+                                                       unsigned short* __restrict__ r16, Brick bk,
+                                                       int w, int h, double* __restrict__ partials) {
+    constexpr int G = ROWS < 2 ? ROWS : 2;  // rows whose loads are in flight together
+    __shared__ double smem[16];
+    const double bmean = st->bmean;
+    const float alpha = (float)fd->alpha, beta = (float)fd->beta;
+    const int x0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y0 = blockIdx.y * blockDim.y * ROWS + threadIdx.y;
+    const int x = min(x0, w - 1);
+    const int xl = max(x - 1, 0), xr = min(x + 1, w - 1);
+    // new direction and iterate at (xx, yy) from the loaded phi_in, z, p_old
+    auto pnew = [&](float zv, float pv) { return zv + beta * pv; };
+    auto phinew = [&](double ph, float pn) { return ph + (double)(alpha * pn); };
+    double rs = 0.0, rq = 0.0;
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
-        const int x = 2 * j + t;
-        if (x >= w) break;
-        const int c = (x + Y) & 1;
-        const size_t i = brick_row(bk, c, Y) + brick_col(j);
-        const float zc = z[i];
-        const float lz = lap_packed(z, bk, c, Y, j, x, w, h, zc);
-        const float pn = zc + beta * p[i];
-        p[i] = pn;
-        q[i] = lz + beta * q[i];
-        phi[(size_t)Y * w + x] += (double)(alpha * pn);
+    for (int g = 0; g < ROWS; g += G) {
+        double ph[G][5], bb[G];
+        float zv[G][5], pv[G][5];
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const int y = min(y0 + (g + k) * (int)blockDim.y, h - 1);
+            const int yu = max(y - 1, 0), yd = min(y + 1, h - 1);
+            const int xs[5] = {x, xl, x, xr, x};
+            const int ys[5] = {y, y, yu, y, yd};
+#pragma unroll
+            for (int t = 0; t < 5; ++t) {
+                ph[k][t] = phi_in[(size_t)ys[t] * w + xs[t]];
+                const size_t bi = brick_at(bk, xs[t], ys[t]);
+                zv[k][t] = z[bi];
+                pv[k][t] = p_old[bi];
+            }
+            bb[k] = b[(size_t)y * w + x];
+        }
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const int y = y0 + (g + k) * (int)blockDim.y;
+            if (x0 >= w || y >= h) break;
+            double nv[5];
+            float pc = 0.f;
+#pragma unroll
+            for (int t = 0; t < 5; ++t) {
+                const float pn = pnew(zv[k][t], pv[k][t]);
+                if (t == 0) pc = pn;
+                nv[t] = phinew(ph[k][t], pn);
+            }
+            // neighbours in the order left, up, right, down
+            double s = 0.0, cnt = 0.0;
+            if (x > 0) { cnt += 1.0; s += nv[1]; }
+            if (y > 0) { cnt += 1.0; s += nv[2]; }
+            if (x < w - 1) { cnt += 1.0; s += nv[3]; }
+            if (y < h - 1) { cnt += 1.0; s += nv[4]; }
+            const double rv = (bb[k] - bmean) - (s - cnt * nv[0]);
+            rs += rv;
+            rq += rv * rv;
+            const size_t i = (size_t)y * w + x;
+            const size_t bi = brick_at(bk, x, y);
+            phi_out[i] = nv[0];
+            p_new[bi] = pc;
+            r16[bi] = f32_to_bf16((float)rv);
+        }
+    }
+    const double a = cd_block_sum(rs, smem);
+    const double q = cd_block_sum(rq, smem);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        const size_t bid = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+        partials[2 * bid] = a;
+        partials[2 * bid + 1] = q;
+        if (bid == 0) fd->pprev = p_new;
+    }
+}
+
+// launch of k_residual_step<rows> for rows in {1, 2, 4, 8}
+void launch_residual_step(int rows, dim3 grid, dim3 blk, hipStream_t s, const double* phi_in,
+                          double* phi_out, const double* b, const SolveDev* st, const float* z,
+                          const float* p_old, float* p_new, FcgDev* fd, unsigned short* r16,
+                          const Brick& bk, int w, int h, double* partials) {
+    switch (rows) {
+    case 8: hipLaunchKernelGGL(k_residual_step<8>, grid, blk, 0, s, phi_in, phi_out, b, st, z, p_old, p_new, fd, r16, bk, w, h, partials); break;
+    case 4: hipLaunchKernelGGL(k_residual_step<4>, grid, blk, 0, s, phi_in, phi_out, b, st, z, p_old, p_new, fd, r16, bk, w, h, partials); break;
+    case 2: hipLaunchKernelGGL(k_residual_step<2>, grid, blk, 0, s, phi_in, phi_out, b, st, z, p_old, p_new, fd, r16, bk, w, h, partials); break;
+    default: hipLaunchKernelGGL(k_residual_step<1>, grid, blk, 0, s, phi_in, phi_out, b, st, z, p_old, p_new, fd, r16, bk, w, h, partials); break;
     }
 }
 
@@ -1370,36 +1529,19 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     const int fblocks = std::min(div_up((long long)n, CD_BLOCK), 65536);
     // residual launch of a check: kResidualRows rows per thread, fewer on
     // smaller grids (more workgroups in flight; the partials stay <= nred)
-    const int kResidualRows = std::max(1, std::min(8, nred / 2048));
+    const int kResidualRows = nred >= 16384 ? 8 : nred >= 8192 ? 4 : nred >= 4096 ? 2 : 1;
     const dim3 cgrid(div_up(w, RED_BX), div_up(h, RED_BY * kResidualRows));
     const int ncheck = cgrid.x * cgrid.y;  // <= nred
     // flush: fold the current FP32 correction into phi64 first (the learned
     // iterations add theirs to phi64 themselves)
-    auto enqueue_check = [&](int check_id, bool flush = true) -> int {
-        CD_TRY(timer_begin());
-        if (blocked) {
-            if (flush)
-                hipLaunchKernelGGL(k_flush_packed, rgrid, rblk, 0, s, phi, e32, e32b, sd, w, h, bk);
-            if (track)
-                hipLaunchKernelGGL((k_residual<3>), cgrid, rblk, 0, s, phi, b, mask, sd, r32, r16,
-                                   bk, w, h, kResidualRows, partials);
-            else
-                hipLaunchKernelGGL((k_residual<2>), cgrid, rblk, 0, s, phi, b, mask, sd, r32, r16,
-                                   bk, w, h, kResidualRows, partials);
-        } else if (mixed) {
-            hipLaunchKernelGGL(k_flush, dim3(fblocks), dim3(CD_BLOCK), 0, s, phi, e32, n);
-            hipLaunchKernelGGL((k_residual<1>), cgrid, rblk, 0, s, phi, b, mask, sd, r32, r16, bk,
-                               w, h, kResidualRows, partials);
-        } else {
-            hipLaunchKernelGGL((k_residual<0>), cgrid, rblk, 0, s, phi, b, mask, sd, r32, r16, bk,
-                               w, h, kResidualRows, partials);
-        }
+    // the end of every check: k_check on the residual pass's partials, then the
+    // host's wait for the status (the residual pass's timer is open)
+    auto finish_check = [&](int check_id, int timer_kernels, double bytes) -> int {
         const unsigned seq = ++ctx->status_seq;
         hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, partials, ncheck, sd,
                            ctx->host_status, seq, count, check_id, o->stop, o->tol);
         CD_CHECK_LAUNCH(ctx);
-        CD_TRY(timer_end(CD_KERNEL_RESIDUAL, (mixed && flush) ? 3 : 2,
-                              (blocked ? (flush ? 48.0 : 18.0) : (mixed ? 44.0 : 16.0)) * (double)n));
+        CD_TRY(timer_end(CD_KERNEL_RESIDUAL, timer_kernels, bytes));
         if (ctx->profiling) {
             CD_HIP(ctx, hipStreamSynchronize(s));
             CD_TRY(ctx->resolve_timers());
@@ -1407,6 +1549,28 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
             CD_TRY(cd_wait_status(ctx, seq));
         }
         return CD_OK;
+    };
+    auto enqueue_check = [&](int check_id, bool flush = true) -> int {
+        CD_TRY(timer_begin());
+        if (blocked) {
+            if (flush)
+                hipLaunchKernelGGL(k_flush_packed, rgrid, rblk, 0, s, phi, e32, e32b, sd, w, h, bk);
+            if (track)
+                launch_residual<3>(kResidualRows, cgrid, rblk, s, phi, b, mask, sd, r32, r16, bk,
+                                   w, h, partials);
+            else
+                launch_residual<2>(kResidualRows, cgrid, rblk, s, phi, b, mask, sd, r32, r16, bk,
+                                   w, h, partials);
+        } else if (mixed) {
+            hipLaunchKernelGGL(k_flush, dim3(fblocks), dim3(CD_BLOCK), 0, s, phi, e32, n);
+            launch_residual<1>(kResidualRows, cgrid, rblk, s, phi, b, mask, sd, r32, r16, bk, w, h,
+                               partials);
+        } else {
+            launch_residual<0>(kResidualRows, cgrid, rblk, s, phi, b, mask, sd, r32, r16, bk, w, h,
+                               partials);
+        }
+        return finish_check(check_id, (mixed && flush) ? 3 : 2,
+                            (blocked ? (flush ? 48.0 : 18.0) : (mixed ? 44.0 : 16.0)) * (double)n);
     };
 
     const dim3 sblk(SWEEP_BX, SWEEP_BY);
@@ -1509,6 +1673,7 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         CD_TRY(ctx->ensure(ctx->fcg_p, sizeof(float) * npk));
         CD_TRY(ctx->ensure(ctx->fcg_q, sizeof(float) * npk));
         CD_TRY(ctx->ensure(ctx->fcg_dev, sizeof(FcgDev)));
+        CD_TRY(ctx->ensure(ctx->fcg_phi, sizeof(double) * n));
     }
     // intermediate grid of the preconditioner (coarsening a multiple of 4, >= 8)
     const int mid_sweeps = arch ? arch->mid_sweeps : 0;
@@ -1537,14 +1702,21 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     // restrict the residual of the last check to (wc x hc) into lc_r; stats
     // slot receives (sum, sum^2) of the coarse residual
     // x8 (may be NULL): also write the network input there (coarse grids of up
-    // to 64Ki cells; *x8_done tells whether it was)
+    // to 64Ki cells; *x8_done tells whether it was).  direct: x8 receives the
+    // coarse residual itself (the input of a linear network) and no statistics
+    // are computed
     auto restrict_stats = [&](int f, int wc, int hc, double* stats, double* derived,
-                              unsigned short* x8 = nullptr, bool* x8_done = nullptr) -> int {
+                              unsigned short* x8 = nullptr, bool* x8_done = nullptr,
+                              bool direct = false) -> int {
         const int nc = wc * hc;
         const int rb = f < 8 ? div_up(nc, 256) : div_up(nc, 4);  // thread / wave per cell
         hipLaunchKernelGGL(k_restrict_r16, dim3(rb), dim3(256), 0, s, r16, bk, sd, f, wc, hc,
-                           ctx->lc_r.as<float>());
-        if (x8_done) *x8_done = false;
+                           ctx->lc_r.as<float>(), direct ? x8 : nullptr);
+        if (x8_done) *x8_done = direct && x8;
+        if (direct && x8) {
+            CD_CHECK_LAUNCH(ctx);
+            return CD_OK;
+        }
         if (nc <= 65536) {
             hipLaunchKernelGGL(k_coarse_stats1, dim3(1), dim3(1024), 0, s, ctx->lc_r.as<float>(),
                                nc, stats, derived, x8);
@@ -1570,6 +1742,9 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         CD_CHECK_LAUNCH(ctx);
         return CD_OK;
     };
+    // the FP64 iterate: phi, or during learned iterations either buffer of the
+    // pair (phi, fcg_phi) that the folded FCG steps alternate between
+    double* phi_cur = phi;
     auto capture_pair = [&]() -> int {
         if (ctx->cap_count >= ctx->cap_capacity || (int)cap_slots.size() >= kMaxPairsPerSolve)
             return CD_OK;
@@ -1578,7 +1753,7 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         const int nc = cwc * chc;
         double* stats = ctx->lc_stats.as<double>() + 2 * (2 + i);
         CD_TRY(restrict_normalize(ctx->cap_coarsen, cwc, chc, ctx->cap_x + (size_t)slot * nc, stats));
-        hipLaunchKernelGGL(k_restrict_phi, dim3(div_up(nc, 256)), dim3(256), 0, s, phi, w,
+        hipLaunchKernelGGL(k_restrict_phi, dim3(div_up(nc, 256)), dim3(256), 0, s, phi_cur, w,
                            ctx->cap_coarsen, cwc, chc, ctx->cap_phi.as<double>() + (size_t)(i + 1) * nc);
         CD_CHECK_LAUNCH(ctx);
         cap_slots.push_back(slot);
@@ -1601,6 +1776,9 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     if (dot_blocks > nred) learned = false;  // tiny grids: partials hold nred blocks
     float* zbuf = (smooth_launches & 1) ? e32b : e32;
     int learned_applied = 0, learned_rejected = 0;
+    // a linear network maps the unnormalised coarse residual (scaling commutes
+    // with it): the restriction writes the network input, no statistics pass
+    const bool direct_input = learned && arch->linear;
 
     int check_id = 0;
     int cum = 0;
@@ -1618,8 +1796,14 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     if (learned && !done) {
         CD_TRY(cd_fill_f32(ctx, ctx->fcg_p.as<float>(), npk, 0.f));
         CD_TRY(cd_fill_f32(ctx, ctx->fcg_q.as<float>(), npk, 0.f));
-        hipLaunchKernelGGL(k_fcg_init, dim3(1), dim3(64), 0, s, ctx->fcg_dev.as<FcgDev>(), phi);
+        hipLaunchKernelGGL(k_fcg_init, dim3(1), dim3(64), 0, s, ctx->fcg_dev.as<FcgDev>(),
+                           ctx->fcg_p.as<float>());
         CD_CHECK_LAUNCH(ctx);
+        if (direct_input) {  // (mean, rms) of the unnormalised input: (0, 1)
+            double* derived = ctx->lc_stats.as<double>() + 2;
+            CD_TRY(cd_fill_f64(ctx, derived, 1, 0.0));
+            CD_TRY(cd_fill_f64(ctx, derived + 1, 1, 1.0));
+        }
         CD_TRY(ctx->ensure(ctx->phi_start, sizeof(double) * n));
         CD_HIP(ctx, hipMemcpyAsync(ctx->phi_start.ptr, phi, sizeof(double) * n,
                                    hipMemcpyDeviceToDevice, s));
@@ -1640,7 +1824,7 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         unsigned short* x8 = nullptr;
         bool prepared = false;
         CD_TRY(cd_net_coarse_input(ctx->net, lhc, lwc, &x8));
-        CD_TRY(restrict_stats(lf, lwc, lhc, stats, stats + 2, x8, &prepared));
+        CD_TRY(restrict_stats(lf, lwc, lhc, stats, stats + 2, x8, &prepared, direct_input));
         if (prepared)
             CD_TRY(cd_net_forward_prepared(ctx->net, ctx->lc_y.as<float>(), lhc, lwc));
         else
@@ -1681,15 +1865,28 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
             CD_TRY(enqueue_blocked_om(first_sweep, nsmooth, oms / 2.f, oms / 3.f, oms / 4.f));
         }
         CD_TRY(timer_begin());
-        hipLaunchKernelGGL(k_fcg_dots, dot_grid, dim3(256), 0, s, zbuf,
-                           ctx->fcg_p.as<float>(), ctx->fcg_q.as<float>(), r16, sd, bk, w, h,
+        hipLaunchKernelGGL(k_fcg_dots, dot_grid, dim3(256), 0, s, zbuf, fd, r16, sd, bk, w, h,
                            partials);
         hipLaunchKernelGGL(k_fcg_coef, dim3(1), dim3(1024), 0, s, partials, dot_blocks, fd);
-        hipLaunchKernelGGL(k_fcg_update, dim3(pbx, h), dim3(256), 0, s, zbuf,
-                           ctx->fcg_p.as<float>(), ctx->fcg_q.as<float>(), fd, bk, w, h);
         CD_CHECK_LAUNCH(ctx);
-        CD_TRY(timer_end(CD_KERNEL_LEARNED, 2, 38.0 * (double)n));
+        CD_TRY(timer_end(CD_KERNEL_LEARNED, 2, 10.0 * (double)n));
         return CD_OK;
+    };
+    // the check that ends an FCG iteration, with the iteration's step folded
+    // in: p = z + beta p_old, phi += alpha p, into the other buffer of each
+    // pair (the direction pair is fcg_p / fcg_q)
+    double* const phis[2] = {phi, ctx->fcg_phi.as<double>()};
+    float* const dirs[2] = {ctx->fcg_p.as<float>(), ctx->fcg_q.as<float>()};
+    int fk = 0, pk = 0;  // buffers holding the iterate and the last direction
+    auto enqueue_step_check = [&](int check_id) -> int {
+        CD_TRY(timer_begin());
+        launch_residual_step(kResidualRows, cgrid, rblk, s, phis[fk], phis[1 - fk], b, sd, zbuf,
+                             dirs[pk], dirs[1 - pk], ctx->fcg_dev.as<FcgDev>(), r16, bk, w, h,
+                             partials);
+        fk = 1 - fk;
+        pk = 1 - pk;
+        phi_cur = phis[fk];
+        return finish_check(check_id, 2, 38.0 * (double)n);
     };
     int fcg_iter = 0;
     while (!done && learned && cum < o->max_sweeps) {
@@ -1725,10 +1922,8 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         }
         ++fcg_iter;
         cum += nsmooth;
-        CD_TRY(enqueue_check(++check_id, false));
+        CD_TRY(enqueue_step_check(++check_id));
         ++learned_applied;
-        FcgDev* fd = ctx->fcg_dev.as<FcgDev>();
-        (void)fd;
         const SolveDev& hs = *ctx->host_status;
         done = hs.done != 0;
         if (!std::isfinite(hs.rel)) stall = 7;  // abandoned below, back to the warm start
@@ -1738,7 +1933,7 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
 #ifdef CD_TRACE_LEARNED
         {
             FcgDev hf;
-            CD_HIP(ctx, hipMemcpy(&hf, fd, sizeof(hf), hipMemcpyDeviceToHost));
+            CD_HIP(ctx, hipMemcpy(&hf, ctx->fcg_dev.ptr, sizeof(hf), hipMemcpyDeviceToHost));
             fprintf(stderr, "fcg %d: rel %.3e alpha %.3e beta %.3e pq %.3e bad %d\n",
                     learned_applied, hs.rel, hf.alpha, hf.beta, hf.pq, hf.bad);
         }
@@ -1755,11 +1950,18 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
                 // back to the warm start; the next check recomputes its residual
                 CD_HIP(ctx, hipMemcpyAsync(phi, ctx->phi_start.ptr, sizeof(double) * n,
                                            hipMemcpyDeviceToDevice, s));
+                fk = 0;
+                phi_cur = phi;
                 CD_TRY(enqueue_check(++check_id, false));
                 done = ctx->host_status->done != 0;
             }
         }
         prev_rel = ctx->host_status->rel;
+    }
+    if (fk) {  // the iterate ended in the second buffer
+        CD_HIP(ctx, hipMemcpyAsync(phi, phis[1], sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        fk = 0;
+        phi_cur = phi;
     }
     // plain sweeps after learned iterations start from a zero correction (the
     // iterations left their FP32 work in both ping-pong buffers)
