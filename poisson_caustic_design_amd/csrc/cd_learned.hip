@@ -201,59 +201,91 @@ __global__ __launch_bounds__(256) void k_conv3x3_tile(const u16* __restrict__ x1
     const int tx0 = blockIdx.x * CT, ty0 = blockIdx.y * CT;
     // Stage the input tile, 100 pixels x cin/8 vectors: thread t takes vector
     // t % ng of pixels t / ng, t / ng + 256 / ng, ... (one division per thread,
-    // none in the loop)
+    // none in the loop).  The loads go in groups of SB without branches (each
+    // from a clamped, valid pixel; out-of-image pixels are zeroed after the
+    // load), so that a group waits for one L2 round trip, not one per vector.
+    constexpr int SB = 8;
     const int ng = cin >> 3;
     const int pstride = 256 / ng;
     const int gi = threadIdx.x % ng;
     const int ci = gi * 8;
+    // this thread's source: channels ci.. of x2 (after the c1 of x1) or x1;
+    // mode1 applies to x1 only (a pooled input has no x2)
+    const bool use2 = ci >= c1;
+    const u16* __restrict__ src = use2 ? x2 + (ci - c1) : x1 + ci;
+    const int cs = use2 ? c2 : c1;
+    const int up = (!use2 && mode1 == 1) ? 1 : 0;  // nearest upsampling: source at half size
     if (threadIdx.x < pstride * ng) {
-        for (int px = threadIdx.x / ng; px < 100; px += pstride) {
-            const int yy = ty0 - 1 + px / 10, xx = tx0 - 1 + px % 10;
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
-                const long long q = ((long long)b * H + yy) * W + xx;
-                if (ci >= c1) {
-                    v = *reinterpret_cast<const uint4*>(x2 + q * c2 + (ci - c1));
-                } else if (mode1 == 0) {
-                    v = *reinterpret_cast<const uint4*>(x1 + q * c1 + ci);
-                } else if (mode1 == 1) {
-                    const long long q1 =
-                        ((long long)b * (H >> 1) + (yy >> 1)) * (W >> 1) + (xx >> 1);
-                    v = *reinterpret_cast<const uint4*>(x1 + q1 * c1 + ci);
-                } else {
-                    // x1 at twice the resolution, 2x2 average pooled as k_avgpool2 does
-                    const long long r0 = ((long long)b * (2 * H) + 2 * yy) * (2 * W) + 2 * xx;
+        for (int pb = threadIdx.x / ng; pb < 100; pb += SB * pstride) {
+            uint4 v[SB];
+            bool ok[SB];
+            if (mode1 == 2) {
+                // x1 at twice the resolution, 2x2 average pooled as k_avgpool2 does
+#pragma unroll
+                for (int u = 0; u < SB; ++u) {
+                    const int px = pb + u * pstride;
+                    const int yy = ty0 - 1 + px / 10, xx = tx0 - 1 + px % 10;
+                    ok[u] = px < 100 && yy >= 0 && yy < H && xx >= 0 && xx < W;
+                    const int yc = min(max(yy, 0), H - 1), xc = min(max(xx, 0), W - 1);
+                    const long long r0 = ((long long)b * (2 * H) + 2 * yc) * (2 * W) + 2 * xc;
                     const long long r1 = r0 + 2 * W;
                     V8 p00, p01, p10, p11, o;
-                    p00.u = *reinterpret_cast<const uint4*>(x1 + r0 * c1 + ci);
-                    p01.u = *reinterpret_cast<const uint4*>(x1 + (r0 + 1) * c1 + ci);
-                    p10.u = *reinterpret_cast<const uint4*>(x1 + r1 * c1 + ci);
-                    p11.u = *reinterpret_cast<const uint4*>(x1 + (r1 + 1) * c1 + ci);
+                    p00.u = *reinterpret_cast<const uint4*>(src + r0 * cs);
+                    p01.u = *reinterpret_cast<const uint4*>(src + (r0 + 1) * cs);
+                    p10.u = *reinterpret_cast<const uint4*>(src + r1 * cs);
+                    p11.u = *reinterpret_cast<const uint4*>(src + (r1 + 1) * cs);
 #pragma unroll
                     for (int e = 0; e < 8; ++e) {
                         const float sm = from_bf16(p00.h[e]) + from_bf16(p01.h[e]) +
                                          from_bf16(p10.h[e]) + from_bf16(p11.h[e]);
                         o.h[e] = to_bf16(0.25f * sm);
                     }
-                    v = o.u;
+                    v[u] = o.u;
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < SB; ++u) {
+                    const int px = pb + u * pstride;
+                    const int yy = ty0 - 1 + px / 10, xx = tx0 - 1 + px % 10;
+                    ok[u] = px < 100 && yy >= 0 && yy < H && xx >= 0 && xx < W;
+                    const int yc = min(max(yy, 0), H - 1) >> up, xc = min(max(xx, 0), W - 1) >> up;
+                    const long long q = ((long long)b * (H >> up) + yc) * (W >> up) + xc;
+                    v[u] = *reinterpret_cast<const uint4*>(src + q * cs);
                 }
             }
-            *reinterpret_cast<uint4*>(xs + px * xstride + ci) = v;
+#pragma unroll
+            for (int u = 0; u < SB; ++u) {
+                const int px = pb + u * pstride;
+                if (px < 100)
+                    *reinterpret_cast<uint4*>(xs + px * xstride + ci) =
+                        ok[u] ? v[u] : make_uint4(0, 0, 0, 0);
+            }
         }
     }
     // stage this block's weights: NB*16 rows of kpad, vector it = row * kv + cv
-    // for it = threadIdx.x, +256, ... with (row, cv) advanced incrementally
+    // for it = threadIdx.x, +256, ... with (row, cv) advanced incrementally,
+    // in branch-free groups of SB loads as the tile (a vector past the block's
+    // last row is loaded from that row and stored there: the value it has)
     const int kv = kpad >> 3;
     const u16* wsrc = wt + (size_t)cblk * NB * 16 * kpad;
     {
         const int dq = 256 / kv, dr = 256 - dq * kv;
         int row = threadIdx.x / kv, cv = threadIdx.x - row * kv;
         while (row < NB * 16) {
-            *reinterpret_cast<uint4*>(ws + row * wstride + cv * 8) =
-                *reinterpret_cast<const uint4*>(wsrc + (size_t)row * kpad + cv * 8);
-            row += dq;
-            cv += dr;
-            if (cv >= kv) { cv -= kv; ++row; }
+            uint4 v[SB];
+            int rr[SB], cc[SB];
+#pragma unroll
+            for (int u = 0; u < SB; ++u) {
+                rr[u] = min(row, NB * 16 - 1);
+                cc[u] = cv;
+                v[u] = *reinterpret_cast<const uint4*>(wsrc + (size_t)rr[u] * kpad + cv * 8);
+                row += dq;
+                cv += dr;
+                if (cv >= kv) { cv -= kv; ++row; }
+            }
+#pragma unroll
+            for (int u = 0; u < SB; ++u)
+                *reinterpret_cast<uint4*>(ws + rr[u] * wstride + cc[u] * 8) = v[u];
         }
     }
     __syncthreads();
@@ -783,6 +815,7 @@ int launch_conv(cd_net* n, const NetConv& cv, int B, int H, int W, bool grad_inp
     const size_t lds_t = sizeof(u16) * (100 * (size_t)(cv.c1 + cv.c2 + 8) +
                                         (size_t)nbw_t * 16 * (cv.kpad + 8));
     const bool tiled = !grad_input && n->tiled_forward && lds_t <= tiled_lds_limit(c);
+    if (mode1 == 2 && cv.c2 > 0) return CD_ERR_BAD_ARG;  // a pooled input has no skip part
     if (mode1 == 2 && !tiled) {
         const long long cnt = npix * cv.c1;
         hipLaunchKernelGGL(k_avgpool2, dim3(div_up(cnt, 256)), dim3(256), 0, s, x1src,
