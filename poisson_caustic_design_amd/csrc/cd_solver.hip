@@ -1310,6 +1310,12 @@ __global__ void k_fcg_coef(const double* __restrict__ partials, int nblocks, Fcg
 // phi_in and p_old at the four neighbours and forms their new values itself
 // (the same FP32 and FP64 operations as at the centre, so every cell's new
 // value is one number).  fd->pprev is pointed at p_new for the next iteration.
+// x8 (may be NULL): the workgroup also writes the next iteration's network
+// input, the lf x lf block means of the new residual (BF16, channel 0 of 8 of
+// a coarse grid lwc wide).  The host passes x8 only when the workgroups tile
+// the grid exactly and cover whole coarse cells: lf divides 64, and the
+// 4*ROWS rows of a workgroup are one or two coarse rows.  The FP64 residual is
+// mean-free to rounding, so its block means need no projection.
 template <int ROWS>
 __global__ __launch_bounds__(256) void k_residual_step(const double* __restrict__ phi_in,
                                                        double* __restrict__ phi_out,
@@ -1320,9 +1326,13 @@ __global__ __launch_bounds__(256) void k_residual_step(const double* __restrict_
                                                        float* __restrict__ p_new,
                                                        FcgDev* __restrict__ fd,
                                                        unsigned short* __restrict__ r16, Brick bk,
-                                                       int w, int h, double* __restrict__ partials) {
+                                                       int w, int h, double* __restrict__ partials,
+                                                       unsigned short* __restrict__ x8, int lf,
+                                                       int lwc) {
     constexpr int G = ROWS < 2 ? ROWS : 2;  // rows whose loads are in flight together
     __shared__ double smem[16];
+    __shared__ float csum[2][4][64];  // (coarse row, thread row, column) partial block sums
+    float cpart0 = 0.f, cpart1 = 0.f;  // this thread's sums per coarse row
     const double bmean = st->bmean;
     const float alpha = (float)fd->alpha, beta = (float)fd->beta;
     const int x0 = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1373,6 +1383,10 @@ __global__ __launch_bounds__(256) void k_residual_step(const double* __restrict_
             const double rv = (bb[k] - bmean) - (s - cnt * nv[0]);
             rs += rv;
             rq += rv * rv;
+            if (x8) {
+                if ((int)threadIdx.y + 4 * (g + k) >= lf) cpart1 += (float)rv;
+                else cpart0 += (float)rv;
+            }
             const size_t i = (size_t)y * w + x;
             const size_t bi = brick_at(bk, x, y);
             phi_out[i] = nv[0];
@@ -1388,18 +1402,39 @@ __global__ __launch_bounds__(256) void k_residual_step(const double* __restrict_
         partials[2 * bid + 1] = q;
         if (bid == 0) fd->pprev = p_new;
     }
+    if (x8) {  // uniform: every thread of the launch takes this branch or none
+        const int ncr = (4 * ROWS) / lf > 1 ? 2 : 1;  // coarse rows of this workgroup
+        csum[0][threadIdx.y][threadIdx.x] = cpart0;
+        csum[1][threadIdx.y][threadIdx.x] = cpart1;
+        __syncthreads();
+        if (threadIdx.y < ncr) {
+            // thread row cr sums coarse row cr over the four thread rows, then
+            // over groups of lf lanes (fixed order: xor tree)
+            const int cr = threadIdx.y;
+            float v = ((csum[cr][0][threadIdx.x] + csum[cr][1][threadIdx.x]) +
+                       csum[cr][2][threadIdx.x]) + csum[cr][3][threadIdx.x];
+            for (int o = 1; o < lf; o <<= 1) v += __shfl_xor(v, o, 64);
+            if ((threadIdx.x & (lf - 1)) == 0) {
+                const int cx = (blockIdx.x * 64 + threadIdx.x) / lf;
+                const int cy = (blockIdx.y * 4 * ROWS) / lf + cr;
+                *reinterpret_cast<uint4*>(x8 + ((size_t)cy * lwc + cx) * 8) =
+                    make_uint4(f32_to_bf16(v / (float)(lf * lf)), 0, 0, 0);
+            }
+        }
+    }
 }
 
 // launch of k_residual_step<rows> for rows in {1, 2, 4, 8}
 void launch_residual_step(int rows, dim3 grid, dim3 blk, hipStream_t s, const double* phi_in,
                           double* phi_out, const double* b, const SolveDev* st, const float* z,
                           const float* p_old, float* p_new, FcgDev* fd, unsigned short* r16,
-                          const Brick& bk, int w, int h, double* partials) {
+                          const Brick& bk, int w, int h, double* partials, unsigned short* x8,
+                          int lf, int lwc) {
     switch (rows) {
-    case 8: hipLaunchKernelGGL(k_residual_step<8>, grid, blk, 0, s, phi_in, phi_out, b, st, z, p_old, p_new, fd, r16, bk, w, h, partials); break;
-    case 4: hipLaunchKernelGGL(k_residual_step<4>, grid, blk, 0, s, phi_in, phi_out, b, st, z, p_old, p_new, fd, r16, bk, w, h, partials); break;
-    case 2: hipLaunchKernelGGL(k_residual_step<2>, grid, blk, 0, s, phi_in, phi_out, b, st, z, p_old, p_new, fd, r16, bk, w, h, partials); break;
-    default: hipLaunchKernelGGL(k_residual_step<1>, grid, blk, 0, s, phi_in, phi_out, b, st, z, p_old, p_new, fd, r16, bk, w, h, partials); break;
+    case 8: hipLaunchKernelGGL(k_residual_step<8>, grid, blk, 0, s, phi_in, phi_out, b, st, z, p_old, p_new, fd, r16, bk, w, h, partials, x8, lf, lwc); break;
+    case 4: hipLaunchKernelGGL(k_residual_step<4>, grid, blk, 0, s, phi_in, phi_out, b, st, z, p_old, p_new, fd, r16, bk, w, h, partials, x8, lf, lwc); break;
+    case 2: hipLaunchKernelGGL(k_residual_step<2>, grid, blk, 0, s, phi_in, phi_out, b, st, z, p_old, p_new, fd, r16, bk, w, h, partials, x8, lf, lwc); break;
+    default: hipLaunchKernelGGL(k_residual_step<1>, grid, blk, 0, s, phi_in, phi_out, b, st, z, p_old, p_new, fd, r16, bk, w, h, partials, x8, lf, lwc); break;
     }
 }
 
@@ -1779,6 +1814,16 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     // a linear network maps the unnormalised coarse residual (scaling commutes
     // with it): the restriction writes the network input, no statistics pass
     const bool direct_input = learned && arch->linear;
+    // ... and where the check's workgroups cover whole coarse cells, the check
+    // that ends an FCG iteration writes the next iteration's network input
+    // itself (the iterations after the first start from it)
+    const int crows = 4 * kResidualRows;  // rows of a check workgroup
+    const bool fuse_restrict = direct_input && lf >= 2 && lf <= 64 && (lf & (lf - 1)) == 0 &&
+                               w % 64 == 0 && h % crows == 0 && crows % lf == 0 &&
+                               crows / lf <= 2 && lwc * lf == w && lhc * lf == h;
+    unsigned short* net_x8 = nullptr;
+    if (fuse_restrict) CD_TRY(cd_net_coarse_input(ctx->net, lhc, lwc, &net_x8));
+    bool x8_ready = false;  // the last check wrote the network input
 
     int check_id = 0;
     int cum = 0;
@@ -1824,7 +1869,10 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         unsigned short* x8 = nullptr;
         bool prepared = false;
         CD_TRY(cd_net_coarse_input(ctx->net, lhc, lwc, &x8));
-        CD_TRY(restrict_stats(lf, lwc, lhc, stats, stats + 2, x8, &prepared, direct_input));
+        if (x8_ready && x8 == net_x8)
+            prepared = true;
+        else
+            CD_TRY(restrict_stats(lf, lwc, lhc, stats, stats + 2, x8, &prepared, direct_input));
         if (prepared)
             CD_TRY(cd_net_forward_prepared(ctx->net, ctx->lc_y.as<float>(), lhc, lwc));
         else
@@ -1882,7 +1930,8 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         CD_TRY(timer_begin());
         launch_residual_step(kResidualRows, cgrid, rblk, s, phis[fk], phis[1 - fk], b, sd, zbuf,
                              dirs[pk], dirs[1 - pk], ctx->fcg_dev.as<FcgDev>(), r16, bk, w, h,
-                             partials);
+                             partials, fuse_restrict ? net_x8 : nullptr, lf, lwc);
+        x8_ready = fuse_restrict;
         fk = 1 - fk;
         pk = 1 - pk;
         phi_cur = phis[fk];
