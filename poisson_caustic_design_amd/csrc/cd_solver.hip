@@ -1316,6 +1316,14 @@ __global__ void k_fcg_coef(const double* __restrict__ partials, int nblocks, Fcg
 // the grid exactly and cover whole coarse cells: lf divides 64, and the
 // 4*ROWS rows of a workgroup are one or two coarse rows.  The FP64 residual is
 // mean-free to rounding, so its block means need no projection.
+// rows whose loads a thread issues together, and the most rows a thread takes
+// (2 / 8 measured as fast as 4 / 8 and 2 / 4 at 1024^2 and 2048^2)
+#ifndef CD_STEP_G
+#define CD_STEP_G 2
+#endif
+#ifndef CD_STEP_ROWS
+#define CD_STEP_ROWS 8
+#endif
 template <int ROWS>
 __global__ __launch_bounds__(256) void k_residual_step(const double* __restrict__ phi_in,
                                                        double* __restrict__ phi_out,
@@ -1329,7 +1337,7 @@ __global__ __launch_bounds__(256) void k_residual_step(const double* __restrict_
                                                        int w, int h, double* __restrict__ partials,
                                                        unsigned short* __restrict__ x8, int lf,
                                                        int lwc) {
-    constexpr int G = ROWS < 2 ? ROWS : 2;  // rows whose loads are in flight together
+    constexpr int G = ROWS < CD_STEP_G ? ROWS : CD_STEP_G;  // rows whose loads are in flight together
     __shared__ double smem[16];
     __shared__ float csum[2][4][64];  // (coarse row, thread row, column) partial block sums
     float cpart0 = 0.f, cpart1 = 0.f;  // this thread's sums per coarse row
@@ -1339,6 +1347,14 @@ __global__ __launch_bounds__(256) void k_residual_step(const double* __restrict_
     const int y0 = blockIdx.y * blockDim.y * ROWS + threadIdx.y;
     const int x = min(x0, w - 1);
     const int xl = max(x - 1, 0), xr = min(x + 1, w - 1);
+    // bricked offsets (a plane pair holds < 2^29 elements in blocked mode): the
+    // column parts of this cell and of its left / right neighbours are the same
+    // on every row; their planes alternate with the row
+    const unsigned cj = brick_col(x >> 1), cjl = brick_col(xl >> 1), cjr = brick_col(xr >> 1);
+    auto rowo = [&](int c, int y) -> unsigned {
+        return (unsigned)c * (unsigned)bk.plane + ((unsigned)(y >> 3) * (unsigned)bk.nbx) * 512u +
+               (unsigned)(y & 7) * 64u;
+    };
     // new direction and iterate at (xx, yy) from the loaded phi_in, z, p_old
     auto pnew = [&](float zv, float pv) { return zv + beta * pv; };
     auto phinew = [&](double ph, float pn) { return ph + (double)(alpha * pn); };
@@ -1351,16 +1367,22 @@ __global__ __launch_bounds__(256) void k_residual_step(const double* __restrict_
         for (int k = 0; k < G; ++k) {
             const int y = min(y0 + (g + k) * (int)blockDim.y, h - 1);
             const int yu = max(y - 1, 0), yd = min(y + 1, h - 1);
-            const int xs[5] = {x, xl, x, xr, x};
-            const int ys[5] = {y, y, yu, y, yd};
+            const int c = (x + y) & 1, o = 1 - c;
+            // centre, left, up, right, down (a clamped neighbour at the border
+            // is some valid cell: its value is not used)
+            const unsigned ro = rowo(o, y);
+            const unsigned bis[5] = {rowo(c, y) + cj, ro + cjl, rowo(o, yu) + cj, ro + cjr,
+                                     rowo(o, yd) + cj};
+            const size_t ry = (size_t)y * w;
+            const size_t pis[5] = {ry + x, ry + xl, (size_t)yu * w + x, ry + xr,
+                                   (size_t)yd * w + x};
 #pragma unroll
             for (int t = 0; t < 5; ++t) {
-                ph[k][t] = phi_in[(size_t)ys[t] * w + xs[t]];
-                const size_t bi = brick_at(bk, xs[t], ys[t]);
-                zv[k][t] = z[bi];
-                pv[k][t] = p_old[bi];
+                ph[k][t] = phi_in[pis[t]];
+                zv[k][t] = z[bis[t]];
+                pv[k][t] = p_old[bis[t]];
             }
-            bb[k] = b[(size_t)y * w + x];
+            bb[k] = b[ry + x];
         }
 #pragma unroll
         for (int k = 0; k < G; ++k) {
@@ -1388,7 +1410,7 @@ __global__ __launch_bounds__(256) void k_residual_step(const double* __restrict_
                 else cpart0 += (float)rv;
             }
             const size_t i = (size_t)y * w + x;
-            const size_t bi = brick_at(bk, x, y);
+            const unsigned bi = rowo((x + y) & 1, y) + cj;
             phi_out[i] = nv[0];
             p_new[bi] = pc;
             r16[bi] = f32_to_bf16((float)rv);
@@ -1571,9 +1593,10 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     // iterations add theirs to phi64 themselves)
     // the end of every check: k_check on the residual pass's partials, then the
     // host's wait for the status (the residual pass's timer is open)
-    auto finish_check = [&](int check_id, int timer_kernels, double bytes) -> int {
+    auto finish_check = [&](int check_id, int timer_kernels, double bytes,
+                            int nparts) -> int {
         const unsigned seq = ++ctx->status_seq;
-        hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, partials, ncheck, sd,
+        hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, partials, nparts, sd,
                            ctx->host_status, seq, count, check_id, o->stop, o->tol);
         CD_CHECK_LAUNCH(ctx);
         CD_TRY(timer_end(CD_KERNEL_RESIDUAL, timer_kernels, bytes));
@@ -1605,7 +1628,8 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
                                partials);
         }
         return finish_check(check_id, (mixed && flush) ? 3 : 2,
-                            (blocked ? (flush ? 48.0 : 18.0) : (mixed ? 44.0 : 16.0)) * (double)n);
+                            (blocked ? (flush ? 48.0 : 18.0) : (mixed ? 44.0 : 16.0)) * (double)n,
+                            ncheck);
     };
 
     const dim3 sblk(SWEEP_BX, SWEEP_BY);
@@ -1817,7 +1841,11 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     // ... and where the check's workgroups cover whole coarse cells, the check
     // that ends an FCG iteration writes the next iteration's network input
     // itself (the iterations after the first start from it)
-    const int crows = 4 * kResidualRows;  // rows of a check workgroup
+    // the step check's rows per thread and grid (its partials stay <= nred)
+    const int kStepRows = std::min(kResidualRows, CD_STEP_ROWS);
+    const dim3 stgrid(div_up(w, RED_BX), div_up(h, RED_BY * kStepRows));
+    const int nstep = stgrid.x * stgrid.y;
+    const int crows = 4 * kStepRows;  // rows of a step-check workgroup
     const bool fuse_restrict = direct_input && lf >= 2 && lf <= 64 && (lf & (lf - 1)) == 0 &&
                                w % 64 == 0 && h % crows == 0 && crows % lf == 0 &&
                                crows / lf <= 2 && lwc * lf == w && lhc * lf == h;
@@ -1928,14 +1956,14 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     int fk = 0, pk = 0;  // buffers holding the iterate and the last direction
     auto enqueue_step_check = [&](int check_id) -> int {
         CD_TRY(timer_begin());
-        launch_residual_step(kResidualRows, cgrid, rblk, s, phis[fk], phis[1 - fk], b, sd, zbuf,
+        launch_residual_step(kStepRows, stgrid, rblk, s, phis[fk], phis[1 - fk], b, sd, zbuf,
                              dirs[pk], dirs[1 - pk], ctx->fcg_dev.as<FcgDev>(), r16, bk, w, h,
                              partials, fuse_restrict ? net_x8 : nullptr, lf, lwc);
         x8_ready = fuse_restrict;
         fk = 1 - fk;
         pk = 1 - pk;
         phi_cur = phis[fk];
-        return finish_check(check_id, 2, 38.0 * (double)n);
+        return finish_check(check_id, 2, 38.0 * (double)n, nstep);
     };
     int fcg_iter = 0;
     while (!done && learned && cum < o->max_sweeps) {
