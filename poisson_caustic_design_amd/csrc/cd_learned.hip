@@ -184,6 +184,157 @@ __global__ __launch_bounds__(256) void k_conv3x3(const u16* __restrict__ x1,
 // LDS: 100 * (cin + 8) + NB*16 * (kpad + 8) u16 (row paddings spread banks).
 // ---------------------------------------------------------------------------
 constexpr int CT = 8;  // output tile side
+
+// Stage a P x P patch of the (concatenated) input, image origin (oy, ox), into
+// LDS at xs (u16 pixel stride xstride): thread t takes vector t % ng of pixels
+// t / ng, t / ng + 256 / ng, ... (one division per thread, none in the loop).
+// The loads go in groups of SB without branches (each from a clamped, valid
+// pixel; out-of-image pixels are zeroed after the load), so that a group waits
+// for one L2 round trip, not one per vector.  mode1 as for k_conv3x3_tile.
+template <int P>
+__device__ __forceinline__ void stage_patch(u16* __restrict__ xs, int xstride,
+                                            const u16* __restrict__ x1,
+                                            const u16* __restrict__ x2, int c1, int c2, int H,
+                                            int W, int b, int oy, int ox, int mode1) {
+    constexpr int SB = 8;
+    constexpr int NP = P * P;
+    const int ng = (c1 + c2) >> 3;
+    const int pstride = 256 / ng;
+    const int gi = threadIdx.x % ng;
+    const int ci = gi * 8;
+    // this thread's source: channels ci.. of x2 (after the c1 of x1) or x1;
+    // mode1 applies to x1 only (a pooled input has no x2)
+    const bool use2 = ci >= c1;
+    const u16* __restrict__ src = use2 ? x2 + (ci - c1) : x1 + ci;
+    const int cs = use2 ? c2 : c1;
+    const int up = (!use2 && mode1 == 1) ? 1 : 0;  // nearest upsampling: source at half size
+    if (threadIdx.x >= pstride * ng) return;
+    for (int pb = threadIdx.x / ng; pb < NP; pb += SB * pstride) {
+        uint4 v[SB];
+        bool ok[SB];
+        if (mode1 == 2) {
+            // x1 at twice the resolution, 2x2 average pooled as k_avgpool2 does
+#pragma unroll
+            for (int u = 0; u < SB; ++u) {
+                const int px = pb + u * pstride;
+                const int yy = oy + px / P, xx = ox + px % P;
+                ok[u] = px < NP && yy >= 0 && yy < H && xx >= 0 && xx < W;
+                const int yc = min(max(yy, 0), H - 1), xc = min(max(xx, 0), W - 1);
+                const long long r0 = ((long long)b * (2 * H) + 2 * yc) * (2 * W) + 2 * xc;
+                const long long r1 = r0 + 2 * W;
+                V8 p00, p01, p10, p11, o;
+                p00.u = *reinterpret_cast<const uint4*>(src + r0 * cs);
+                p01.u = *reinterpret_cast<const uint4*>(src + (r0 + 1) * cs);
+                p10.u = *reinterpret_cast<const uint4*>(src + r1 * cs);
+                p11.u = *reinterpret_cast<const uint4*>(src + (r1 + 1) * cs);
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const float sm = from_bf16(p00.h[e]) + from_bf16(p01.h[e]) +
+                                     from_bf16(p10.h[e]) + from_bf16(p11.h[e]);
+                    o.h[e] = to_bf16(0.25f * sm);
+                }
+                v[u] = o.u;
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < SB; ++u) {
+                const int px = pb + u * pstride;
+                const int yy = oy + px / P, xx = ox + px % P;
+                ok[u] = px < NP && yy >= 0 && yy < H && xx >= 0 && xx < W;
+                const int yc = min(max(yy, 0), H - 1) >> up, xc = min(max(xx, 0), W - 1) >> up;
+                const long long q = ((long long)b * (H >> up) + yc) * (W >> up) + xc;
+                v[u] = *reinterpret_cast<const uint4*>(src + q * cs);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+            const int px = pb + u * pstride;
+            if (px < NP)
+                *reinterpret_cast<uint4*>(xs + px * xstride + ci) =
+                    ok[u] ? v[u] : make_uint4(0, 0, 0, 0);
+        }
+    }
+}
+
+// Stage `rows` weight rows of kpad (u16) from wsrc into ws (row stride kpad + 8):
+// vector it = row * kv + cv for it = threadIdx.x, +256, ... with (row, cv)
+// advanced incrementally, in branch-free groups of 8 loads (a vector past the
+// last row is loaded from that row and stored there: the value it has)
+__device__ __forceinline__ void stage_weights(u16* __restrict__ ws,
+                                              const u16* __restrict__ wsrc, int rows,
+                                              int kpad) {
+    constexpr int SB = 8;
+    const int kv = kpad >> 3, wstride = kpad + 8;
+    const int dq = 256 / kv, dr = 256 - dq * kv;
+    int row = threadIdx.x / kv, cv = threadIdx.x - row * kv;
+    while (row < rows) {
+        uint4 v[SB];
+        int rr[SB], cc[SB];
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+            rr[u] = min(row, rows - 1);
+            cc[u] = cv;
+            v[u] = *reinterpret_cast<const uint4*>(wsrc + (size_t)rr[u] * kpad + cv * 8);
+            row += dq;
+            cv += dr;
+            if (cv >= kv) { cv -= kv; ++row; }
+        }
+#pragma unroll
+        for (int u = 0; u < SB; ++u)
+            *reinterpret_cast<uint4*>(ws + rr[u] * wstride + cc[u] * 8) = v[u];
+    }
+}
+
+// The K loop of a 3x3 convolution read from LDS: an input patch of side P
+// (pixel stride xstride, cin channels) and NB*16 weight rows (stride kpad + 8).
+// Lane l's A pixel of M-tile m has patch index pbase[m] (its 3x3 neighbourhood
+// starts there); acc[m][nb] accumulates output channels nb*16 + (l % 16) in
+// the order of the K chunks, the order k_conv3x3 uses.  k = kc + kq = tap *
+// cin + cc is tracked incrementally: cin is a multiple of 8, so the lane's 8
+// consecutive k lie in one tap.
+template <int NB, int MT, int P>
+__device__ __forceinline__ void conv_lds(floatx4 (&acc)[MT][NB], const u16* __restrict__ xs,
+                                         int xstride, int cin, const u16* __restrict__ ws,
+                                         int kpad, const int (&pbase)[MT], int lane) {
+    const int K = 9 * cin;
+    const int kq = 8 * (lane >> 4);
+    const int wstride = kpad + 8;
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) acc[m][nb] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const u16* wrow = ws + (lane & 15) * wstride + kq;
+    int cc = kq, tdy = 0, tdx = 0;
+    while (cc >= cin) {
+        cc -= cin;
+        if (++tdx == 3) { tdx = 0; ++tdy; }
+    }
+    for (int kc = 0; kc < kpad; kc += 32) {
+        V8 a[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+            a[m].u = make_uint4(0, 0, 0, 0);
+            if (kc + kq < K)
+                a[m].u = *reinterpret_cast<const uint4*>(
+                    xs + (pbase[m] + tdy * P + tdx) * xstride + cc);
+        }
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            V8 bw;
+            bw.u = *reinterpret_cast<const uint4*>(wrow + nb * 16 * wstride + kc);
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+                acc[m][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m].v, bw.v, acc[m][nb],
+                                                                     0, 0, 0);
+        }
+        cc += 32;
+        while (cc >= cin) {
+            cc -= cin;
+            if (++tdx == 3) { tdx = 0; ++tdy; }
+        }
+    }
+}
+
 template <int NB, int OUT>
 __global__ __launch_bounds__(256) void k_conv3x3_tile(const u16* __restrict__ x1,
                                                       const u16* __restrict__ x2, int c1, int c2,
@@ -196,134 +347,18 @@ __global__ __launch_bounds__(256) void k_conv3x3_tile(const u16* __restrict__ x1
     const int cin = c1 + c2;
     const int xstride = cin + 8;                      // u16 per tile pixel
     u16* ws = xs + 100 * xstride;                     // 16-byte aligned: xstride % 8 == 0
-    const int wstride = kpad + 8;
     const int cblk = blockIdx.z % cblocks, b = blockIdx.z / cblocks;
     const int tx0 = blockIdx.x * CT, ty0 = blockIdx.y * CT;
-    // Stage the input tile, 100 pixels x cin/8 vectors: thread t takes vector
-    // t % ng of pixels t / ng, t / ng + 256 / ng, ... (one division per thread,
-    // none in the loop).  The loads go in groups of SB without branches (each
-    // from a clamped, valid pixel; out-of-image pixels are zeroed after the
-    // load), so that a group waits for one L2 round trip, not one per vector.
-    constexpr int SB = 8;
-    const int ng = cin >> 3;
-    const int pstride = 256 / ng;
-    const int gi = threadIdx.x % ng;
-    const int ci = gi * 8;
-    // this thread's source: channels ci.. of x2 (after the c1 of x1) or x1;
-    // mode1 applies to x1 only (a pooled input has no x2)
-    const bool use2 = ci >= c1;
-    const u16* __restrict__ src = use2 ? x2 + (ci - c1) : x1 + ci;
-    const int cs = use2 ? c2 : c1;
-    const int up = (!use2 && mode1 == 1) ? 1 : 0;  // nearest upsampling: source at half size
-    if (threadIdx.x < pstride * ng) {
-        for (int pb = threadIdx.x / ng; pb < 100; pb += SB * pstride) {
-            uint4 v[SB];
-            bool ok[SB];
-            if (mode1 == 2) {
-                // x1 at twice the resolution, 2x2 average pooled as k_avgpool2 does
-#pragma unroll
-                for (int u = 0; u < SB; ++u) {
-                    const int px = pb + u * pstride;
-                    const int yy = ty0 - 1 + px / 10, xx = tx0 - 1 + px % 10;
-                    ok[u] = px < 100 && yy >= 0 && yy < H && xx >= 0 && xx < W;
-                    const int yc = min(max(yy, 0), H - 1), xc = min(max(xx, 0), W - 1);
-                    const long long r0 = ((long long)b * (2 * H) + 2 * yc) * (2 * W) + 2 * xc;
-                    const long long r1 = r0 + 2 * W;
-                    V8 p00, p01, p10, p11, o;
-                    p00.u = *reinterpret_cast<const uint4*>(src + r0 * cs);
-                    p01.u = *reinterpret_cast<const uint4*>(src + (r0 + 1) * cs);
-                    p10.u = *reinterpret_cast<const uint4*>(src + r1 * cs);
-                    p11.u = *reinterpret_cast<const uint4*>(src + (r1 + 1) * cs);
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) {
-                        const float sm = from_bf16(p00.h[e]) + from_bf16(p01.h[e]) +
-                                         from_bf16(p10.h[e]) + from_bf16(p11.h[e]);
-                        o.h[e] = to_bf16(0.25f * sm);
-                    }
-                    v[u] = o.u;
-                }
-            } else {
-#pragma unroll
-                for (int u = 0; u < SB; ++u) {
-                    const int px = pb + u * pstride;
-                    const int yy = ty0 - 1 + px / 10, xx = tx0 - 1 + px % 10;
-                    ok[u] = px < 100 && yy >= 0 && yy < H && xx >= 0 && xx < W;
-                    const int yc = min(max(yy, 0), H - 1) >> up, xc = min(max(xx, 0), W - 1) >> up;
-                    const long long q = ((long long)b * (H >> up) + yc) * (W >> up) + xc;
-                    v[u] = *reinterpret_cast<const uint4*>(src + q * cs);
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < SB; ++u) {
-                const int px = pb + u * pstride;
-                if (px < 100)
-                    *reinterpret_cast<uint4*>(xs + px * xstride + ci) =
-                        ok[u] ? v[u] : make_uint4(0, 0, 0, 0);
-            }
-        }
-    }
-    // stage this block's weights: NB*16 rows of kpad, vector it = row * kv + cv
-    // for it = threadIdx.x, +256, ... with (row, cv) advanced incrementally,
-    // in branch-free groups of SB loads as the tile (a vector past the block's
-    // last row is loaded from that row and stored there: the value it has)
-    const int kv = kpad >> 3;
-    const u16* wsrc = wt + (size_t)cblk * NB * 16 * kpad;
-    {
-        const int dq = 256 / kv, dr = 256 - dq * kv;
-        int row = threadIdx.x / kv, cv = threadIdx.x - row * kv;
-        while (row < NB * 16) {
-            uint4 v[SB];
-            int rr[SB], cc[SB];
-#pragma unroll
-            for (int u = 0; u < SB; ++u) {
-                rr[u] = min(row, NB * 16 - 1);
-                cc[u] = cv;
-                v[u] = *reinterpret_cast<const uint4*>(wsrc + (size_t)rr[u] * kpad + cv * 8);
-                row += dq;
-                cv += dr;
-                if (cv >= kv) { cv -= kv; ++row; }
-            }
-#pragma unroll
-            for (int u = 0; u < SB; ++u)
-                *reinterpret_cast<uint4*>(ws + rr[u] * wstride + cc[u] * 8) = v[u];
-        }
-    }
+    stage_patch<10>(xs, xstride, x1, x2, c1, c2, H, W, b, ty0 - 1, tx0 - 1, mode1);
+    stage_weights(ws, wt + (size_t)cblk * NB * 16 * kpad, NB * 16, kpad);
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     // this lane's A pixel: tile row 2*wave + (l%16)/8, column (l%16)%8
     const int ar = 2 * wave + ((lane & 15) >> 3), ac = lane & 7;
-    const int K = 9 * cin;
-    const int kq = 8 * (lane >> 4);
-    floatx4 acc[NB];
-#pragma unroll
-    for (int nb = 0; nb < NB; ++nb) acc[nb] = floatx4{0.f, 0.f, 0.f, 0.f};
-    const u16* wrow = ws + (lane & 15) * wstride + kq;
-    // k = kc + kq = tap * cin + ci, tracked incrementally: cin is a multiple
-    // of 8, so the lane's 8 consecutive k lie in one tap
-    int tap = 0, cc = kq, tdy = 0, tdx = 0;
-    while (cc >= cin) {
-        cc -= cin;
-        ++tap;
-        if (++tdx == 3) { tdx = 0; ++tdy; }
-    }
-    const int pbase = ar * 10 + ac;
-    for (int kc = 0; kc < kpad; kc += 32) {
-        V8 a;
-        a.u = make_uint4(0, 0, 0, 0);
-        if (kc + kq < K)
-            a.u = *reinterpret_cast<const uint4*>(xs + (pbase + tdy * 10 + tdx) * xstride + cc);
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-            V8 bw;
-            bw.u = *reinterpret_cast<const uint4*>(wrow + nb * 16 * wstride + kc);
-            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, bw.v, acc[nb], 0, 0, 0);
-        }
-        cc += 32;
-        while (cc >= cin) {
-            cc -= cin;
-            if (++tdx == 3) { tdx = 0; ++tdy; }
-        }
-    }
+    const int pbase[1] = {ar * 10 + ac};
+    floatx4 acc2[1][NB];
+    conv_lds<NB, 1, 10>(acc2, xs, xstride, cin, ws, kpad, pbase, lane);
+    floatx4(&acc)[NB] = acc2[0];
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
 #pragma unroll
@@ -343,6 +378,95 @@ __global__ __launch_bounds__(256) void k_conv3x3_tile(const u16* __restrict__ x1
             }
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Two consecutive 3x3 convolutions of one resolution level in one launch
+// (inference): conv a reads its inputs as k_conv3x3_tile does (mode1) on the
+// 12x12 patch around an 8x8 output tile and computes the 10x10 region that
+// conv b needs (7 M-tiles of 16 pixels, two per wave), which stays in LDS as
+// BF16 (zero outside the image: conv b's padding); conv b then computes the
+// tile from it.  Every workgroup computes all output channels of both convs
+// (NBA*16 and NBB*16).  The A/B fragments and the K order of both are those of
+// k_conv3x3_tile, and the intermediate is rounded to BF16 as the stored tensor
+// is, so the output is bit-identical to the two launches; conv a's output is
+// not written (inference only).  wb_early: conv b's weights are staged with
+// conv a's at the start (separate LDS); else after conv a, over conv a's.
+// grid: (ceil(W/8), ceil(H/8), B).  LDS (u16): 144 (cin + 8) + 100 (cya + 8) +
+// NBA*16 (kpada + 8) [+ NBB*16 (kpadb + 8)], or max of the two weight blocks.
+// ---------------------------------------------------------------------------
+template <int NBA, int NBB>
+__global__ __launch_bounds__(256) void k_conv_pair_tile(
+    const u16* __restrict__ x1, const u16* __restrict__ x2, int c1, int c2,
+    const u16* __restrict__ wa, int kpada, int cya, int relua, const u16* __restrict__ wb,
+    int kpadb, int cyb, int relub, u16* __restrict__ ybf, int H, int W, int mode1,
+    int wb_early) {
+    extern __shared__ uint4 tile_lds[];
+    const int cin = c1 + c2;
+    const int xstride = cin + 8, istride = cya + 8;
+    u16* xs = reinterpret_cast<u16*>(tile_lds);
+    u16* ism = xs + 144 * xstride;
+    u16* wsa = ism + 100 * istride;
+    u16* wsb = wb_early ? wsa + NBA * 16 * (kpada + 8) : wsa;
+    const int b = blockIdx.z;
+    const int tx0 = blockIdx.x * CT, ty0 = blockIdx.y * CT;
+    stage_patch<12>(xs, xstride, x1, x2, c1, c2, H, W, b, ty0 - 2, tx0 - 2, mode1);
+    stage_weights(wsa, wa, NBA * 16, kpada);
+    if (wb_early) stage_weights(wsb, wb, NBB * 16, kpadb);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    {
+        // conv a: M-tiles wave and wave + 4 of the 10x10 region (pixels past
+        // 100 compute on pixel 0 and are dropped)
+        int pbase[2];
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            const int q = (wave + 4 * m) * 16 + (lane & 15);
+            pbase[m] = q < 100 ? (q / 10) * 12 + q % 10 : 0;
+        }
+        floatx4 acc[2][NBA];
+        conv_lds<NBA, 2, 12>(acc, xs, xstride, cin, wsa, kpada, pbase, lane);
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+            for (int nb = 0; nb < NBA; ++nb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int q = (wave + 4 * m) * 16 + 4 * (lane >> 4) + r;
+                    const int ch = nb * 16 + (lane & 15);
+                    if (q >= 100 || ch >= cya) continue;
+                    const int yy = ty0 - 1 + q / 10, xx = tx0 - 1 + q % 10;
+                    float v = acc[m][nb][r];
+                    if (relua) v = fmaxf(v, 0.f);
+                    ism[q * istride + ch] =
+                        (yy >= 0 && yy < H && xx >= 0 && xx < W) ? to_bf16(v) : (u16)0;
+                }
+    }
+    __syncthreads();
+    if (!wb_early) {
+        stage_weights(wsb, wb, NBB * 16, kpadb);
+        __syncthreads();
+    }
+    // conv b on the 8x8 tile: lane's A pixel tile row 2*wave + (l%16)/8,
+    // column (l%16)%8, i.e. region pixel (row + 1, column + 1) whose 3x3
+    // neighbourhood starts at region index row*10 + column
+    const int ar = 2 * wave + ((lane & 15) >> 3), ac = lane & 7;
+    const int pbase[1] = {ar * 10 + ac};
+    floatx4 acc[1][NBB];
+    conv_lds<NBB, 1, 10>(acc, ism, istride, cya, wsb, kpadb, pbase, lane);
+#pragma unroll
+    for (int nb = 0; nb < NBB; ++nb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 4 * (lane >> 4) + r;
+            const int yy = ty0 + 2 * wave + (row >> 3), xx = tx0 + (row & 7);
+            const int ch = nb * 16 + (lane & 15);
+            if (yy >= H || xx >= W || ch >= cyb) continue;
+            const long long p = ((long long)b * H + yy) * W + xx;
+            float v = acc[0][nb][r];
+            if (relub) v = fmaxf(v, 0.f);
+            ybf[p * cyb + ch] = to_bf16(v);
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -778,7 +902,9 @@ size_t tiled_lds_limit(cd_context* c) {
         bytes = 0;
     const void* fns[] = {reinterpret_cast<const void*>(&k_conv3x3_tile<1, 0>),
                          reinterpret_cast<const void*>(&k_conv3x3_tile<2, 0>),
-                         reinterpret_cast<const void*>(&k_conv3x3_tile<1, 1>)};
+                         reinterpret_cast<const void*>(&k_conv3x3_tile<1, 1>),
+                         reinterpret_cast<const void*>(&k_conv_pair_tile<1, 1>),
+                         reinterpret_cast<const void*>(&k_conv_pair_tile<2, 2>)};
     for (const void* f : fns)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) !=
             hipSuccess) {
@@ -1039,21 +1165,67 @@ int refresh_weights(cd_net* n) {
 // materialised; the output is bit-identical.  Training needs the materialised
 // tensors (weight gradients read them).  Pooling stays a separate pass: read
 // through the convolution (mode 2) it costs more than it saves.
+// Convolutions a and b (b reads only a's output, at a's resolution) in one
+// k_conv_pair_tile launch where the LDS allows; returns false (nothing
+// launched) where it does not apply.
+bool launch_conv_pair(cd_net* n, const NetConv& a, const NetConv& b, int B, int H, int W,
+                      int mode1, const u16* x1src, int* err) {
+    *err = CD_OK;
+    // up to 32 output channels: one workgroup computing all 64 channels of
+    // both convs of a 32x32 or 16x16 level (4-16 workgroups) measured 3.6-4.6 us
+    // slower than the two launches that split the channels over 4 workgroups
+    // per tile
+    const int nba = a.coutpad / 16, nbb = b.coutpad / 16;
+    if (nba != nbb || !(nba == 1 || nba == 2)) return false;
+    if (b.in1 != a.out || b.in2 >= 0 || b.out < 0 || a.out < 0 || b.c1 != a.coutA) return false;
+    if (mode1 == 2 && a.c2 > 0) return false;
+    const size_t fixed = 144 * (size_t)(a.c1 + a.c2 + 8) + 100 * (size_t)(a.coutA + 8);
+    const size_t wa = (size_t)nba * 16 * (a.kpad + 8), wb = (size_t)nbb * 16 * (b.kpad + 8);
+    const size_t limit = tiled_lds_limit(n->ctx);
+    int early = 1;
+    size_t lds = sizeof(u16) * (fixed + wa + wb);
+    if (lds > limit) {
+        early = 0;
+        lds = sizeof(u16) * (fixed + std::max(wa, wb));
+        if (lds > limit) return false;
+    }
+    const u16* x1 = mode1 ? x1src : (a.in1 < 0 ? n->x8.as<u16>() : n->t[a.in1].act.as<u16>());
+    const u16* x2 = a.in2 < 0 ? nullptr : n->t[a.in2].act.as<u16>();
+    const u16* wta = n->wbf.as<u16>() + a.woff;
+    const u16* wtb = n->wbf.as<u16>() + b.woff;
+    u16* y = n->t[b.out].act.as<u16>();
+    const dim3 grid(div_up(W, CT), div_up(H, CT), B);
+    hipStream_t s = n->ctx->stream;
+#define PAIR(NB) hipLaunchKernelGGL((k_conv_pair_tile<NB, NB>), grid, dim3(256), lds, s, x1, x2, a.c1, a.c2, wta, a.kpad, a.coutA, a.relu, wtb, b.kpad, b.coutA, b.relu, y, H, W, mode1, early)
+    if (nba == 1) PAIR(1);
+    else PAIR(2);
+#undef PAIR
+    return true;
+}
+
 int forward(cd_net* n, int B, int H, int W, bool fused = false) {
     cd_context* c = n->ctx;
     CD_TRY(refresh_weights(n));
     const int L = n->arch.levels;
     const u16* pooled_from = nullptr;  // fused: the next conv pools this tensor as it reads
-    for (const NetConv& cv : n->convs) {
-        const int Hl = H >> cv.level, Wl = W >> cv.level;
-        const int idx = (int)(&cv - &n->convs[0]);
-        if (fused && cv.in2 >= 0) {
-            CD_TRY(launch_conv(n, cv, B, Hl, Wl, false, 1, n->t[n->convs[idx - 1].out].act.as<u16>()));
-        } else if (pooled_from) {
-            CD_TRY(launch_conv(n, cv, B, Hl, Wl, false, 2, pooled_from));
-        } else {
-            CD_TRY(launch_conv(n, cv, B, Hl, Wl, false));
+    for (int idx = 0; idx < (int)n->convs.size(); ++idx) {
+        const NetConv& cv0 = n->convs[idx];
+        const int Hl = H >> cv0.level, Wl = W >> cv0.level;
+        const int mode1 = (fused && cv0.in2 >= 0) ? 1 : pooled_from ? 2 : 0;
+        const u16* x1src = mode1 == 1 ? n->t[n->convs[idx - 1].out].act.as<u16>()
+                                      : mode1 == 2 ? pooled_from : nullptr;
+        // fused: the first conv of a level's pair (encoder, or the decoder's
+        // concatenating one) and the next one in one launch where possible
+        bool paired = false;
+        if (fused && n->tiled_forward && idx + 1 < (int)n->convs.size() &&
+            (cv0.in2 >= 0 || (idx < 2 * L && idx % 2 == 0))) {
+            int err = CD_OK;
+            paired = launch_conv_pair(n, cv0, n->convs[idx + 1], B, Hl, Wl, mode1, x1src, &err);
+            CD_TRY(err);
+            if (paired) ++idx;  // the pair's second conv is done
         }
+        const NetConv& cv = n->convs[idx];
+        if (!paired) CD_TRY(launch_conv(n, cv, B, Hl, Wl, false, mode1, x1src));
         pooled_from = nullptr;
         // after the second conv of an encoder level below the bottleneck: pool
         if (idx < 2 * L && idx % 2 == 1 && cv.level < L - 1) {

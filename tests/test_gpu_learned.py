@@ -64,10 +64,14 @@ def test_fused_forward_is_bit_identical(ctx, linear):
                                                    ((16, 32, 64, 64), (3, 24, 56), 0)])
 def test_tiled_forward_is_bit_identical(ctx, channels, shape, linear):
     """Forward convolutions on LDS-staged 8x8 tiles (the default) give exactly
-    the output of the per-chunk gather kernel: the architectures of the
-    shipped weights (16/32/32/32 and 16/32/64/64, so 64+64-channel decoder
-    inputs and 64-channel outputs split over two workgroups), batches, and
-    image sides that leave partial tiles at the coarsest level."""
+    the output of the per-chunk gather kernel, and with the fused inference
+    forward (pairs of convolutions in one launch, conv b's weights staged with
+    conv a's or after it, and the 64+64-channel pair that does not fit the LDS
+    left as two launches) exactly that of the materialised tensors: the
+    architectures of the shipped weights (16/32/32/32 and 16/32/64/64, so
+    64+64-channel decoder inputs and 64-channel outputs split over two
+    workgroups), ReLU, batches, and image sides that leave partial tiles at the
+    coarsest level."""
     arch = capi.default_arch(linear=linear, channels=(*channels, 0, 0))
     net = capi.Net(ctx, arch)
     net.set_params(NetRef(4, channels, linear=bool(linear)).random_params(5))
