@@ -155,6 +155,18 @@ __device__ __forceinline__ unsigned brick_col(int j) { return (unsigned)((j >> 6
 __device__ __forceinline__ size_t brick_at(const Brick& bk, int x, int y) {
     return brick_row(bk, (x + y) & 1, y) + brick_col(x >> 1);
 }
+// XCD-contiguous workgroup order.  The dispatcher hands consecutive workgroups
+// (x fastest, then y, then z) round-robin to the 8 XCDs, each with its own L2;
+// this maps the dispatch index to an order in which XCD k takes the k-th
+// contiguous eighth of the n workgroups (a bijection on [0, n)), so that
+// neighbouring workgroups, whose halos overlap, share an L2.
+__device__ __forceinline__ int xcd_order() {
+    const int n = gridDim.x * gridDim.y * gridDim.z;
+    const int lin = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    const int q = n >> 3, r = n & 7, x = lin & 7;
+    return x * q + min(x, r) + (lin >> 3);
+}
+
 // Buffer access to a bricked plane pair of at most 2^31 bytes: scalar
 // resource + scalar (row) byte offset + the lane's vector byte offset, no
 // per-lane 64-bit address arithmetic.  Every access stays in range.
@@ -222,8 +234,9 @@ __device__ __forceinline__ void residual_rows(const double* __restrict__ phi,
                                               unsigned short* __restrict__ r16, const Brick& bk,
                                               int w, int h, double& rs, double& rq) {
     constexpr int G = ROWS < 4 ? ROWS : 4;  // rows whose loads are in flight together
-    const int x0 = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y0 = blockIdx.y * blockDim.y * ROWS + threadIdx.y;
+    const int ob = xcd_order();
+    const int x0 = (ob % gridDim.x) * blockDim.x + threadIdx.x;
+    const int y0 = (ob / gridDim.x) * blockDim.y * ROWS + threadIdx.y;
     const int x = min(x0, w - 1);
     const int xl = max(x - 1, 0), xr = min(x + 1, w - 1);
 #pragma unroll
@@ -292,7 +305,7 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ phi
     double a = cd_block_sum(rs, smem);
     double q = cd_block_sum(rq, smem);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
-        size_t bid = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+        const size_t bid = (size_t)xcd_order();
         partials[2 * bid] = a;
         partials[2 * bid + 1] = q;
     }
@@ -1207,9 +1220,12 @@ __device__ __forceinline__ float lap_packed(const float* __restrict__ z, const B
 
 // partials[4 * block + k]: p_old.Lz (= z.Lp_old: L is symmetric, so Lp_old
 // needs neither a stencil pass nor storage), z.r, p_old.r, z.Lz over the
-// packed elements of the rows the workgroup strides through (fixed order).
-// Rows go in groups of 4 whose loads are all issued before the first is used
-// (clamped, in-range addresses; elements outside the grid contribute nothing).
+// packed elements of the workgroup's band of consecutive rows of one colour
+// plane (fixed order).  Workgroups in XCD-contiguous order: both planes of a
+// band, whose stencils read each other, and the neighbouring bands share an
+// L2.  Rows go in groups of 4 whose loads are all issued before the first is
+// used (clamped, in-range addresses; elements outside the grid contribute
+// nothing).
 __global__ __launch_bounds__(256) void k_fcg_dots(const float* __restrict__ z,
                                                   const FcgDev* __restrict__ fd,
                                                   const unsigned short* __restrict__ r16,
@@ -1219,18 +1235,23 @@ __global__ __launch_bounds__(256) void k_fcg_dots(const float* __restrict__ z,
     __shared__ double smem[16];
     const float rm = st->rmean32;
     const float* __restrict__ p = fd->pprev;
-    const int c = blockIdx.z, o = 1 - c;
-    const int j0 = blockIdx.x * blockDim.x + threadIdx.x;
+    // block order (column block fastest, then plane, then band of rows)
+    const int ob = xcd_order();
+    const int bx = ob % gridDim.x, c = (ob / gridDim.x) & 1, band = ob / (2 * gridDim.x);
+    const int o = 1 - c;
+    const int rows = (h + gridDim.y - 1) / gridDim.y;  // rows per band
+    const int y0 = band * rows, y1 = min(h, y0 + rows);
+    const int j0 = bx * blockDim.x + threadIdx.x;
     const int j = min(j0, bk.wpb - 1);
     const unsigned cj = brick_col(j), cjm = brick_col(max(j - 1, 0)),
                    cjp = brick_col(min(j + 1, bk.wpb - 1));
     double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
-    for (int Yb = blockIdx.y; Yb < h; Yb += G * gridDim.y) {
+    for (int Yb = y0; Yb < y1; Yb += G) {
         float zc[G], zl[G], zr[G], zu[G], zd[G], pv[G];
         unsigned short rv[G];
 #pragma unroll
         for (int k = 0; k < G; ++k) {
-            const int Y = min(Yb + k * (int)gridDim.y, h - 1);
+            const int Y = min(Yb + k, h - 1);
             const int par = (c + Y) & 1;  // x = 2j + par
             const size_t ro = brick_row(bk, o, Y);
             const size_t i = brick_row(bk, c, Y) + cj;
@@ -1244,9 +1265,9 @@ __global__ __launch_bounds__(256) void k_fcg_dots(const float* __restrict__ z,
         }
 #pragma unroll
         for (int k = 0; k < G; ++k) {
-            const int Y = Yb + k * (int)gridDim.y;
+            const int Y = Yb + k;
             const int x = 2 * j0 + ((c + Y) & 1);
-            if (Y >= h || j0 >= bk.wpb || x >= w) continue;
+            if (Y >= y1 || j0 >= bk.wpb || x >= w) continue;
             float sn = 0.f, cnt = 0.f;
             if (x > 0) { sn += zl[k]; cnt += 1.f; }
             if (x < w - 1) { sn += zr[k]; cnt += 1.f; }
@@ -1265,11 +1286,10 @@ __global__ __launch_bounds__(256) void k_fcg_dots(const float* __restrict__ z,
     d2 = cd_block_sum(d2, smem);
     d3 = cd_block_sum(d3, smem);
     if (threadIdx.x == 0) {
-        const int bid = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-        partials[4 * bid] = d0;
-        partials[4 * bid + 1] = d1;
-        partials[4 * bid + 2] = d2;
-        partials[4 * bid + 3] = d3;
+        partials[4 * ob] = d0;
+        partials[4 * ob + 1] = d1;
+        partials[4 * ob + 2] = d2;
+        partials[4 * ob + 3] = d3;
     }
 }
 
@@ -1343,17 +1363,33 @@ __global__ __launch_bounds__(256) void k_residual_step(const double* __restrict_
     float cpart0 = 0.f, cpart1 = 0.f;  // this thread's sums per coarse row
     const double bmean = st->bmean;
     const float alpha = (float)fd->alpha, beta = (float)fd->beta;
-    const int x0 = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y0 = blockIdx.y * blockDim.y * ROWS + threadIdx.y;
+    const int ob = xcd_order();
+    const int bx = ob % gridDim.x, by = ob / gridDim.x;
+    const int x0 = bx * blockDim.x + threadIdx.x;
+    const int y0 = by * blockDim.y * ROWS + threadIdx.y;
     const int x = min(x0, w - 1);
     const int xl = max(x - 1, 0), xr = min(x + 1, w - 1);
     // bricked offsets (a plane pair holds < 2^29 elements in blocked mode): the
     // column parts of this cell and of its left / right neighbours are the same
     // on every row; their planes alternate with the row
     const unsigned cj = brick_col(x >> 1), cjl = brick_col(xl >> 1), cjr = brick_col(xr >> 1);
+    // (24-bit multiplies, full rate: row and brick indices are < 2^24)
+    const unsigned plane = (unsigned)bk.plane, brow = (unsigned)bk.nbx * 512u;
     auto rowo = [&](int c, int y) -> unsigned {
-        return (unsigned)c * (unsigned)bk.plane + ((unsigned)(y >> 3) * (unsigned)bk.nbx) * 512u +
-               (unsigned)(y & 7) * 64u;
+        return (c ? plane : 0u) + __umul24((unsigned)(y >> 3), brow) + (unsigned)(y & 7) * 64u;
+    };
+    // buffer resources: 32-bit byte offsets, and reads past either end of a
+    // buffer return 0 (the host passes grids of < 2^28 cells)
+    const size_t n = (size_t)w * h;
+    const __amdgpu_buffer_rsrc_t phi_rs = plane_rsrc(phi_in, n * sizeof(double));
+    const __amdgpu_buffer_rsrc_t b_rs = plane_rsrc(b, n * sizeof(double));
+    const __amdgpu_buffer_rsrc_t z_rs = plane_rsrc(z, 2 * bk.plane * sizeof(float));
+    const __amdgpu_buffer_rsrc_t p_rs = plane_rsrc(p_old, 2 * bk.plane * sizeof(float));
+    const __amdgpu_buffer_rsrc_t phio_rs = plane_rsrc(phi_out, n * sizeof(double));
+    const __amdgpu_buffer_rsrc_t pn_rs = plane_rsrc(p_new, 2 * bk.plane * sizeof(float));
+    const __amdgpu_buffer_rsrc_t r16_rs = plane_rsrc(r16, 2 * bk.plane * sizeof(unsigned short));
+    auto ld64 = [](__amdgpu_buffer_rsrc_t rs, unsigned off) {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
     };
     // new direction and iterate at (xx, yy) from the loaded phi_in, z, p_old
     auto pnew = [&](float zv, float pv) { return zv + beta * pv; };
@@ -1373,16 +1409,19 @@ __global__ __launch_bounds__(256) void k_residual_step(const double* __restrict_
             const unsigned ro = rowo(o, y);
             const unsigned bis[5] = {rowo(c, y) + cj, ro + cjl, rowo(o, yu) + cj, ro + cjr,
                                      rowo(o, yd) + cj};
-            const size_t ry = (size_t)y * w;
-            const size_t pis[5] = {ry + x, ry + xl, (size_t)yu * w + x, ry + xr,
-                                   (size_t)yd * w + x};
+            // byte offsets of the natural-layout cells: centre, left, up, right,
+            // down (the clamped ones at the border are valid cells, unused)
+            const unsigned co = (__umul24((unsigned)y, (unsigned)w) + (unsigned)x) * 8u;
+            const unsigned ws8 = (unsigned)w * 8u;
+            const unsigned pos[5] = {co, co - (unsigned)(x - xl) * 8u, co - (unsigned)(y - yu) * ws8,
+                                     co + (unsigned)(xr - x) * 8u, co + (unsigned)(yd - y) * ws8};
 #pragma unroll
             for (int t = 0; t < 5; ++t) {
-                ph[k][t] = phi_in[pis[t]];
-                zv[k][t] = z[bis[t]];
-                pv[k][t] = p_old[bis[t]];
+                ph[k][t] = ld64(phi_rs, pos[t]);
+                zv[k][t] = buf_load<float>(z_rs, bis[t] * 4u, 0);
+                pv[k][t] = buf_load<float>(p_rs, bis[t] * 4u, 0);
             }
-            bb[k] = b[ry + x];
+            bb[k] = ld64(b_rs, co);
         }
 #pragma unroll
         for (int k = 0; k < G; ++k) {
@@ -1409,20 +1448,20 @@ __global__ __launch_bounds__(256) void k_residual_step(const double* __restrict_
                 if ((int)threadIdx.y + 4 * (g + k) >= lf) cpart1 += (float)rv;
                 else cpart0 += (float)rv;
             }
-            const size_t i = (size_t)y * w + x;
             const unsigned bi = rowo((x + y) & 1, y) + cj;
-            phi_out[i] = nv[0];
-            p_new[bi] = pc;
-            r16[bi] = f32_to_bf16((float)rv);
+            __builtin_amdgcn_raw_buffer_store_b64(
+                __builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, nv[0]), phio_rs,
+                (__umul24((unsigned)y, (unsigned)w) + (unsigned)x) * 8u, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(pc), pn_rs, bi * 4u, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b16(f32_to_bf16((float)rv), r16_rs, bi * 2u, 0, 0);
         }
     }
     const double a = cd_block_sum(rs, smem);
     const double q = cd_block_sum(rq, smem);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
-        const size_t bid = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
-        partials[2 * bid] = a;
-        partials[2 * bid + 1] = q;
-        if (bid == 0) fd->pprev = p_new;
+        partials[2 * ob] = a;
+        partials[2 * ob + 1] = q;
+        if (ob == 0) fd->pprev = p_new;
     }
     if (x8) {  // uniform: every thread of the launch takes this branch or none
         const int ncr = (4 * ROWS) / lf > 1 ? 2 : 1;  // coarse rows of this workgroup
@@ -1437,8 +1476,8 @@ __global__ __launch_bounds__(256) void k_residual_step(const double* __restrict_
                        csum[cr][2][threadIdx.x]) + csum[cr][3][threadIdx.x];
             for (int o = 1; o < lf; o <<= 1) v += __shfl_xor(v, o, 64);
             if ((threadIdx.x & (lf - 1)) == 0) {
-                const int cx = (blockIdx.x * 64 + threadIdx.x) / lf;
-                const int cy = (blockIdx.y * 4 * ROWS) / lf + cr;
+                const int cx = (bx * 64 + threadIdx.x) / lf;
+                const int cy = (by * 4 * ROWS) / lf + cr;
                 *reinterpret_cast<uint4*>(x8 + ((size_t)cy * lwc + cx) * 8) =
                     make_uint4(f32_to_bf16(v / (float)(lf * lf)), 0, 0, 0);
             }
@@ -1716,8 +1755,10 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
         return f > 0 && w % f == 0 && h % f == 0 && (w / f) % (1 << (lv - 1)) == 0 &&
                (h / f) % (1 << (lv - 1)) == 0 && m > 0;
     };
+    // (the step check addresses the FP64 grids through 32-bit byte offsets)
     bool learned = o->learned_init && arch && blocked && o->stop == CD_STOP_RESIDUAL &&
-                   coarse_ok(arch->coarsen, arch->levels) && !work;
+                   coarse_ok(arch->coarsen, arch->levels) && !work &&
+                   (size_t)n * sizeof(double) < (1ull << 31);
     const int lf = arch ? arch->coarsen : 1;
     const int lwc = w / std::max(lf, 1), lhc = h / std::max(lf, 1);
     const bool capture = ctx->cap_x && ctx->cap_coarsen > 0 && w == ctx->cap_coarsen * ctx->cap_w &&
