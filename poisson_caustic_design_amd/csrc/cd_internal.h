@@ -193,6 +193,30 @@ __device__ __forceinline__ double cd_block_sum(double v, double* smem /* >= 16 *
     return t;
 }
 
+// N block sums at once (each in the order cd_block_sum uses, so the results
+// are the same): one pair of barriers instead of N
+template <int N>
+__device__ __forceinline__ void cd_block_sum_n(double (&v)[N], double* smem /* >= 16 * N */) {
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    const int nthr = blockDim.x * blockDim.y;
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = cd_wave_sum(v[k]);
+    if ((tid & (CD_WAVE - 1)) == 0)
+#pragma unroll
+        for (int k = 0; k < N; ++k) smem[16 * k + tid / CD_WAVE] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        double t = 0.0;
+        if (tid < CD_WAVE) {
+            t = (tid < nthr / CD_WAVE) ? smem[16 * k + tid] : 0.0;
+            t = cd_wave_sum(t);
+        }
+        v[k] = t;
+    }
+    __syncthreads();
+}
+
 // ---- host-side helpers shared across units ---------------------------------
 // Deterministic sum of `n` doubles at stride 1 (count values) into out[k] for
 // each of `ncomp` interleaved components: partials laid out [block][ncomp].

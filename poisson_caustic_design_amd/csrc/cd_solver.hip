@@ -295,15 +295,16 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ phi
                                                   float* __restrict__ r32,
                                                   unsigned short* __restrict__ r16, Brick bk,
                                                   int w, int h, double* __restrict__ partials) {
-    __shared__ double smem[16];
+    __shared__ double smem[32];
     const double bmean = st->bmean;
     double rs = 0.0, rq = 0.0;
     if (mask)
         residual_rows<OUT, ROWS, true>(phi, b, mask, bmean, r32, r16, bk, w, h, rs, rq);
     else
         residual_rows<OUT, ROWS, false>(phi, b, mask, bmean, r32, r16, bk, w, h, rs, rq);
-    double a = cd_block_sum(rs, smem);
-    double q = cd_block_sum(rq, smem);
+    double aq[2] = {rs, rq};
+    cd_block_sum_n(aq, smem);
+    const double a = aq[0], q = aq[1];
     if (threadIdx.x == 0 && threadIdx.y == 0) {
         const size_t bid = (size_t)xcd_order();
         partials[2 * bid] = a;
@@ -333,15 +334,17 @@ __global__ void k_check(const double* __restrict__ partials, int nblocks, SolveD
     constexpr int kWords = (int)(sizeof(SolveDev) / 4);
     constexpr int kSeq = (int)(offsetof(SolveDev, seq) / 4);
     static_assert(sizeof(SolveDev) % 4 == 0 && kWords <= 1024, "status words");
-    __shared__ double smem[16];
+    __shared__ double smem[32];
     __shared__ unsigned words[kWords];
     double a = 0.0, q = 0.0;
     for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
         a += partials[2 * b];
         q += partials[2 * b + 1];
     }
-    a = cd_block_sum(a, smem);
-    q = cd_block_sum(q, smem);
+    double aq[2] = {a, q};
+    cd_block_sum_n(aq, smem);
+    a = aq[0];
+    q = aq[1];
     if (threadIdx.x == 0) {
         double mean = count > 0 ? a / count : 0.0;
         double rn2 = q - count * mean * mean;
@@ -1232,7 +1235,7 @@ __global__ __launch_bounds__(256) void k_fcg_dots(const float* __restrict__ z,
                                                   const SolveDev* __restrict__ st, Brick bk,
                                                   int w, int h, double* __restrict__ partials) {
     constexpr int G = 4;
-    __shared__ double smem[16];
+    __shared__ double smem[64];
     const float rm = st->rmean32;
     const float* __restrict__ p = fd->pprev;
     // block order (column block fastest, then plane, then band of rows)
@@ -1281,10 +1284,12 @@ __global__ __launch_bounds__(256) void k_fcg_dots(const float* __restrict__ z,
             d3 += (double)(zc[k] * lz);
         }
     }
-    d0 = cd_block_sum(d0, smem);
-    d1 = cd_block_sum(d1, smem);
-    d2 = cd_block_sum(d2, smem);
-    d3 = cd_block_sum(d3, smem);
+    double dd[4] = {d0, d1, d2, d3};
+    cd_block_sum_n(dd, smem);
+    d0 = dd[0];
+    d1 = dd[1];
+    d2 = dd[2];
+    d3 = dd[3];
     if (threadIdx.x == 0) {
         partials[4 * ob] = d0;
         partials[4 * ob + 1] = d1;
@@ -1295,11 +1300,11 @@ __global__ __launch_bounds__(256) void k_fcg_dots(const float* __restrict__ z,
 
 // single block: step coefficients from the dot products (fixed summation order)
 __global__ void k_fcg_coef(const double* __restrict__ partials, int nblocks, FcgDev* fd) {
-    __shared__ double smem[16];
+    __shared__ double smem[64];
     double d[4] = {0.0, 0.0, 0.0, 0.0};
     for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
         for (int k = 0; k < 4; ++k) d[k] += partials[4 * b + k];
-    for (int k = 0; k < 4; ++k) d[k] = cd_block_sum(d[k], smem);
+    cd_block_sum_n(d, smem);
     if (threadIdx.x == 0) {
         const double zq = d[0], zr = d[1], pr = d[2], zlz = d[3];  // zq = z.Lp_old
         double beta = 0.0;
@@ -1358,7 +1363,7 @@ __global__ __launch_bounds__(256) void k_residual_step(const double* __restrict_
                                                        unsigned short* __restrict__ x8, int lf,
                                                        int lwc) {
     constexpr int G = ROWS < CD_STEP_G ? ROWS : CD_STEP_G;  // rows whose loads are in flight together
-    __shared__ double smem[16];
+    __shared__ double smem[32];
     __shared__ float csum[2][4][64];  // (coarse row, thread row, column) partial block sums
     float cpart0 = 0.f, cpart1 = 0.f;  // this thread's sums per coarse row
     const double bmean = st->bmean;
@@ -1456,8 +1461,9 @@ __global__ __launch_bounds__(256) void k_residual_step(const double* __restrict_
             __builtin_amdgcn_raw_buffer_store_b16(f32_to_bf16((float)rv), r16_rs, bi * 2u, 0, 0);
         }
     }
-    const double a = cd_block_sum(rs, smem);
-    const double q = cd_block_sum(rq, smem);
+    double aq[2] = {rs, rq};
+    cd_block_sum_n(aq, smem);
+    const double a = aq[0], q = aq[1];
     if (threadIdx.x == 0 && threadIdx.y == 0) {
         partials[2 * ob] = a;
         partials[2 * ob + 1] = q;
