@@ -677,38 +677,24 @@ __global__ __launch_bounds__(256) void k_smooth_step(const float* __restrict__ i
 #define CD_SMOOTH_TILE 8
 #endif
 constexpr int kSmoothTile = CD_SMOOTH_TILE;
-constexpr int kSmoothMaxK = 12;  // halo <= 12: 2 (16 + 2k)^2 floats of LDS
-// Each thread keeps the region coordinates of its (at most kSmoothCells)
-// cells for every step, and issues the loads of all of them before the first
-// LDS store.
+constexpr int kSmoothMaxK = 12;  // halo <= 12: 2 (tile + 2k)^2 floats of LDS
+// cells of a region per thread, at most
 constexpr int kSmoothCells =
     ((kSmoothTile + 2 * kSmoothMaxK) * (kSmoothTile + 2 * kSmoothMaxK) + 255) / 256;
-__global__ __launch_bounds__(256) void k_smooth_tiled(const float* __restrict__ in, int istride,
-                                                      float* __restrict__ out, int B, int H,
-                                                      int W, int k) {
-    extern __shared__ float lds[];
-    const int R = kSmoothTile + 2 * k, RR = R * R;
-    float* a = lds;
-    float* b = lds + RR;
-    const int bz = blockIdx.z;
-    const int X0 = blockIdx.x * kSmoothTile - k, Y0 = blockIdx.y * kSmoothTile - k;
-    const long long base = (long long)bz * H * W;
+// The k smoothing steps on an R x R region (R = kSmoothTile + 2k, image origin
+// (X0, Y0)) held in LDS at a (b: scratch of the same size); returns the buffer
+// holding the result.  Each thread keeps the region coordinates of its (at
+// most kSmoothCells) cells for every step.
+__device__ __forceinline__ float* smooth_lds(float* a, float* b, int R, int k, int X0, int Y0,
+                                             int H, int W) {
+    const int RR = R * R;
     int rxs[kSmoothCells], rys[kSmoothCells];
-    float v[kSmoothCells];
 #pragma unroll
     for (int u = 0; u < kSmoothCells; ++u) {
         const int idx = threadIdx.x + 256 * u;
         rys[u] = idx / R;
         rxs[u] = idx - rys[u] * R;
-        const int gx = X0 + rxs[u], gy = Y0 + rys[u];
-        v[u] = (idx < RR && gx >= 0 && gx < W && gy >= 0 && gy < H)
-                   ? in[(base + (long long)gy * W + gx) * istride]
-                   : 0.f;
     }
-#pragma unroll
-    for (int u = 0; u < kSmoothCells; ++u)
-        if (threadIdx.x + 256 * u < RR) a[threadIdx.x + 256 * u] = v[u];
-    __syncthreads();
     for (int st = 1; st <= k; ++st) {
 #pragma unroll
         for (int u = 0; u < kSmoothCells; ++u) {
@@ -734,11 +720,103 @@ __global__ __launch_bounds__(256) void k_smooth_tiled(const float* __restrict__ 
         a = b;
         b = t;
     }
+    return a;
+}
+
+// the tile's interior of the smoothed region to out [B][H][W]
+__device__ __forceinline__ void smooth_store(const float* a, int R, int k, int X0, int Y0,
+                                             int H, int W, long long base,
+                                             float* __restrict__ out) {
     for (int idx = threadIdx.x; idx < kSmoothTile * kSmoothTile; idx += blockDim.x) {
         const int ty = idx / kSmoothTile, tx = idx - ty * kSmoothTile;
         const int gx = X0 + k + tx, gy = Y0 + k + ty;
         if (gx < W && gy < H) out[base + (long long)gy * W + gx] = a[(ty + k) * R + tx + k];
     }
+}
+
+// k smoothing steps of a field read with element stride istride (8: head
+// channel 0), one tile plus its k-cell halo per workgroup; the loads of all of
+// a thread's cells are issued before the first LDS store
+__global__ __launch_bounds__(256) void k_smooth_tiled(const float* __restrict__ in, int istride,
+                                                      float* __restrict__ out, int B, int H,
+                                                      int W, int k) {
+    extern __shared__ float lds[];
+    const int R = kSmoothTile + 2 * k, RR = R * R;
+    float* a = lds;
+    float* b = lds + RR;
+    const int bz = blockIdx.z;
+    const int X0 = blockIdx.x * kSmoothTile - k, Y0 = blockIdx.y * kSmoothTile - k;
+    const long long base = (long long)bz * H * W;
+    float v[kSmoothCells];
+#pragma unroll
+    for (int u = 0; u < kSmoothCells; ++u) {
+        const int idx = threadIdx.x + 256 * u;
+        const int ry = idx / R, rx = idx - ry * R;
+        const int gx = X0 + rx, gy = Y0 + ry;
+        v[u] = (idx < RR && gx >= 0 && gx < W && gy >= 0 && gy < H)
+                   ? in[(base + (long long)gy * W + gx) * istride]
+                   : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < kSmoothCells; ++u)
+        if (threadIdx.x + 256 * u < RR) a[threadIdx.x + 256 * u] = v[u];
+    __syncthreads();
+    a = smooth_lds(a, b, R, k, X0, Y0, H, W);
+    smooth_store(a, R, k, X0, Y0, H, W, base, out);
+}
+
+// Head convolution and output smoothing in one launch (inference, K steps):
+// the head (16 output channels of which channel 0 is the output, no ReLU) on
+// the (8 + 2K)^2 region around an 8x8 tile, read from a (10 + 2K)^2 input
+// patch in LDS with the A/B fragments and K order of k_conv3x3_tile, so its
+// values are those k_conv3x3_tile<1, 1> stores; outside the image the region
+// holds 0, as k_smooth_tiled loads it; then the K steps as k_smooth_tiled
+// takes them.  The output is bit-identical to the two launches.  grid
+// (ceil(W/8), ceil(H/8), B); LDS (bytes): 2 P^2 (C + 8) + 2*16 (kpad + 8)
+// + 8 R^2, P = R + 2, R = 8 + 2K.
+template <int K>
+__global__ __launch_bounds__(256) void k_head_smooth(const u16* __restrict__ x, int C,
+                                                     const u16* __restrict__ wt, int kpad, int H,
+                                                     int W, float* __restrict__ out) {
+    constexpr int R = kSmoothTile + 2 * K, P = R + 2, NQ = R * R;
+    constexpr int MT = (NQ + 63) / 64;  // M-tiles of 16 region pixels per wave
+    static_assert(kSmoothTile == CT, "the head region is centred on the conv tile");
+    extern __shared__ uint4 tile_lds[];
+    const int xstride = C + 8;
+    u16* xs = reinterpret_cast<u16*>(tile_lds);
+    u16* ws = xs + P * P * xstride;
+    float* a = reinterpret_cast<float*>(ws + 16 * (kpad + 8));
+    float* b2 = a + NQ;
+    const int b = blockIdx.z;
+    const int tx0 = blockIdx.x * CT, ty0 = blockIdx.y * CT;
+    const int X0 = tx0 - K, Y0 = ty0 - K;  // region origin
+    // (x2 = x with no channels: never read; a null x2 here crashed the compiler)
+    stage_patch<P>(xs, xstride, x, x, C, 0, H, W, b, Y0 - 1, X0 - 1, 0);
+    stage_weights(ws, wt, 16, kpad);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int pbase[MT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+        const int q = (wave + 4 * m) * 16 + (lane & 15);
+        pbase[m] = q < NQ ? (q / R) * P + q % R : 0;
+    }
+    floatx4 acc[MT][1];
+    conv_lds<1, MT, P>(acc, xs, xstride, C, ws, kpad, pbase, lane);
+    if ((lane & 15) == 0) {  // output channel 0
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int q = (wave + 4 * m) * 16 + 4 * (lane >> 4) + r;
+                if (q >= NQ) continue;
+                const int gy = Y0 + q / R, gx = X0 + q % R;
+                a[q] = (gy >= 0 && gy < H && gx >= 0 && gx < W) ? acc[m][0][r] : 0.f;
+            }
+    }
+    __syncthreads();
+    const float* res = smooth_lds(a, b2, R, K, X0, Y0, H, W);
+    smooth_store(res, R, K, X0, Y0, H, W, (long long)b * H * W, out);
 }
 
 // operator residual u = cs * Lc y - x
@@ -892,6 +970,7 @@ struct cd_net {
     bool fused_forward = true;  // inference forwards fold the decoder upsampling and the
                                 // encoder pooling into the convs
     bool tiled_forward = true;  // forward convolutions on LDS-staged 8x8 tiles (bit-identical)
+    bool head_pending = false;  // fused forward left the head to finish_output (k_head_smooth)
 };
 
 namespace {
@@ -910,7 +989,8 @@ size_t tiled_lds_limit(cd_context* c) {
                          reinterpret_cast<const void*>(&k_conv3x3_tile<2, 0>),
                          reinterpret_cast<const void*>(&k_conv3x3_tile<1, 1>),
                          reinterpret_cast<const void*>(&k_conv_pair_tile<1, 1>),
-                         reinterpret_cast<const void*>(&k_conv_pair_tile<2, 2>)};
+                         reinterpret_cast<const void*>(&k_conv_pair_tile<2, 2>),
+                         reinterpret_cast<const void*>(&k_head_smooth<6>)};
     for (const void* f : fns)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) !=
             hipSuccess) {
@@ -1209,9 +1289,22 @@ bool launch_conv_pair(cd_net* n, const NetConv& a, const NetConv& b, int B, int 
     return true;
 }
 
+// the head + output smoothing kernel applies: 6 smoothing steps (the kernel's
+// instantiation), one 16-channel block, no skip input, and the LDS fits
+constexpr int kHeadSmoothK = 6;
+bool head_smooth_ok(cd_net* n, const NetConv& hd) {
+    if (n->arch.out_smooth != kHeadSmoothK || hd.coutpad != 16 || hd.in2 >= 0 || hd.in1 < 0)
+        return false;
+    constexpr int R = kSmoothTile + 2 * kHeadSmoothK, P = R + 2;
+    const size_t lds = sizeof(u16) * ((size_t)P * P * (hd.c1 + 8) + 16 * (size_t)(hd.kpad + 8)) +
+                       sizeof(float) * 2 * R * R;
+    return lds <= tiled_lds_limit(n->ctx);
+}
+
 int forward(cd_net* n, int B, int H, int W, bool fused = false) {
     cd_context* c = n->ctx;
     CD_TRY(refresh_weights(n));
+    n->head_pending = false;
     const int L = n->arch.levels;
     const u16* pooled_from = nullptr;  // fused: the next conv pools this tensor as it reads
     for (int idx = 0; idx < (int)n->convs.size(); ++idx) {
@@ -1231,6 +1324,12 @@ int forward(cd_net* n, int B, int H, int W, bool fused = false) {
             if (paired) ++idx;  // the pair's second conv is done
         }
         const NetConv& cv = n->convs[idx];
+        // fused: the head and the output smoothing run as one launch in
+        // finish_output
+        if (!paired && fused && n->tiled_forward && cv.out < 0 && head_smooth_ok(n, cv)) {
+            n->head_pending = true;
+            continue;
+        }
         if (!paired) CD_TRY(launch_conv(n, cv, B, Hl, Wl, false, mode1, x1src));
         pooled_from = nullptr;
         // after the second conv of an encoder level below the bottleneck: pool
@@ -1327,6 +1426,19 @@ int finish_output(cd_net* n, int B, int H, int W, float* dst = nullptr) {
     const long long px = (long long)B * H * W;
     float* y = dst ? dst : n->yout.as<float>();
     const int k = n->arch.out_smooth;
+    if (n->head_pending) {  // the fused forward's head with the smoothing
+        n->head_pending = false;
+        const NetConv& hd = n->convs.back();
+        constexpr int R = kSmoothTile + 2 * kHeadSmoothK, P = R + 2;
+        const size_t lds = sizeof(u16) * ((size_t)P * P * (hd.c1 + 8) + 16 * (size_t)(hd.kpad + 8)) +
+                           sizeof(float) * 2 * R * R;
+        const dim3 grid(div_up(W, CT), div_up(H, CT), B);
+        hipLaunchKernelGGL(k_head_smooth<kHeadSmoothK>, grid, dim3(256), lds, n->ctx->stream,
+                           n->t[hd.in1].act.as<u16>(), hd.c1, n->wbf.as<u16>() + hd.woff, hd.kpad,
+                           H, W, y);
+        CD_CHECK_LAUNCH(n->ctx);
+        return CD_OK;
+    }
     if (k > 0 && k <= kSmoothMaxK) {
         // head channel 0 -> k steps -> y in one launch
         const int R = kSmoothTile + 2 * k;

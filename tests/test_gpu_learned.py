@@ -38,12 +38,16 @@ def test_net_forward_matches_netref(ctx, linear):
     assert np.abs(y - yr).max() <= 2e-2 * np.abs(yr).max()
 
 
-@pytest.mark.parametrize("linear", [1, 0])
-def test_fused_forward_is_bit_identical(ctx, linear):
+@pytest.mark.parametrize("linear,out_smooth", [(1, 0), (0, 0), (1, 6), (0, 6)])
+def test_fused_forward_is_bit_identical(ctx, linear, out_smooth):
     """Upsampled decoder inputs and pooled encoder inputs read through the
-    convolutions (inference) give exactly the output of the materialised
-    tensors (training)."""
-    arch = capi.default_arch(linear=linear, channels=(16, 32, 32, 32, 0, 0))
+    convolutions, the two convolutions of a level in one launch, and (with 6
+    output smoothing steps, the shipped setting) the head convolution and the
+    smoothing in one launch (inference) give exactly the output of the
+    materialised tensors and separate launches (training), on a batch of two
+    32 x 40 fields (partial tiles at every level)."""
+    arch = capi.default_arch(linear=linear, channels=(16, 32, 32, 32, 0, 0),
+                             out_smooth=out_smooth)
     net = capi.Net(ctx, arch)
     net.set_params(NetRef(4, (16, 32, 32, 32), linear=bool(linear)).random_params(3))
     x = np.random.default_rng(2).standard_normal((2, 32, 40))
