@@ -1229,6 +1229,9 @@ __device__ __forceinline__ float lap_packed(const float* __restrict__ z, const B
     return s - cnt * zc;
 }
 
+#ifndef CD_DOT_BANDS
+#define CD_DOT_BANDS 256
+#endif
 // partials[4 * block + k]: p_old.Lz (= z.Lp_old: L is symmetric, so Lp_old
 // needs neither a stencil pass nor storage), z.r, p_old.r, z.Lz over the
 // packed elements of the workgroup's band of consecutive rows of one colour
@@ -1882,9 +1885,10 @@ int cd_solve_device_internal(cd_context* ctx, const double* b, double* phi, cons
     // dot products stride over rows so that their partials fit nred blocks
     const int pbx = div_up(bk.wpb, 256);
     const dim3 packed_grid(pbx, h, 2);
-    // (at most 256 row groups: each thread accumulates several rows before
-    // the block reduction; 1024 measured slower at 1024^2)
-    const int dot_rows = std::max(1, std::min(std::min(h, 256), nred / (2 * pbx)));
+    // (at most CD_DOT_BANDS bands of rows: each thread accumulates several rows
+    // before the block reduction; 1024 measured slower at 1024^2, 128 and 512
+    // no faster at 2048^2)
+    const int dot_rows = std::max(1, std::min(std::min(h, CD_DOT_BANDS), nred / (2 * pbx)));
     const dim3 dot_grid(pbx, dot_rows, 2);
     const int dot_blocks = 2 * pbx * dot_rows;
     if (dot_blocks > nred) learned = false;  // tiny grids: partials hold nred blocks
