@@ -72,7 +72,7 @@ def test_tiled_forward_is_bit_identical(ctx, channels, shape, linear):
     forward (pairs of convolutions in one launch, conv b's weights staged with
     conv a's or after it, and the 64+64-channel pair that does not fit the LDS
     left as two launches) exactly that of the materialised tensors: the
-    architectures of the shipped weights (16/32/32/32 and 16/32/64/64, so
+    architectures 16/32/32/32 (the shipped weights) and 16/32/64/64 (so
     64+64-channel decoder inputs and 64-channel outputs split over two
     workgroups), ReLU, batches, and image sides that leave partial tiles at the
     coarsest level."""
