@@ -6,8 +6,10 @@ builds the same layer list in the same order, uses the same flat parameter
 layout ([coutpad][kpad] per convolution, k = tap * Cin + ci, tap = 3*dy + dx)
 and computes forward, loss and gradients in float64.  `emulate_bf16` rounds
 activations and weights to BF16 the way the GPU stores them, for tight
-forward comparisons.  Its gradients are pinned by central finite differences
-(tests/test_netref.py).
+forward comparisons.  Its gradients are pinned by central finite differences,
+its layers and their wiring by closed forms (impulse responses, SciPy's
+cross-correlation, adjoint pairs, identity weights that reduce the network to
+block means or to the identity) in tests/test_netref.py.
 """
 from __future__ import annotations
 

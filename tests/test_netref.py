@@ -1,5 +1,6 @@
 """Pins the FP64 network restatement (oracle/netref.py) by central finite
-differences; it is then the checker of the GPU network kernels."""
+differences and by closed forms of its layers and wiring; it is then the
+checker of the GPU network kernels."""
 import numpy as np
 import pytest
 
@@ -132,3 +133,85 @@ def test_linear_netref_is_linear():
     y2, _ = net.forward(p, x2)
     y3, _ = net.forward(p, 2.0 * x1 - 0.5 * x2)
     assert np.allclose(y3, 2.0 * y1 - 0.5 * y2, atol=1e-12)
+
+
+# ---- closed-form checks of the layers and of the layer wiring -------------
+
+def _set_tap(p, cv, co, ci, tap, val):
+    """Write one weight of the flat [coutpad][kpad] layout, k = tap*Cin + ci."""
+    p[cv.woff + co * cv.kpad + tap * (cv.c1 + cv.c2) + ci] = val
+
+
+def test_conv_is_a_zero_padded_cross_correlation():
+    from scipy.signal import correlate2d
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((1, 7, 9, 3))
+    wt = rng.standard_normal((9, 3, 2))
+    y = NetRef.conv(x, wt)
+    for co in range(2):
+        want = sum(correlate2d(x[0, :, :, ci], wt[:, ci, co].reshape(3, 3), mode="same",
+                               boundary="fill") for ci in range(3))
+        assert np.allclose(y[0, :, :, co], want, atol=1e-12)
+    # an impulse reproduces the taps: tap (dy, dx) lands at offset (1 - dy, 1 - dx)
+    d = np.zeros((1, 5, 5, 3))
+    d[0, 2, 2, 1] = 1.0
+    y = NetRef.conv(d, wt)
+    for tap in range(9):
+        dy, dx = divmod(tap, 3)
+        assert np.array_equal(y[0, 3 - dy, 3 - dx], wt[tap, 1])
+
+
+def test_pool_and_upsampling_closed_forms():
+    c = np.full((2, 6, 8, 3), -1.75)
+    assert np.array_equal(NetRef.pool(c), c[:, :3, :4])  # constants survive the pool
+    chk = np.indices((6, 8)).sum(axis=0) % 2 * 2.0 - 1.0
+    assert np.all(NetRef.pool(chk[None, :, :, None]) == 0)  # a checkerboard averages out
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal((1, 4, 6, 2))
+    assert np.array_equal(NetRef.pool(NetRef.up(x)), x)
+    assert np.array_equal(NetRef.up(x)[0, 3, 5], x[0, 1, 2])  # nearest neighbour
+    # the backward passes are the adjoints
+    X, G = rng.standard_normal((1, 8, 12, 2)), rng.standard_normal((1, 4, 6, 2))
+    assert np.isclose(np.sum(NetRef.pool(X) * G), np.sum(X * NetRef.pool_bwd(G)))
+    assert np.isclose(np.sum(NetRef.up(G) * X), np.sum(G * NetRef.up_bwd(X)))
+
+
+def _identity_net(levels, through_skip):
+    """Linear network whose every convolution copies channel 0 (centre tap) and
+    whose decoder convolutions take either the upsampled path or the skip."""
+    net = NetRef(levels=levels, channels=(8, 16, 16, 16)[:levels], linear=True)
+    p = np.zeros(net.nparam)
+    for cv in net.convs:
+        ci = cv.c1 if (cv.c2 > 0 and through_skip) else 0
+        _set_tap(p, cv, 0, ci, 4, 1.0)
+    return net, p
+
+
+def test_identity_wiring_reproduces_block_means_and_skips():
+    x = np.random.default_rng(7).standard_normal((2, 16, 16))
+    for levels in (2, 3):
+        net, p = _identity_net(levels, through_skip=False)
+        y, _ = net.forward(p, x)
+        f = 2 ** (levels - 1)  # the deepest level sees f x f block means
+        means = x.reshape(2, 16 // f, f, 16 // f, f).mean(axis=(2, 4))
+        assert np.allclose(y, np.repeat(np.repeat(means, f, axis=1), f, axis=2), atol=1e-13)
+        net, p = _identity_net(levels, through_skip=True)
+        y, _ = net.forward(p, x)
+        assert np.array_equal(y, x)  # the first skip carries the input unchanged
+
+
+def test_head_stencil_and_relu_closed_forms():
+    from scipy.ndimage import correlate
+    x = np.random.default_rng(8).standard_normal((1, 10, 12))
+    net = NetRef(levels=1, channels=(8,), linear=True)
+    p = np.zeros(net.nparam)
+    _set_tap(p, net.convs[0], 0, 0, 4, 1.0)
+    _set_tap(p, net.convs[1], 0, 0, 4, 1.0)
+    st = np.array([[0, 1, 0], [1, -4, 1], [0, 1, 0]], float)
+    for tap in range(9):
+        _set_tap(p, net.convs[2], 0, 0, tap, st.flat[tap])
+    y, _ = net.forward(p, x)
+    assert np.allclose(y[0], correlate(x[0], st, mode="constant"), atol=1e-13)
+    relu_net = NetRef(levels=1, channels=(8,))
+    y, _ = relu_net.forward(p, x)  # same weights, ReLU after the two inner layers
+    assert np.allclose(y[0], correlate(np.maximum(x[0], 0), st, mode="constant"), atol=1e-13)
