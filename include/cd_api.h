@@ -186,6 +186,22 @@ int cd_design_count_folds(cd_design* d, long long* folded_triangles);
  * caustic_design.cpp:215-218 (in the reference an exit(0) at mesh.cpp:279
  * precedes it).  Default off: the step fails with CD_ERR_RASTER_MISS. */
 int cd_design_set_miss_recovery(cd_design* d, int on);
+/* Which triangle a raster sample or grid vertex takes where several triangles
+ * hold it (overlaps of a folded mesh; shared edges).  mode 0 (default): the
+ * lowest triangle index.  mode 1: the triangle the reference's BVH query
+ * returns (Bvh::query, bvh.cpp:196-266, over the tree of Bvh::build,
+ * bvh.cpp:20-155, built with (5, 30) at mesh.cpp:235,292,349), i.e. the
+ * holding triangle of lowest position in cd_bvh_first_hit_order's array.
+ * Mode 1 builds that order on the host at every rasterization of the target
+ * mesh and when the source owners are built (at cd_design_init, or here when
+ * the design is initialised): a copy of the mesh to the host each time.
+ * Returns CD_ERR_BAD_ARG for other modes. */
+int cd_design_set_first_hit(cd_design* d, int mode);
+/* order[k] = the triangle at position k of the array that the reference's
+ * Bvh::build (bvh.cpp:20-155; leaves of at most 5 triangles, depth 30)
+ * partitions, for the mx x my mesh with vertex coordinates xy (x,y pairs,
+ * triangulation of mesh.cpp:57-63, 2(mx-1)(my-1) triangles).  Host code. */
+int cd_bvh_first_hit_order(const double* xy, int mx, int my, int* order);
 
 typedef struct {
     int status;

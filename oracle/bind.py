@@ -314,6 +314,7 @@ class RefLib:
         L.ref_set_target_points.argtypes = [_P, _P]
         L.ref_set_source_z.argtypes = [_P, _P]
         L.ref_get_grid.argtypes = [_P, _I, _P]
+        L.ref_bvh_query.argtypes = [_P, _I, _I, _P, _I, _P]
         L.ref_set_grid.argtypes = [_P, _I, _P]
         L.ref_get_vector.argtypes = [_P, _I, _P]
         L.ref_inverted_map.argtypes = [_P, _P]
@@ -321,6 +322,13 @@ class RefLib:
         L.ref_save_solid_obj.argtypes = [C.c_char_p, _P, _P, _I, _I, _D, _D, _D]
         L.ref_export_grid_svg.argtypes = [C.c_char_p, _P, _I, _I, _I, _D, _D, _D]
         L.ref_export_triangles_svg.argtypes = [C.c_char_p, _P, _I, _I, _I, _D, _D, _D]
+
+    def bvh_query(self, xy, mx, my, pts):
+        """Triangle the reference's BVH returns for each point (-1: none)."""
+        xy, pts = _f64(np.asarray(xy)[:, :2]), _f64(pts)
+        face = np.empty(len(pts), dtype=np.int32)
+        self.L.ref_bvh_query(_ptr(xy), mx, my, _ptr(pts), len(pts), face.ctypes.data)
+        return face
 
     @staticmethod
     def available(path: str = REF_SO) -> bool:

@@ -29,6 +29,7 @@
 #include <cstring>
 #include <vector>
 
+#include "bvh.h"
 #include "caustic_design.h"
 
 // Defined (non-static) in src/caustic_design.cpp but not declared in a header.
@@ -73,6 +74,30 @@ void from_grid(const std::vector<std::vector<double>>& g, double* a) {
 }  // namespace
 
 extern "C" {
+
+// Builds the reference's Bvh (bvh.cpp) over an mx x my mesh with vertices xy
+// (x,y pairs; triangulation of mesh.cpp:57-63, as Mesh builds it) with the
+// parameters of mesh.cpp:235, and queries n points; face[k] = -1 on a miss.
+void ref_bvh_query(const double* xy, int mx, int my, const double* pts, int n, int* face) {
+    std::vector<std::vector<double>> points((size_t)mx * my);
+    for (size_t v = 0; v < points.size(); ++v) points[v] = {xy[2 * v], xy[2 * v + 1], 0.0};
+    std::vector<std::vector<int>> tris;
+    for (int i = 0; i < my - 1; ++i)
+        for (int j = 0; j < mx - 1; ++j) {
+            const int idx = i * mx + j;
+            tris.push_back({idx, idx + 1, idx + mx});
+            tris.push_back({idx + mx, idx + 1, idx + mx + 1});
+        }
+    Bvh bvh(tris, points);
+    bvh.build(5, 30);
+    for (int k = 0; k < n; ++k) {
+        Hit hit;
+        bool found = false;
+        bvh.query({pts[2 * k], pts[2 * k + 1]}, hit, found);
+        face[k] = found ? hit.face_id : -1;
+    }
+}
+
 
 int ref_poisson_solver(const double* D, double* phi, int w, int h, int max_iterations,
                        double threshold, int max_threads) {

@@ -24,7 +24,7 @@ BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "libcaustic_amd.so")
 
 HIP_SOURCES = ["cd_context.hip", "cd_solver.hip", "cd_design.hip", "cd_learned.hip"]
-CXX_SOURCES = ["dropin_solver.cpp", "dropin_caustic_design.cpp", "cd_io.cpp"]
+CXX_SOURCES = ["dropin_solver.cpp", "dropin_caustic_design.cpp", "cd_io.cpp", "cd_bvh_order.cpp"]
 
 ARCH = os.environ.get("CD_OFFLOAD_ARCH", "gfx950")
 

@@ -136,6 +136,8 @@ def load(path: str = LIB_PATH):
         "cd_design_set_miss_recovery": ([_P, _I], _I),
         "cd_design_set_fold_check": ([_P, _I], _I),
         "cd_design_count_folds": ([_P, C.POINTER(_LL)], _I),
+        "cd_design_set_first_hit": ([_P, _I], _I),
+        "cd_bvh_first_hit_order": ([_P, _I, _I, _P], _I),
         "cd_png_read_rgba8": ([C.c_char_p, C.POINTER(_I), C.POINTER(_I), _P, _SZ], _I),
         "cd_image_gray_resize": ([_P, _P, _I, _I, _I, _I, _P], _I),
         "cd_image_gray_resize_device": ([_P, _P, _I, _I, _I, _I, _P], _I),
@@ -193,7 +195,7 @@ def exported_symbols():
         "cd_design_laplacian_smoothing", "cd_design_set_miss_recovery",
         "cd_designs_run_batch", "cd_net_set_fused_forward", "cd_net_set_tiled_forward",
         "cd_design_set_fold_check",
-        "cd_design_count_folds",
+        "cd_design_count_folds", "cd_design_set_first_hit", "cd_bvh_first_hit_order",
     ]
 
 
@@ -360,6 +362,7 @@ class CausticDesign:
         self.phi_opts: Optional[SolveOpts] = None
         self.h_opts: Optional[SolveOpts] = None
         self.step_size: Optional[float] = None  # None: the reference's (double)0.05f
+        self.first_hit = 0
         self.d = None
 
     # setters of caustic_design.cpp:28-53
@@ -401,6 +404,8 @@ class CausticDesign:
         self.ctx.check(self.L.cd_design_create(self.ctx.h, C.byref(p), C.byref(h)),
                        "cd_design_create")
         self.d = h
+        if self.first_hit:
+            self.ctx.check(self.L.cd_design_set_first_hit(self.d, self.first_hit), "set_first_hit")
         self.ctx.children.add(self)
 
     def close(self):
@@ -466,6 +471,16 @@ class CausticDesign:
 
     def set_miss_recovery(self, on: bool):
         self.ctx.check(self.L.cd_design_set_miss_recovery(self.d, int(on)), "set_miss_recovery")
+
+    def set_first_hit(self, mode: int):
+        """0: a sample held by several triangles takes the lowest index; 1: the
+        one the reference's BVH query returns (bvh.cpp:196-266).  Kept for the
+        design that initialize_solvers creates, applied at once to an existing one."""
+        if int(mode) not in (0, 1):
+            raise CDError(BAD_ARG, f"set_first_hit: mode {mode} is not 0 or 1")
+        self.first_hit = int(mode)
+        if self.d is not None:
+            self.ctx.check(self.L.cd_design_set_first_hit(self.d, self.first_hit), "set_first_hit")
 
     # file outputs (caustic_design.cpp:20-26,55-61; main.cpp:269)
     def save_solid_obj_source(self, filename: str):
@@ -565,6 +580,19 @@ def run_designs_batch(designs, d_images=None, max_transport_steps=50,
     for d, r in zip(designs, stats):
         d.ctx.check(r.status, "cd_designs_run_batch", allow)
     return list(stats)
+
+
+def bvh_first_hit_order(xy, mx: int, my: int) -> np.ndarray:
+    """Host: the triangle at each position of the reference BVH's build array
+    (bvh.cpp:20-155) for mesh vertices xy [mx*my, 2]."""
+    L = load()
+    xy = np.ascontiguousarray(np.asarray(xy, dtype=np.float64)[:, :2])
+    assert xy.shape == (mx * my, 2)
+    out = np.empty(2 * (mx - 1) * (my - 1), dtype=np.int32)
+    rc = L.cd_bvh_first_hit_order(xy.ctypes.data, mx, my, out.ctypes.data)
+    if rc != OK:
+        raise CDError(rc, f"cd_bvh_first_hit_order: {status_string(rc)}")
+    return out
 
 
 def png_read_rgba8(path: str) -> np.ndarray:

@@ -313,7 +313,7 @@ __device__ __forceinline__ void tri_rows(const G& g, const Tri2& T, int& i0, int
 }
 
 template <typename G>
-__device__ __forceinline__ void scan_row(const G& g, const Tri2& T, int t, int i,
+__device__ __forceinline__ void scan_row(const G& g, const Tri2& T, int key, int i,
                                          int* __restrict__ owner) {
     const double py = g.py(i);
     if (G::kSkipRows && (py < T.ymin - 1e-9 || py > T.ymax + 1e-9)) return;
@@ -324,18 +324,22 @@ __device__ __forceinline__ void scan_row(const G& g, const Tri2& T, int t, int i
     for (int j = j0; j <= j1; ++j) {
         double u, v, w;
         barycentric(T.ax, T.ay, T.bx, T.by, T.cx, T.cy, g.px(j), py, u, v, w);
-        if (inside(u, v)) atomicMin(&owner[(size_t)i * g.nx + j], t);
+        if (inside(u, v)) atomicMin(&owner[(size_t)i * g.nx + j], key);
     }
 }
 
 // one thread per triangle; triangles spanning more than kBigRows candidate
-// rows go to a list that one workgroup per triangle scans row-parallel
+// rows go to a list that one workgroup per triangle scans row-parallel.  A
+// sample keeps the lowest key of the triangles that hold it: the triangle
+// index, or with `rank` the triangle's position in the reference's BVH
+// first-hit order (the gathers map it back through that order).
 constexpr int kBigRows = 16;
 constexpr int kBigBlocks = 1024;  // workgroups striding over the big-triangle list
 
 template <typename G>
 __global__ __launch_bounds__(256) void k_scatter_small(const double* __restrict__ xy, int mx,
                                                        int my, G g, int* __restrict__ owner,
+                                                       const int* __restrict__ rank,
                                                        int* __restrict__ big, int* __restrict__ nbig) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 2 * (mx - 1) * (my - 1)) return;
@@ -347,12 +351,14 @@ __global__ __launch_bounds__(256) void k_scatter_small(const double* __restrict_
         big[atomicAdd(nbig, 1)] = t;
         return;
     }
-    for (int i = i0; i <= i1; ++i) scan_row(g, T, t, i, owner);
+    const int key = rank ? rank[t] : t;
+    for (int i = i0; i <= i1; ++i) scan_row(g, T, key, i, owner);
 }
 
 template <typename G>
 __global__ __launch_bounds__(256) void k_scatter_big(const double* __restrict__ xy, int mx, G g,
                                                      int* __restrict__ owner,
+                                                     const int* __restrict__ rank,
                                                      const int* __restrict__ big,
                                                      const int* __restrict__ nbig) {
     const int n = *nbig;
@@ -362,7 +368,8 @@ __global__ __launch_bounds__(256) void k_scatter_big(const double* __restrict__ 
         if (!load_tri(xy, mx, t, T)) continue;
         int i0, i1;
         tri_rows(g, T, i0, i1);
-        for (int i = i0 + (int)threadIdx.x; i <= i1; i += blockDim.x) scan_row(g, T, t, i, owner);
+        const int key = rank ? rank[t] : t;
+        for (int i = i0 + (int)threadIdx.x; i <= i1; i += blockDim.x) scan_row(g, T, key, i, owner);
     }
 }
 
@@ -373,6 +380,7 @@ __global__ __launch_bounds__(256) void k_raster_gather(const double* __restrict_
                                                        const double* __restrict__ vals1,
                                                        int nx, int ny, double W, double H,
                                                        const int* __restrict__ owner,
+                                                       const int* __restrict__ order,
                                                        double* __restrict__ out0,
                                                        double* __restrict__ out1,
                                                        double* __restrict__ partials) {
@@ -381,12 +389,13 @@ __global__ __launch_bounds__(256) void k_raster_gather(const double* __restrict_
     double sum = 0.0, miss = 0.0;
     if (s < (long long)nx * ny) {
         const int i = (int)(s / nx), j = (int)(s - (long long)i * nx);
-        const int t = owner[s];
+        int t = owner[s];
         if (t == INT_MAX) {
             out0[s] = NAN;
             if (out1) out1[s] = NAN;
             miss = 1.0;
         } else {
+            if (order) t = order[t];
             const double eps = 1e-8;
             const double py = (double)i * (H - eps) / (ny - 1) + 0.5 * eps;
             const double px = (double)j * (W - eps) / (nx - 1) + 0.5 * eps;
@@ -499,6 +508,7 @@ __global__ __launch_bounds__(256) void k_fold_count(const double* __restrict__ x
 __global__ __launch_bounds__(256) void k_inverse_normals(const double* __restrict__ xy, int mx,
                                                          int my, double W, double H,
                                                          const int* __restrict__ owner,
+                                                         const int* __restrict__ order,
                                                          const double* __restrict__ sz,
                                                          double focal, double ior,
                                                          double* __restrict__ inv,
@@ -510,13 +520,14 @@ __global__ __launch_bounds__(256) void k_inverse_normals(const double* __restric
     double miss = 0.0;
     if (i < mx * my) {
         const int y = i / mx, x = i - y * mx;
-        const int t = owner[i];
+        int t = owner[i];
         const double sx = (double)x * W / (mx - 1), sy = (double)y * H / (my - 1);
         if (t == INT_MAX) {
             miss = 1.0;
             inv[2 * i] = NAN; inv[2 * i + 1] = NAN;
             nrm_x[i] = NAN; nrm_y[i] = NAN;
         } else {
+            if (order) t = order[t];
             const double eps = 1e-8;
             const double px = eps + sx * ((W - 2 * eps) / W);
             const double py = eps + sy * ((H - 2 * eps) / H);
@@ -726,6 +737,10 @@ struct cd_design {
     long long folded_cells = 0;      // dual cells of area <= 0 at the last transport step's start
     long long folded_triangles = 0;  // inverted triangles after the last transport step
     bool fold_fail = false;          // a transport step that folds the mesh fails (CD_ERR_FOLDED)
+    int first_hit = 0;               // 1: samples take the reference BVH's first hit
+    DevBuf bvh_rank, order_t, order_s;   // first-hit position per triangle; orders of both meshes
+    std::vector<double> h_xy;            // host copies for the order's build
+    std::vector<int> h_order, h_rank;
 };
 
 namespace {
@@ -795,10 +810,44 @@ int read_scalars(cd_design* d, double* out, int n) {
 }
 
 // rasterize per-vertex values (one or two fields) over triangulation xy
+// first-hit order of mesh xy (reference BVH, bvh.cpp:20-155) into `order`,
+// positions into d->bvh_rank; both null in the default lowest-index mode
+int first_hit_order(cd_design* d, const double* xy, DevBuf& order, const int** rank_out,
+                    const int** order_out) {
+    *rank_out = nullptr;
+    *order_out = nullptr;
+    if (!d->first_hit) return CD_OK;
+    cd_context* c = d->ctx;
+    const size_t V = d->V, T = d->T;
+    CD_TRY(c->ensure(order, 4 * T));
+    CD_TRY(c->ensure(d->bvh_rank, 4 * T));
+    d->h_xy.resize(2 * V);
+    d->h_order.resize(T);
+    d->h_rank.resize(T);
+    CD_HIP(c, hipMemcpyAsync(d->h_xy.data(), xy, 16 * V, hipMemcpyDeviceToHost, c->stream));
+    CD_HIP(c, hipStreamSynchronize(c->stream));
+    CD_TRY(cd_bvh_first_hit_order(d->h_xy.data(), d->mx, d->my, d->h_order.data()));
+    for (size_t k = 0; k < T; ++k) d->h_rank[d->h_order[k]] = (int)k;
+    CD_HIP(c, hipMemcpyAsync(order.ptr, d->h_order.data(), 4 * T, hipMemcpyHostToDevice,
+                             c->stream));
+    CD_HIP(c, hipMemcpyAsync(d->bvh_rank.ptr, d->h_rank.data(), 4 * T, hipMemcpyHostToDevice,
+                             c->stream));
+    CD_HIP(c, hipStreamSynchronize(c->stream));  // the host vectors are reused
+    *rank_out = d->bvh_rank.as<int>();
+    *order_out = order.as<int>();
+    return CD_OK;
+}
+
 int raster(cd_design* d, const double* xy, int* owner, bool rebuild_owner, const double* v0,
            const double* v1, double* out0, double* out1, double* misses_out) {
     cd_context* c = d->ctx;
     hipStream_t s = c->stream;
+    // owners of the target mesh are rebuilt here; those of the source mesh
+    // were built at initialisation, in the order kept in order_s
+    const int* rank = nullptr;
+    const int* order = nullptr;
+    if (rebuild_owner) CD_TRY(first_hit_order(d, xy, d->order_t, &rank, &order));
+    else if (d->first_hit) order = d->order_s.as<int>();
     const int tb = div_up(d->T, CD_BLOCK);
     const int gb = div_up((long long)d->G, CD_BLOCK);
     CD_TRY(c->timer_begin());
@@ -807,12 +856,12 @@ int raster(cd_design* d, const double* xy, int* owner, bool rebuild_owner, const
         CD_TRY(cd_fill_i32(c, d->nbig.as<int>(), 1, 0));
         const RasterGrid g{d->nx, d->ny, d->p.width, d->p.height, 1e-8};
         hipLaunchKernelGGL(k_scatter_small<RasterGrid>, dim3(tb), dim3(CD_BLOCK), 0, s, xy, d->mx,
-                           d->my, g, owner, d->big.as<int>(), d->nbig.as<int>());
+                           d->my, g, owner, rank, d->big.as<int>(), d->nbig.as<int>());
         hipLaunchKernelGGL(k_scatter_big<RasterGrid>, dim3(kBigBlocks), dim3(CD_BLOCK), 0, s, xy,
-                           d->mx, g, owner, d->big.as<int>(), d->nbig.as<int>());
+                           d->mx, g, owner, rank, d->big.as<int>(), d->nbig.as<int>());
     }
     hipLaunchKernelGGL(k_raster_gather, dim3(gb), dim3(CD_BLOCK), 0, s, xy, d->mx, v0, v1, d->nx,
-                       d->ny, d->p.width, d->p.height, owner, out0, out1,
+                       d->ny, d->p.width, d->p.height, owner, order, out0, out1,
                        d->partials.as<double>());
     CD_CHECK_LAUNCH(c);
     CD_TRY(finish(d, gb, 2, R_SUM, R_SUM, R_SUM, 0));  // scalars[0]=sum, [1]=misses
@@ -823,6 +872,27 @@ int raster(cd_design* d, const double* xy, int* owner, bool rebuild_owner, const
         CD_TRY(read_scalars(d, sc, 2));
         *misses_out = sc[1];
     }
+    return CD_OK;
+}
+
+// owners of the static source triangulation (mesh.cpp:291-345 locates
+// samples in the undeformed source mesh, whose xy never change)
+int source_owners(cd_design* d) {
+    cd_context* c = d->ctx;
+    hipStream_t s = c->stream;
+    const int* rank = nullptr;
+    const int* order = nullptr;
+    CD_TRY(first_hit_order(d, d->sxy.as<double>(), d->order_s, &rank, &order));
+    CD_TRY(cd_fill_i32(c, d->owner_src.as<int>(), d->G, INT_MAX));
+    CD_TRY(cd_fill_i32(c, d->nbig.as<int>(), 1, 0));
+    const RasterGrid g{d->nx, d->ny, d->p.width, d->p.height, 1e-8};
+    hipLaunchKernelGGL(k_scatter_small<RasterGrid>, dim3(div_up(d->T, CD_BLOCK)), dim3(CD_BLOCK),
+                       0, s, d->sxy.as<double>(), d->mx, d->my, g, d->owner_src.as<int>(), rank,
+                       d->big.as<int>(), d->nbig.as<int>());
+    hipLaunchKernelGGL(k_scatter_big<RasterGrid>, dim3(kBigBlocks), dim3(CD_BLOCK), 0, s,
+                       d->sxy.as<double>(), d->mx, g, d->owner_src.as<int>(), rank,
+                       d->big.as<int>(), d->nbig.as<int>());
+    CD_CHECK_LAUNCH(c);
     return CD_OK;
 }
 
@@ -866,18 +936,7 @@ int design_init_from_device_image(cd_design* d, const double* img) {
     // phi = h = 0 (caustic_design.cpp:354-363)
     CD_TRY(cd_fill_f64(c, d->phi.as<double>(), d->G, 0.0));
     CD_TRY(cd_fill_f64(c, d->h.as<double>(), d->G, 0.0));
-    // owners of the static source triangulation (mesh.cpp:291-345 locates
-    // samples in the undeformed source mesh, whose xy never change)
-    CD_TRY(cd_fill_i32(c, d->owner_src.as<int>(), d->G, INT_MAX));
-    CD_TRY(cd_fill_i32(c, d->nbig.as<int>(), 1, 0));
-    const RasterGrid g{d->nx, d->ny, d->p.width, d->p.height, 1e-8};
-    hipLaunchKernelGGL(k_scatter_small<RasterGrid>, dim3(div_up(d->T, CD_BLOCK)), dim3(CD_BLOCK),
-                       0, s, d->sxy.as<double>(), d->mx, d->my, g, d->owner_src.as<int>(),
-                       d->big.as<int>(), d->nbig.as<int>());
-    hipLaunchKernelGGL(k_scatter_big<RasterGrid>, dim3(kBigBlocks), dim3(CD_BLOCK), 0, s,
-                       d->sxy.as<double>(), d->mx, g, d->owner_src.as<int>(), d->big.as<int>(),
-                       d->nbig.as<int>());
-    CD_CHECK_LAUNCH(c);
+    CD_TRY(source_owners(d));
     d->recoveries = 0;
     d->initialized = true;
     d->folded_cells = 0;
@@ -1018,6 +1077,18 @@ int cd_design_count_folds(cd_design* d, long long* folded_triangles) {
     return CD_OK;
 }
 
+int cd_design_set_first_hit(cd_design* d, int mode) {
+    if (!d || (mode != 0 && mode != 1)) return CD_ERR_BAD_ARG;
+    if (!d->ctx) return CD_ERR_STATE;
+    d->first_hit = mode;
+    if (d->initialized) {  // the source owners were built in the previous mode
+        CD_HIP(d->ctx, hipSetDevice(d->ctx->device));
+        CD_TRY(source_owners(d));
+        CD_HIP(d->ctx, hipStreamSynchronize(d->ctx->stream));
+    }
+    return CD_OK;
+}
+
 int cd_design_set_miss_recovery(cd_design* d, int on) {
     if (!d) return CD_ERR_BAD_ARG;
     if (!d->ctx) return CD_ERR_STATE;
@@ -1099,18 +1170,21 @@ int cd_design_height_step(cd_design* d, double* update_out) {
     const int vb = div_up(V, CD_BLOCK);
     // inverse transport map + refractive normals (caustic_design.cpp:271, mesh.cpp:348-409, 677-722)
     const double focal = d->nx / d->p.width * d->p.focal_l;
+    const int* rank = nullptr;
+    const int* order = nullptr;
+    CD_TRY(first_hit_order(d, d->txy.as<double>(), d->order_t, &rank, &order));
     CD_TRY(c->timer_begin());
     CD_TRY(cd_fill_i32(c, d->owner_v.as<int>(), d->V, INT_MAX));
     CD_TRY(cd_fill_i32(c, d->nbig.as<int>(), 1, 0));
     const InverseGrid ig{d->mx, d->my, d->p.width, d->p.height, 1e-8};
     hipLaunchKernelGGL(k_scatter_small<InverseGrid>, dim3(div_up(d->T, CD_BLOCK)), dim3(CD_BLOCK),
-                       0, s, d->txy.as<double>(), d->mx, d->my, ig, d->owner_v.as<int>(),
+                       0, s, d->txy.as<double>(), d->mx, d->my, ig, d->owner_v.as<int>(), rank,
                        d->big.as<int>(), d->nbig.as<int>());
     hipLaunchKernelGGL(k_scatter_big<InverseGrid>, dim3(kBigBlocks), dim3(CD_BLOCK), 0, s,
-                       d->txy.as<double>(), d->mx, ig, d->owner_v.as<int>(), d->big.as<int>(),
+                       d->txy.as<double>(), d->mx, ig, d->owner_v.as<int>(), rank, d->big.as<int>(),
                        d->nbig.as<int>());
     hipLaunchKernelGGL(k_inverse_normals, dim3(vb), dim3(CD_BLOCK), 0, s, d->txy.as<double>(),
-                       d->mx, d->my, d->p.width, d->p.height, d->owner_v.as<int>(),
+                       d->mx, d->my, d->p.width, d->p.height, d->owner_v.as<int>(), order,
                        d->sz.as<double>(), focal, d->p.ior, d->inv.as<double>(),
                        d->nrm_x.as<double>(), d->nrm_y.as<double>(), d->partials.as<double>());
     CD_CHECK_LAUNCH(c);

@@ -174,6 +174,9 @@ int main(int argc, char** argv) {
     p.thickness = lens_thickness;
     cd_design* d = nullptr;
     if ((st = cd_design_create(ctx, &p, &d)) != CD_OK) return fail(ctx, st, "design");
+    // overlapping triangles of a folded mesh: the reference BVH's first hit
+    // (bvh.cpp:196-266), as the reference's own flow takes them
+    if ((st = cd_design_set_first_hit(d, 1)) != CD_OK) return fail(ctx, st, "first hit");
     if ((st = cd_design_init(d, image.data())) != CD_OK) return fail(ctx, st, "initialize_solvers");
 
     if (f.has_progress &&

@@ -79,6 +79,9 @@ void Caustic_design::initialize_solvers(std::vector<std::vector<double>> image) 
         p.h_solve.omega = 0.0;
     }
     check(ctx_, cd_design_create(ctx_, &p, &design_), "cd_design_create");
+    // samples in overlapping triangles of a folded mesh take the triangle the
+    // reference's BVH query returns (bvh.cpp:196-266), as in Mesh
+    check(ctx_, cd_design_set_first_hit(design_, 1), "cd_design_set_first_hit");
     if ((int)image.size() < resolution_y) throw std::invalid_argument("image rows < resolution_y");
     std::vector<double> img((size_t)resolution_x * resolution_y);
     for (int y = 0; y < resolution_y; ++y) {

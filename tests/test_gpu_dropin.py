@@ -28,25 +28,39 @@ def test_dropin_program_matches_c_abi(ctx, tmp_path):
     # the same design through the C ABI with the reference's default solves
     N = 4 * M
     img = np.fromfile(tmp_path / "image.f64", dtype=np.float64).reshape(N, N)
-    cd = capi.CausticDesign(ctx)
-    cd.set_mesh_resolution(M, M)
-    cd.set_domain_resolution(N, N)
-    cd.set_mesh_size(1.0, 1.0)
-    cd.set_lens_focal_length(1.5)
-    cd.set_lens_thickness(0.2)
-    cd.initialize_solvers(img)
-    steps = 0
-    for _ in range(50):
-        steps += 1
-        if cd.perform_transport_iteration() < float(np.float32(0.01)):
-            break
-    for i in range(3):
-        cd.perform_height_map_iteration(i)
+    def design(first_hit):
+        cd = capi.CausticDesign(ctx)
+        cd.set_mesh_resolution(M, M)
+        cd.set_domain_resolution(N, N)
+        cd.set_mesh_size(1.0, 1.0)
+        cd.set_lens_focal_length(1.5)
+        cd.set_lens_thickness(0.2)
+        cd.set_first_hit(first_hit)  # the drop-in takes the reference BVH's first hit
+        cd.initialize_solvers(img)
+        steps = folds = 0
+        for _ in range(50):
+            steps += 1
+            step = cd.perform_transport_iteration()
+            folds = max(folds, cd.count_folds())
+            if step < float(np.float32(0.01)):
+                break
+        for i in range(3):
+            cd.perform_height_map_iteration(i)
+        return cd, steps, folds
+
+    cd, steps, _ = design(1)
     assert steps == int(vals["steps"])
-    assert float(vals["h_abs_sum"]) == pytest.approx(np.abs(cd.get("h")).sum(), rel=1e-14)
+    h1 = np.abs(cd.get("h")).sum()
+    assert float(vals["h_abs_sum"]) == pytest.approx(h1, rel=1e-14)
     cd.save_solid_obj_source(str(tmp_path / "api.obj"))
     assert (tmp_path / "output.obj").read_bytes() == (tmp_path / "api.obj").read_bytes()
     cd.close()
+    # the lowest-index mode ends elsewhere only if the mesh folded on the way
+    cd0, _, folds0 = design(0)
+    h0 = np.abs(cd0.get("h")).sum()
+    print(f"first hit vs lowest index: h sums {h1!r} {h0!r}, folded triangles {folds0}")
+    assert folds0 > 0 or h0 == pytest.approx(h1, rel=1e-10)
+    cd0.close()
 
 
 def test_dropin_members_match_reference(ctx, tmp_path, reflib):

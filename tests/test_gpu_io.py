@@ -90,6 +90,7 @@ def test_cli_matches_library_calls(ctx, tmp_path):
     cd.set_mesh_size(1.0, mh)
     cd.set_lens_focal_length(1.5)
     cd.set_lens_thickness(0.2)
+    cd.set_first_hit(1)  # as the CLI: the reference BVH's first hit
     cd.initialize_solvers(capi.image_gray_resize(ctx, rgba, gw, gh))
     conv = float(np.float32(0.02))
     steps = 0
